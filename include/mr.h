@@ -1,0 +1,243 @@
+/*
+ * mr.h -- C-ABI of the MI355X (gfx950) SpMV locality pipeline: load a CSR,
+ * reorder it (pGOrder and the CPU baselines), predict the locality headroom
+ * from reuse distances, and multiply (MERBIT merge-path SpMV, CSR scalar and
+ * vector SpMV, SpMM).
+ *
+ * The reference has no library ABI; its only operator interface is the C++
+ * member set of GRAPH<V> (include/class.cuh:59-99) plus the free functions of
+ * src/class.cu and include/merbit.cuh.  Every entry point below names the
+ * reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *  - Every function returns an int status (MR_OK = 0); mr_last_error() gives a
+ *    thread-local message for the last failure on the calling thread.
+ *  - Index arrays are int32.  Host pointers are plain C arrays owned by the
+ *    caller; "device" pointers come from mr_malloc or any allocator of the same
+ *    HIP runtime.
+ *  - A matrix handle owns its device copy of the CSR (row_ptr, col_idx, val)
+ *    and the MERBIT metadata.  The reorder entry points relabel the handle in
+ *    place, as the reference's GRAPH::gorder* / relabel_* do, and accumulate the
+ *    permutation (perm[old] = new) that mr_mat_perm returns.
+ *  - SpMV multiplies the standard CSR A: y[i] = sum_k val[k] * x[col_idx[k]] for
+ *    k in [row_ptr[i], row_ptr[i+1]).  The reference's multiplied matrix is the
+ *    transpose of its MatrixMarket COO (class.cu:180,192); pass that transpose as
+ *    the CSR to reproduce it (mr_mtx_load does so).
+ *  - Value storage dtypes: F64, F32, F16, BF16.  F64 matrices take F64 x/y; all
+ *    other matrices take F32 x/y and accumulate in F32.
+ */
+#ifndef MR_H
+#define MR_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define MR_OK                  0
+#define MR_ERR_INVALID_ARG     1   /* bad argument / size                        */
+#define MR_ERR_HIP             2   /* a HIP runtime call failed                  */
+#define MR_ERR_OUT_OF_MEMORY   3   /* device or host allocation failed           */
+#define MR_ERR_NOT_PERMUTATION 4   /* array is not a permutation of [0, n)       */
+#define MR_ERR_UNSUPPORTED     5   /* combination not supported                  */
+#define MR_ERR_MALFORMED_CSR   6   /* row_ptr not monotone / columns out of range */
+#define MR_ERR_IO              7   /* file could not be read / parsed            */
+#define MR_ERR_ADMISSION       8   /* serving: request refused (memory budget)   */
+#define MR_ERR_NO_DEVICE       9   /* no HIP device visible                      */
+#define MR_ERR_NUMERIC        10   /* non-finite input or result                 */
+#define MR_ERR_TIMEOUT        11   /* serving: request timed out                 */
+#define MR_ERR_CANCELLED      12   /* serving: request cancelled                 */
+
+typedef enum { MR_F64 = 0, MR_F32 = 1, MR_F16 = 2, MR_BF16 = 3 } mr_dtype;
+typedef enum { MR_KERNEL_MERBIT = 0, MR_KERNEL_CSR_SCALAR = 1, MR_KERNEL_CSR_VECTOR = 2 } mr_kernel;
+typedef enum { MR_H2D = 0, MR_D2H = 1, MR_D2D = 2 } mr_copy_kind;
+
+typedef struct mr_ctx_s* mr_ctx;
+typedef struct mr_mat_s* mr_mat;
+typedef void* mr_stream;   /* hipStream_t; NULL = the context's stream         */
+typedef void* mr_event;    /* hipEvent_t                                        */
+
+/* ---- errors, version, context -------------------------------------------- */
+/* replaces the print-and-continue CHECK_CUDA/CHECK_CUSPARSE macros
+ * (include/inc.cuh:101-117) with a status + thread-local message */
+const char* mr_last_error(void);
+const char* mr_version(void);
+int mr_device_count(int* count);
+/* replaces ENV (include/class.cuh:13-36); the device id is honoured */
+int mr_ctx_create(int device, mr_ctx* out);
+int mr_ctx_destroy(mr_ctx ctx);
+int mr_ctx_stream(mr_ctx ctx, mr_stream* out);
+int mr_ctx_mem_info(mr_ctx ctx, size_t* free_bytes, size_t* total_bytes);
+int mr_ctx_arch(mr_ctx ctx, char* name, int cap, int* cu_count);
+
+/* ---- device memory, streams, events (plumbing for callers without a HIP
+ * runtime of their own; no reference counterpart) -------------------------- */
+int mr_malloc(mr_ctx ctx, size_t bytes, void** dptr);
+int mr_free(mr_ctx ctx, void* dptr);
+int mr_memcpy(mr_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
+int mr_memset(mr_ctx ctx, void* dptr, int value, size_t bytes);
+int mr_stream_create(mr_ctx ctx, mr_stream* out);
+int mr_stream_destroy(mr_ctx ctx, mr_stream s);
+int mr_stream_sync(mr_ctx ctx, mr_stream s);
+int mr_event_create(mr_ctx ctx, mr_event* out);
+int mr_event_destroy(mr_ctx ctx, mr_event e);
+int mr_event_record(mr_ctx ctx, mr_event e, mr_stream s);
+int mr_event_elapsed_ms(mr_ctx ctx, mr_event start, mr_event stop, float* ms);
+
+/* ---- host-side input generators (configurations of the benchmark; the
+ * reference reads MatrixMarket files only) ---------------------------------- */
+/* R-MAT edge list, 2^scale vertices, edge_factor * 2^scale edges, counter-based
+ * SplitMix64 stream per edge; scramble != 0 applies a seeded bijection to the
+ * vertex ids. src/dst must hold edge_factor << scale entries. */
+int mr_gen_rmat_edges(int scale, int edge_factor, double a, double b, double c,
+                      uint64_t seed, int scramble, int32_t* src, int32_t* dst);
+/* COO -> CSR: rows sorted, columns sorted within rows, duplicates merged when
+ * dedup != 0 (self loops kept). col_idx must hold m entries; *nnz_out = nnz. */
+int mr_coo_to_csr(int64_t n_rows, int64_t m, const int32_t* src, const int32_t* dst,
+                  int dedup, int32_t* row_ptr, int32_t* col_idx, int64_t* nnz_out);
+/* values: kind 0 = uniform[lo,hi]; kind 1 = log-uniform magnitude in [lo,hi]
+ * with a random sign. Counter-based: out[i] depends only on (seed, i). */
+int mr_gen_values(uint64_t seed, int kind, double lo, double hi, int64_t count, double* out);
+/* 27-point stencil on an nx*ny*nz grid: row_ptr[n+1], col_idx[nnz] (query the
+ * sizes with NULL arrays: *n_out, *nnz_out) */
+int mr_gen_stencil27(int nx, int ny, int nz, int32_t* row_ptr, int32_t* col_idx,
+                     int64_t* n_out, int64_t* nnz_out);
+/* random banded matrix: each row draws nnz_per_row distinct columns in
+ * [i-bw, i+bw] (diagonal always present). Query with NULL arrays. */
+int mr_gen_banded(int64_t n, int bandwidth, int nnz_per_row, uint64_t seed,
+                  int32_t* row_ptr, int32_t* col_idx, int64_t* nnz_out);
+
+/* ---- matrices ------------------------------------------------------------- */
+/* replaces GRAPH ctors (class.cu:84-216, 330-421): copies a host CSR in.
+ * val is in val_dtype (NULL = all ones, as the reference's loader forces),
+ * stored as store_dtype. Validates monotone row_ptr and column range. */
+int mr_csr_create(mr_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                  const int32_t* row_ptr, const int32_t* col_idx, const void* val,
+                  int val_dtype, int store_dtype, mr_mat* out);
+/* MatrixMarket loader (BASEGRAPH ctor, class.cu:5-73): header "symmetric" doubles
+ * every entry (diagonal included), "basezero" keeps 0-based ids, values forced
+ * to 1 when force_ones != 0. The stored CSR is the reference's multiplied
+ * matrix (rows over the file's column index). */
+int mr_mtx_load(mr_ctx ctx, const char* path, int store_dtype, int force_ones, mr_mat* out);
+int mr_mat_destroy(mr_mat m);
+int mr_mat_info(mr_mat m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz, int* dtype);
+/* copy the (current, possibly relabelled) CSR back to the host; val in the
+ * storage dtype; any pointer may be NULL */
+int mr_mat_export(mr_mat m, int32_t* row_ptr, int32_t* col_idx, void* val);
+/* new handle holding the same matrix with values re-stored in dtype */
+int mr_mat_convert(mr_mat m, int dtype, mr_mat* out);
+/* accumulated relabelling perm[old] = new (identity before any reorder),
+ * GRAPH::perm (class.cuh:78) */
+int mr_mat_perm(mr_mat m, int32_t* perm);
+
+/* ---- permutation ---------------------------------------------------------- */
+/* check_perm (include/utils.cuh:5-55), host array; MR_OK or MR_ERR_NOT_PERMUTATION */
+int mr_check_perm(const int32_t* perm, int64_t n);
+/* symmetric relabel A' = P A P^T, columns sorted within rows; perm[old] = new,
+ * host array.  Replaces the scatter + gather + GRAPH::sort(false) sequence of
+ * every relabel (class.cu:3099-3116, 423-457). Requires a square matrix. */
+int mr_permute(mr_mat m, const int32_t* perm);
+/* vector relabel on the device: dir 0: out[perm[j]] = in[j] (x into the new
+ * order); dir 1: out[i] = in[perm[i]] (y back to the old order, as the
+ * reference's gather at class.cu:1440). Element size from the matrix's x/y type. */
+int mr_permute_vec(mr_mat m, int dir, const void* in, void* out, mr_stream s);
+
+/* ---- reordering ------------------------------------------------------------ */
+typedef struct {
+    int32_t width, window, batches;
+    double gpu_ms;       /* GPU placement loop (incl. transpose)                 */
+    double cpu_ms;       /* host phase (per-batch Gorder / whole CPU order)      */
+    double relabel_ms;   /* device relabel P A P^T + MERBIT rebuild               */
+    double total_ms;
+} mr_reorder_stats;
+
+/* GRAPH::get_width (class.cu:733-750): 1024, 16384 or 262144 */
+int mr_get_width(int64_t n, int64_t nnz, int* width);
+/* pGOrder = GRAPH::gorder6 (class.cu:2955-3140): GPU greedy placement in
+ * batches (32 doubling up to width, then width), then Gorder with `window`
+ * (5 in the reference) on host threads for every batch from log2(width).
+ * width 0 = get_width.  Relabels the matrix in place; perm_out (host, n,
+ * nullable) receives perm[old] = new. */
+int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_reorder_stats* stats);
+/* the two placement stages of gorder6 without relabelling (position -> vertex):
+ * gpu_order after the GPU loop (class.cu:3062), final_order after the CPU
+ * phase (class.cu:3093); either may be NULL */
+int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
+                     mr_reorder_stats* stats);
+/* sequential CPU Gorder = GRAPH::gorder1 (class.cu:1825-2013); relabels */
+int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_stats* stats);
+/* Reverse Cuthill-McKee = GRAPH::relabel_rcm (class.cu:1554-1602) over
+ * R_Cuthill_Mckee (rcm/src/rcm.cpp:138-154); relabels */
+int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats);
+
+/* ---- reuse distance and the locality predictor ---------------------------- */
+#define MR_RD_EXACT   0
+#define MR_RD_SAMPLED 1
+#define MR_MODEL_MERBIT 0
+#define MR_MODEL_CSR    1
+typedef struct {
+    int32_t sector_size;     /* 32                                               */
+    int32_t samples;         /* 30960                                            */
+    int32_t rd_cap;          /* 589824 (= last edge)                             */
+    int32_t hll_p;           /* 12                                               */
+    int32_t max_scan;        /* 0 = scan to the stream start                     */
+    int32_t stream_type;     /* seed component (class.cu:1203)                   */
+    int32_t graph_id;        /* seed component (class.cu:1203)                   */
+    int32_t n_edges;         /* 9                                                */
+    int32_t bin_edges[32];   /* {1,112,512,18432,36864,73728,147456,294912,589824} */
+} mr_rd_params;
+typedef struct {
+    int64_t reuses, cold;                      /* exact mode                     */
+    int32_t found, cold_samples, large, unresolved;   /* sampled mode status counts */
+    double ms;
+} mr_rd_stats;
+/* the fixed parameters of GRAPH::locality_predict (class.cu:1265-1277) */
+int mr_rd_params_default(mr_rd_params* p);
+/* exact sector RD histogram of a host access stream, compute_sector_rd_histogram
+ * (class.cu:895-936); sectors sized by n as the reference does (:899).
+ * counts / feat hold n_edges + 1 entries (either may be NULL). */
+int mr_rd_exact_stream(const int32_t* stream, int64_t n, int64_t nnz, const mr_rd_params* p, int64_t* counts,
+                       double* feat, mr_rd_stats* stats);
+/* RD features of the matrix's x-access stream (col_idx in CSR order):
+ * MR_RD_EXACT (host Fenwick pass) or MR_RD_SAMPLED (device HyperLogLog,
+ * sampled_hll_rd_block_gpu, class.cu:1133-1261); feat: n_edges + 1 doubles */
+int mr_rd_features(mr_mat m, int mode, const mr_rd_params* p, double* feat, mr_rd_stats* stats);
+/* per-sample outputs of the sampled kernel (estimate, status, position) */
+int mr_rd_sampled_raw(mr_mat m, const mr_rd_params* p, float* est, int32_t* status, int32_t* pos);
+/* rd_merbit_model::PredictSpeedup (include/model.cuh:91-94) and the CSR model
+ * of ridge/out/csr/sampled; MR_ERR_NUMERIC for non-finite input or result */
+int mr_predict(const double* feat, int model, double* speedup);
+/* GRAPH::locality_predict (class.cu:1263-1326): sampled RD features with the
+ * default parameters, then the model; feat_out (10 doubles) may be NULL */
+int mr_locality_predict(mr_mat m, int model, double* speedup, double* feat_out, mr_rd_stats* stats);
+
+/* ---- SpMV / SpMM ---------------------------------------------------------- */
+/* y = A x (overwrite).  MERBIT replaces merbit_kernel (include/merbit.cuh:408-493);
+ * CSR_SCALAR / CSR_VECTOR replace cusparseSpMV CSR_ALG1 (class.cu:1359-1406).
+ * x, y: device pointers (F64 for F64 matrices, F32 otherwise).  Results are
+ * deterministic (no floating-point atomics).  Calls on one handle that overlap
+ * in time on different streams share the handle's MERBIT carry buffer and race;
+ * use one stream per handle or mr_spmv_ws. */
+int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s);
+/* MERBIT with a caller workspace of mr_merbit_ws_bytes() bytes */
+int mr_merbit_ws_bytes(mr_mat m, size_t* bytes);
+int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s);
+/* Y = A X with k right-hand sides, row-major X[n_cols][ldx], Y[n_rows][ldy] */
+int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
+            mr_stream s);
+
+/* ---- MERBIT metadata (TILE<BLOCK,SIGMA>, include/merbit.cuh:7-155) ---------- */
+/* tile geometry: sigma merge-path items per lane (8, 16 or 32), 256 lanes
+ * (4 wave64s) per block; rebuilds the metadata on the device */
+int mr_merbit_configure(mr_mat m, int sigma);
+int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms);
+/* row-end bitmap over the merge path (bit p of word p/32 set when item p ends a
+ * row) and the merge coordinates at every block start (+ sentinel) */
+int mr_merbit_export(mr_mat m, uint32_t* flags, int32_t* blk_x, int32_t* blk_y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,538 @@
+// api.cpp -- C-ABI entry points of libmr: errors, context, device plumbing,
+// matrix handles, SpMV/SpMM dispatch and MERBIT metadata.
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+#include "kernels/launch.hpp"
+
+namespace mr {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+void clear_error() { g_error.clear(); }
+
+static int use_device(mr_ctx c)
+{
+    MR_HIP_TRY(hipSetDevice(c->device));
+    return MR_OK;
+}
+
+static hipStream_t stream_of(mr_ctx c, mr_stream s)
+{
+    return s ? static_cast<hipStream_t>(s) : c->stream;
+}
+
+int merbit_build(mr_mat m, hipStream_t s)
+{
+    MerbitPlan& p = m->merbit;
+    p.built = false;
+    const int64_t T = 256LL * p.sigma;
+    p.items = m->n_rows + m->nnz;
+    if (p.items > INT32_MAX)
+        return set_error(MR_ERR_UNSUPPORTED, "merge path of %lld items exceeds int32",
+                         (long long)p.items);
+    p.n_blocks = (p.items + T - 1) / T;
+    const int64_t words = p.n_blocks * T / 32;
+    MR_TRY(p.flags.alloc(sizeof(uint32_t) * (size_t)(words > 0 ? words : 1)));
+    MR_TRY(p.blk_x.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
+    MR_TRY(p.blk_y.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
+    MR_TRY(p.carry.alloc(vec_size(m->dtype) * (size_t)(p.n_blocks > 0 ? p.n_blocks : 1)));
+    MR_TRY(p.carry_row.alloc(sizeof(int32_t) * (size_t)(p.n_blocks > 0 ? p.n_blocks : 1)));
+    hipEvent_t e0, e1;
+    MR_HIP_TRY(hipEventCreate(&e0));
+    MR_HIP_TRY(hipEventCreate(&e1));
+    MR_HIP_TRY(hipEventRecord(e0, s));
+    MR_HIP_TRY(k::merbit_build(m->n_rows, m->nnz, m->rp.as<int32_t>(), p.sigma, p.n_blocks,
+                               p.flags.as<uint32_t>(), words, p.blk_x.as<int32_t>(),
+                               p.blk_y.as<int32_t>(), s));
+    MR_HIP_TRY(hipEventRecord(e1, s));
+    MR_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    p.build_ms = ms;
+    p.built = true;
+    return MR_OK;
+}
+
+static int check_mat(mr_mat m)
+{
+    if (!m || !m->ctx) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    return use_device(m->ctx);
+}
+
+}  // namespace mr
+
+using namespace mr;
+
+extern "C" {
+
+const char* mr_last_error(void) { return g_error.c_str(); }
+const char* mr_version(void) { return "matrix-reorder-mi355x 0.1 (gfx950)"; }
+
+int mr_device_count(int* count)
+{
+    if (!count) return set_error(MR_ERR_INVALID_ARG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return MR_OK;
+}
+
+int mr_ctx_create(int device, mr_ctx* out)
+{
+    if (!out) return set_error(MR_ERR_INVALID_ARG, "null output");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return set_error(MR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n)
+        return set_error(MR_ERR_INVALID_ARG, "device %d out of range [0,%d)", device, n);
+    MR_HIP_TRY(hipSetDevice(device));
+    auto c = std::make_unique<mr_ctx_s>();
+    c->device = device;
+    MR_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    MR_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    c->arch = prop.gcnArchName;
+    *out = c.release();
+    return MR_OK;
+}
+
+int mr_ctx_destroy(mr_ctx ctx)
+{
+    if (!ctx) return MR_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        hipStreamSynchronize(ctx->stream);
+        hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return MR_OK;
+}
+
+int mr_ctx_stream(mr_ctx ctx, mr_stream* out)
+{
+    if (!ctx || !out) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    *out = ctx->stream;
+    return MR_OK;
+}
+
+int mr_ctx_mem_info(mr_ctx ctx, size_t* free_bytes, size_t* total_bytes)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    MR_TRY(use_device(ctx));
+    size_t f = 0, t = 0;
+    MR_HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return MR_OK;
+}
+
+int mr_ctx_arch(mr_ctx ctx, char* name, int cap, int* cu_count)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    if (name && cap > 0) {
+        std::strncpy(name, ctx->arch.c_str(), (size_t)cap - 1);
+        name[cap - 1] = 0;
+    }
+    if (cu_count) *cu_count = ctx->cu_count;
+    return MR_OK;
+}
+
+int mr_malloc(mr_ctx ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(hipMalloc(dptr, bytes ? bytes : 256));
+    return MR_OK;
+}
+
+int mr_free(mr_ctx ctx, void* dptr)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    if (!dptr) return MR_OK;
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(hipFree(dptr));
+    return MR_OK;
+}
+
+int mr_memcpy(mr_ctx ctx, void* dst, const void* src, size_t bytes, int kind)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    if (bytes == 0) return MR_OK;
+    if (!dst || !src) return set_error(MR_ERR_INVALID_ARG, "null pointer");
+    MR_TRY(use_device(ctx));
+    hipMemcpyKind k = kind == MR_H2D ? hipMemcpyHostToDevice
+                    : kind == MR_D2H ? hipMemcpyDeviceToHost
+                                     : hipMemcpyDeviceToDevice;
+    MR_HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+    MR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MR_OK;
+}
+
+int mr_memset(mr_ctx ctx, void* dptr, int value, size_t bytes)
+{
+    if (!ctx || !dptr) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(hipMemsetAsync(dptr, value, bytes, ctx->stream));
+    MR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MR_OK;
+}
+
+int mr_stream_create(mr_ctx ctx, mr_stream* out)
+{
+    if (!ctx || !out) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    MR_TRY(use_device(ctx));
+    hipStream_t s;
+    MR_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = s;
+    return MR_OK;
+}
+
+int mr_stream_destroy(mr_ctx ctx, mr_stream s)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    if (!s) return MR_OK;
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(s)));
+    return MR_OK;
+}
+
+int mr_stream_sync(mr_ctx ctx, mr_stream s)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(hipStreamSynchronize(stream_of(ctx, s)));
+    return MR_OK;
+}
+
+int mr_event_create(mr_ctx ctx, mr_event* out)
+{
+    if (!ctx || !out) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    MR_TRY(use_device(ctx));
+    hipEvent_t e;
+    MR_HIP_TRY(hipEventCreate(&e));
+    *out = e;
+    return MR_OK;
+}
+
+int mr_event_destroy(mr_ctx ctx, mr_event e)
+{
+    if (!ctx) return set_error(MR_ERR_INVALID_ARG, "null context");
+    if (!e) return MR_OK;
+    MR_HIP_TRY(hipEventDestroy(static_cast<hipEvent_t>(e)));
+    return MR_OK;
+}
+
+int mr_event_record(mr_ctx ctx, mr_event e, mr_stream s)
+{
+    if (!ctx || !e) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(e), stream_of(ctx, s)));
+    return MR_OK;
+}
+
+int mr_event_elapsed_ms(mr_ctx ctx, mr_event start, mr_event stop, float* ms)
+{
+    if (!ctx || !start || !stop || !ms) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    MR_HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+    MR_HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return MR_OK;
+}
+
+// ---------------------------------------------------------------------------
+int mr_csr_create(mr_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz, const int32_t* row_ptr,
+                  const int32_t* col_idx, const void* val, int val_dtype, int store_dtype,
+                  mr_mat* out)
+{
+    if (!ctx || !out || !row_ptr) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    if (n_rows < 0 || n_cols < 0 || nnz < 0 || n_rows > INT32_MAX - 1 || nnz > INT32_MAX - kPad)
+        return set_error(MR_ERR_INVALID_ARG, "sizes out of range (n_rows=%lld nnz=%lld)",
+                         (long long)n_rows, (long long)nnz);
+    if (nnz > 0 && !col_idx) return set_error(MR_ERR_INVALID_ARG, "null col_idx");
+    if (dtype_size(store_dtype) == 0 || (val && dtype_size(val_dtype) == 0))
+        return set_error(MR_ERR_INVALID_ARG, "unknown dtype");
+    // host-side validation: monotone row pointer and column range
+    if (row_ptr[0] != 0 || row_ptr[n_rows] != nnz)
+        return set_error(MR_ERR_MALFORMED_CSR, "row_ptr[0]=%d row_ptr[n]=%d nnz=%lld", row_ptr[0],
+                         row_ptr[n_rows], (long long)nnz);
+    for (int64_t i = 0; i < n_rows; ++i)
+        if (row_ptr[i + 1] < row_ptr[i])
+            return set_error(MR_ERR_MALFORMED_CSR, "row_ptr decreases at row %lld", (long long)i);
+    for (int64_t q = 0; q < nnz; ++q)
+        if (col_idx[q] < 0 || col_idx[q] >= n_cols)
+            return set_error(MR_ERR_MALFORMED_CSR, "col_idx[%lld]=%d outside [0,%lld)", (long long)q,
+                             col_idx[q], (long long)n_cols);
+    MR_TRY(use_device(ctx));
+    auto m = std::make_unique<mr_mat_s>();
+    m->ctx = ctx;
+    m->n_rows = n_rows;
+    m->n_cols = n_cols;
+    m->nnz = nnz;
+    m->dtype = store_dtype;
+    m->avg_row = n_rows > 0 ? (double)nnz / (double)n_rows : 0.0;
+    hipStream_t s = ctx->stream;
+    MR_TRY(m->rp.alloc(sizeof(int32_t) * (size_t)(n_rows + 1)));
+    MR_TRY(m->ci.alloc(sizeof(int32_t) * (size_t)(nnz + kPad)));
+    MR_TRY(m->val.alloc(dtype_size(store_dtype) * (size_t)(nnz + kPad)));
+    MR_HIP_TRY(hipMemsetAsync(m->ci.p, 0, m->ci.bytes, s));
+    MR_HIP_TRY(hipMemsetAsync(m->val.p, 0, m->val.bytes, s));
+    MR_HIP_TRY(hipMemcpyAsync(m->rp.p, row_ptr, sizeof(int32_t) * (size_t)(n_rows + 1),
+                              hipMemcpyHostToDevice, s));
+    if (nnz > 0)
+        MR_HIP_TRY(hipMemcpyAsync(m->ci.p, col_idx, sizeof(int32_t) * (size_t)nnz,
+                                  hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        if (!val) {
+            MR_HIP_TRY(k::fill_ones(m->val.p, store_dtype, nnz, s));
+        } else if (val_dtype == store_dtype) {
+            MR_HIP_TRY(hipMemcpyAsync(m->val.p, val, dtype_size(store_dtype) * (size_t)nnz,
+                                      hipMemcpyHostToDevice, s));
+        } else {
+            DevBuf tmp;
+            MR_TRY(tmp.alloc(dtype_size(val_dtype) * (size_t)nnz));
+            MR_HIP_TRY(hipMemcpyAsync(tmp.p, val, dtype_size(val_dtype) * (size_t)nnz,
+                                      hipMemcpyHostToDevice, s));
+            MR_HIP_TRY(k::convert_values(tmp.p, val_dtype, m->val.p, store_dtype, nnz, s));
+            MR_HIP_TRY(hipStreamSynchronize(s));
+        }
+    }
+    m->perm.resize((size_t)n_rows);
+    for (int64_t i = 0; i < n_rows; ++i) m->perm[(size_t)i] = (int32_t)i;
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    MR_TRY(merbit_build(m.get(), s));
+    *out = m.release();
+    return MR_OK;
+}
+
+int mr_mat_destroy(mr_mat m)
+{
+    if (!m) return MR_OK;
+    if (m->ctx) {
+        hipSetDevice(m->ctx->device);
+        hipStreamSynchronize(m->ctx->stream);
+    }
+    delete m;
+    return MR_OK;
+}
+
+int mr_mat_info(mr_mat m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz, int* dtype)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (n_rows) *n_rows = m->n_rows;
+    if (n_cols) *n_cols = m->n_cols;
+    if (nnz) *nnz = m->nnz;
+    if (dtype) *dtype = m->dtype;
+    return MR_OK;
+}
+
+int mr_mat_export(mr_mat m, int32_t* row_ptr, int32_t* col_idx, void* val)
+{
+    MR_TRY(check_mat(m));
+    hipStream_t s = m->ctx->stream;
+    if (row_ptr)
+        MR_HIP_TRY(hipMemcpyAsync(row_ptr, m->rp.p, sizeof(int32_t) * (size_t)(m->n_rows + 1),
+                                  hipMemcpyDeviceToHost, s));
+    if (col_idx && m->nnz > 0)
+        MR_HIP_TRY(hipMemcpyAsync(col_idx, m->ci.p, sizeof(int32_t) * (size_t)m->nnz,
+                                  hipMemcpyDeviceToHost, s));
+    if (val && m->nnz > 0)
+        MR_HIP_TRY(hipMemcpyAsync(val, m->val.p, dtype_size(m->dtype) * (size_t)m->nnz,
+                                  hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    return MR_OK;
+}
+
+int mr_mat_convert(mr_mat m, int dtype, mr_mat* out)
+{
+    MR_TRY(check_mat(m));
+    if (!out || dtype_size(dtype) == 0) return set_error(MR_ERR_INVALID_ARG, "bad argument");
+    hipStream_t s = m->ctx->stream;
+    auto c = std::make_unique<mr_mat_s>();
+    c->ctx = m->ctx;
+    c->n_rows = m->n_rows;
+    c->n_cols = m->n_cols;
+    c->nnz = m->nnz;
+    c->dtype = dtype;
+    c->avg_row = m->avg_row;
+    c->perm = m->perm;
+    c->merbit.sigma = m->merbit.sigma;
+    MR_TRY(c->rp.alloc(m->rp.bytes));
+    MR_TRY(c->ci.alloc(m->ci.bytes));
+    MR_TRY(c->val.alloc(dtype_size(dtype) * (size_t)(m->nnz + kPad)));
+    MR_HIP_TRY(hipMemcpyAsync(c->rp.p, m->rp.p, m->rp.bytes, hipMemcpyDeviceToDevice, s));
+    MR_HIP_TRY(hipMemcpyAsync(c->ci.p, m->ci.p, m->ci.bytes, hipMemcpyDeviceToDevice, s));
+    MR_HIP_TRY(hipMemsetAsync(c->val.p, 0, c->val.bytes, s));
+    MR_HIP_TRY(k::convert_values(m->val.p, m->dtype, c->val.p, dtype, m->nnz, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    MR_TRY(merbit_build(c.get(), s));
+    *out = c.release();
+    return MR_OK;
+}
+
+int mr_mat_perm(mr_mat m, int32_t* perm)
+{
+    if (!m || !perm) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    if (m->n_rows > 0) std::memcpy(perm, m->perm.data(), sizeof(int32_t) * (size_t)m->n_rows);
+    return MR_OK;
+}
+
+int mr_check_perm(const int32_t* perm, int64_t n)
+{
+    if (!perm || n <= 0) return set_error(MR_ERR_INVALID_ARG, "invalid input: perm=%p n=%lld",
+                                          (const void*)perm, (long long)n);
+    std::vector<uint8_t> seen((size_t)n, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t v = perm[i];
+        if (v < 0 || v >= n)
+            return set_error(MR_ERR_NOT_PERMUTATION, "perm[%lld]=%d outside [0,%lld)", (long long)i, v,
+                             (long long)n);
+        if (seen[(size_t)v])
+            return set_error(MR_ERR_NOT_PERMUTATION, "value %d repeated (at %lld)", v, (long long)i);
+        seen[(size_t)v] = 1;
+    }
+    return MR_OK;
+}
+
+int mr_permute_vec(mr_mat m, int dir, const void* in, void* out, mr_stream s)
+{
+    MR_TRY(check_mat(m));
+    if (!in || !out || (dir != 0 && dir != 1) || m->n_rows != m->n_cols)
+        return set_error(MR_ERR_INVALID_ARG, "bad argument");
+    if (m->n_rows == 0) return MR_OK;
+    hipStream_t st = stream_of(m->ctx, s);
+    DevBuf dperm;
+    MR_TRY(dperm.alloc(sizeof(int32_t) * (size_t)m->n_rows));
+    MR_HIP_TRY(hipMemcpyAsync(dperm.p, m->perm.data(), sizeof(int32_t) * (size_t)m->n_rows,
+                              hipMemcpyHostToDevice, st));
+    MR_HIP_TRY(k::permute_vec(dperm.as<int32_t>(), m->n_rows, dir, in, out, (int)vec_size(m->dtype), st));
+    MR_HIP_TRY(hipStreamSynchronize(st));
+    return MR_OK;
+}
+
+// ---------------------------------------------------------------------------
+int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
+{
+    MR_TRY(check_mat(m));
+    if ((!x || !y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null x or y");
+    hipStream_t st = stream_of(m->ctx, s);
+    switch (kernel) {
+    case MR_KERNEL_MERBIT: {
+        MerbitPlan& p = m->merbit;
+        if (!p.built) MR_TRY(merbit_build(m, st));
+        if (m->dtype == MR_F64 && p.sigma > 16)
+            return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
+        MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
+                                  p.flags.as<uint32_t>(), p.blk_x.as<int32_t>(), p.blk_y.as<int32_t>(), x,
+                                  y, p.carry.p, p.carry_row.as<int32_t>(), st));
+        return MR_OK;
+    }
+    case MR_KERNEL_CSR_SCALAR:
+    case MR_KERNEL_CSR_VECTOR:
+        MR_HIP_TRY(k::csr_spmv(m->dtype, kernel == MR_KERNEL_CSR_VECTOR, m->n_rows, m->rp.as<int32_t>(),
+                               m->ci.as<int32_t>(), m->val.p, x, y, m->avg_row, st));
+        return MR_OK;
+    default:
+        return set_error(MR_ERR_INVALID_ARG, "unknown kernel %d", kernel);
+    }
+}
+
+int mr_merbit_ws_bytes(mr_mat m, size_t* bytes)
+{
+    if (!m || !bytes) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    const size_t nb = (size_t)(m->merbit.n_blocks > 0 ? m->merbit.n_blocks : 1);
+    *bytes = ((nb * vec_size(m->dtype) + 255) / 256) * 256 + nb * sizeof(int32_t);
+    return MR_OK;
+}
+
+int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s)
+{
+    MR_TRY(check_mat(m));
+    if (!ws) return set_error(MR_ERR_INVALID_ARG, "null workspace");
+    if ((!x || !y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null x or y");
+    hipStream_t st = stream_of(m->ctx, s);
+    MerbitPlan& p = m->merbit;
+    if (!p.built) MR_TRY(merbit_build(m, st));
+    if (m->dtype == MR_F64 && p.sigma > 16)
+        return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
+    const size_t nb = (size_t)(p.n_blocks > 0 ? p.n_blocks : 1);
+    char* base = static_cast<char*>(ws);
+    void* carry = base;
+    int32_t* crow = reinterpret_cast<int32_t*>(base + ((nb * vec_size(m->dtype) + 255) / 256) * 256);
+    MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
+                              p.flags.as<uint32_t>(), p.blk_x.as<int32_t>(), p.blk_y.as<int32_t>(), x, y,
+                              carry, crow, st));
+    return MR_OK;
+}
+
+int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, int64_t ldy, mr_stream s)
+{
+    MR_TRY(check_mat(m));
+    if (k <= 0 || ldx < k || ldy < k) return set_error(MR_ERR_INVALID_ARG, "bad k / leading dimension");
+    if ((!X || !Y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null X or Y");
+    hipStream_t st = stream_of(m->ctx, s);
+    if (k == 1) return mr_spmv(m, kernel, X, Y, s);
+    (void)kernel;
+    MR_HIP_TRY(k::csr_spmm(m->dtype, m->n_rows, k, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p, X,
+                           ldx, Y, ldy, st));
+    return MR_OK;
+}
+
+int mr_merbit_configure(mr_mat m, int sigma)
+{
+    MR_TRY(check_mat(m));
+    if (sigma != 8 && sigma != 16 && sigma != 32)
+        return set_error(MR_ERR_INVALID_ARG, "sigma must be 8, 16 or 32 (got %d)", sigma);
+    if (m->dtype == MR_F64 && sigma > 16)
+        return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
+    m->merbit.sigma = sigma;
+    return merbit_build(m, m->ctx->stream);
+}
+
+int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (sigma) *sigma = m->merbit.sigma;
+    if (items) *items = m->merbit.items;
+    if (n_blocks) *n_blocks = m->merbit.n_blocks;
+    if (build_ms) *build_ms = m->merbit.build_ms;
+    return MR_OK;
+}
+
+int mr_merbit_export(mr_mat m, uint32_t* flags, int32_t* blk_x, int32_t* blk_y)
+{
+    MR_TRY(check_mat(m));
+    MerbitPlan& p = m->merbit;
+    if (!p.built) MR_TRY(merbit_build(m, m->ctx->stream));
+    hipStream_t s = m->ctx->stream;
+    const int64_t words = p.n_blocks * 256LL * p.sigma / 32;
+    if (flags && words > 0)
+        MR_HIP_TRY(hipMemcpyAsync(flags, p.flags.p, sizeof(uint32_t) * (size_t)words, hipMemcpyDeviceToHost, s));
+    if (blk_x)
+        MR_HIP_TRY(hipMemcpyAsync(blk_x, p.blk_x.p, sizeof(int32_t) * (size_t)(p.n_blocks + 1),
+                                  hipMemcpyDeviceToHost, s));
+    if (blk_y)
+        MR_HIP_TRY(hipMemcpyAsync(blk_y, p.blk_y.p, sizeof(int32_t) * (size_t)(p.n_blocks + 1),
+                                  hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    return MR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// common.hpp -- internal helpers of libmr: status/error plumbing, the context
+// and matrix handle layouts, and device buffers.  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mr.h"
+
+namespace mr {
+
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+
+#define MR_HIP_TRY(expr)                                                              \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::mr::set_error(e_ == hipErrorOutOfMemory ? MR_ERR_OUT_OF_MEMORY   \
+                                                             : MR_ERR_HIP,            \
+                                   "%s failed: %s (%s:%d)", #expr,                    \
+                                   hipGetErrorString(e_), __FILE__, __LINE__);        \
+    } while (0)
+
+#define MR_TRY(expr)                  \
+    do {                              \
+        int s_ = (expr);              \
+        if (s_ != MR_OK) return s_;   \
+    } while (0)
+
+inline size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case MR_F64: return 8;
+    case MR_F32: return 4;
+    case MR_F16: return 2;
+    case MR_BF16: return 2;
+    default: return 0;
+    }
+}
+// x / y element size for a matrix stored in dt
+inline size_t vec_size(int dt) { return dt == MR_F64 ? 8 : 4; }
+
+// RAII device buffer (hipMalloc / hipFree on the owning device)
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    int alloc(size_t n)
+    {
+        release();
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(MR_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", n,
+                             hipGetErrorString(e));
+        }
+        bytes = n;
+        return MR_OK;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace mr
+
+struct mr_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string arch;
+    std::mutex mu;
+};
+
+// MERBIT plan: merge-path tiles of 256 lanes x sigma items
+struct MerbitPlan {
+    int sigma = 16;
+    int64_t items = 0;       // n_rows + nnz
+    int64_t n_blocks = 0;
+    mr::DevBuf flags;        // u32 words, row-end bitmap (n_blocks * 256 * sigma bits)
+    mr::DevBuf blk_x;        // int32[n_blocks + 1]
+    mr::DevBuf blk_y;        // int32[n_blocks + 1]
+    mr::DevBuf carry;        // accumulator type [n_blocks]
+    mr::DevBuf carry_row;    // int32[n_blocks]
+    bool built = false;
+    double build_ms = 0.0;
+};
+
+struct mr_mat_s {
+    mr_ctx ctx = nullptr;
+    int64_t n_rows = 0, n_cols = 0, nnz = 0;
+    int dtype = MR_F32;
+    mr::DevBuf rp;           // int32[n_rows + 1]
+    mr::DevBuf ci;           // int32[nnz + pad], zero padded
+    mr::DevBuf val;          // dtype[nnz + pad], zero padded
+    std::vector<int32_t> perm;   // accumulated relabelling perm[old] = new
+    MerbitPlan merbit;
+    double avg_row = 0.0;
+};
+
+namespace mr {
+constexpr int64_t kPad = 256;  // element padding of col/val arrays
+int merbit_build(mr_mat m, hipStream_t s);
+}  // namespace mr

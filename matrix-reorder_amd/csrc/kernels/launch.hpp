@@ -1,0 +1,41 @@
+// launch.hpp -- host launchers of the gfx950 kernels (implemented in *.hip).
+// Each returns hipSuccess or the launch error; callers map it to a status.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+
+namespace mr {
+namespace k {
+
+// ---- utility (util.hip) ----
+hipError_t fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+// convert n values from src dtype to dst dtype (MR_F64/F32/F16/BF16)
+hipError_t convert_values(const void* src, int src_dt, void* dst, int dst_dt, int64_t n,
+                          hipStream_t s);
+// fill n values of dtype with 1
+hipError_t fill_ones(void* dst, int dt, int64_t n, hipStream_t s);
+// dir 0: out[perm[j]] = in[j]; dir 1: out[i] = in[perm[i]]; elem 4 or 8 bytes
+hipError_t permute_vec(const int32_t* perm, int64_t n, int dir, const void* in, void* out,
+                       int elem, hipStream_t s);
+// CSR validation: returns via flag (0 ok, 1 non-monotone, 2 column out of range)
+hipError_t validate_csr(int64_t n_rows, int64_t n_cols, const int32_t* rp, const int32_t* ci,
+                        int64_t nnz, int32_t* flag, hipStream_t s);
+
+// ---- CSR SpMV / SpMM (spmv_csr.hip) ----
+hipError_t csr_spmv(int dt, int vector, int64_t n_rows, const int32_t* rp, const int32_t* ci,
+                    const void* val, const void* x, void* y, double avg_row, hipStream_t s);
+hipError_t csr_spmm(int dt, int64_t n_rows, int k, const int32_t* rp, const int32_t* ci,
+                    const void* val, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                    hipStream_t s);
+
+// ---- MERBIT (merbit.hip) ----
+hipError_t merbit_build(int64_t n_rows, int64_t nnz, const int32_t* rp, int sigma,
+                        int64_t n_blocks, uint32_t* flags, int64_t flag_words, int32_t* blk_x,
+                        int32_t* blk_y, hipStream_t s);
+hipError_t merbit_spmv(int dt, int sigma, int64_t n_rows, int64_t n_blocks, const int32_t* ci,
+                       const void* val, const uint32_t* flags, const int32_t* blk_x,
+                       const int32_t* blk_y, const void* x, void* y, void* carry,
+                       int32_t* carry_row, hipStream_t s);
+
+}  // namespace k
+}  // namespace mr

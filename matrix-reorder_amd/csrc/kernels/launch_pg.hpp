@@ -1,0 +1,28 @@
+// launch_pg.hpp -- host launchers of the pGOrder GPU loop (pgorder.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+
+namespace mr {
+namespace k {
+
+// device-resident bookkeeping of the GPU loop
+struct PgState {
+    int32_t tcount[2];   // sizes of the two touched-vertex lists
+    int32_t nsure, ntie; // candidate list sizes
+    int32_t tb, above, need, pad;
+    int32_t lo_chunk;    // first 1024-id chunk that may hold unplaced vertices
+    int32_t pad2[7];
+};
+
+int pg_hist_bins();
+hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, PgState* st, int32_t* chunk_left,
+                   hipStream_t s);
+// one batch: select |P| (pl) at ps, then score the window [os, os+ol)
+hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
+                    uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
+                    int32_t* chunk_left, uint32_t* placement, int32_t ps, int32_t pl, int32_t os, int32_t ol,
+                    hipStream_t s);
+
+}  // namespace k
+}  // namespace mr

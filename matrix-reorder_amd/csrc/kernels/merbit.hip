@@ -1,0 +1,322 @@
+// merbit.hip -- MERBIT merge-path SpMV for gfx950 (wave64).
+//
+// Reference: TILE / Tile_Kernel / merbit_kernel (include/merbit.cuh:7-155,
+// 264-493).  The merge path of a CSR with n rows and nnz nonzeros has
+// n + nnz items: item p "ends a row" when p == row_ptr[r+1] + r for some row r,
+// otherwise it consumes the next nonzero.  The reference cuts it into 32-lane
+// tiles of SIGMA items per lane and stores a 32-bit descriptor per lane
+// (delta_x, delta_y, row-end flags).  This build keeps the merge-path idea and
+// reshapes the format for CDNA4:
+//
+//  * a block is 256 lanes = 4 wave64s, each lane owns S consecutive items
+//    (S = 8, 16 or 32), T = 256*S items per block;
+//  * metadata = the row-end bitmap over the merge path (one bit per item, the
+//    concatenation of the reference's descriptor flag fields) plus the merge
+//    coordinates (blk_x = nonzeros consumed, blk_y = rows ended) at every block
+//    start.  A lane's own coordinates come from a block-wide exclusive scan of
+//    the popcounts of its flag bits, so no per-lane descriptor is stored
+//    (1/8 byte per item at S=16 instead of 4 bytes per 14 items);
+//  * phase 1: the block's nonzeros [blk_x[b], blk_x[b+1]) are streamed with
+//    coalesced dword loads, x is gathered, products go to LDS (the reference's
+//    load_smem, merbit.cuh:291-303); all S loads of a lane are issued before
+//    the gathers so each lane keeps S misses in flight;
+//  * phase 2: each lane walks its S flag bits over the LDS products
+//    (normal_track, merbit.cuh:340-384); completed rows go to an LDS row
+//    buffer, the partial before the lane's first row end and the tail after
+//    its last are combined by a segmented scan over the block (wave64 shuffles
+//    + one LDS exchange between the 4 waves) instead of LDS float atomics;
+//  * phase 3: the block writes its finished rows with coalesced stores; the
+//    partial of the row still open at the block end goes to carry[b] /
+//    carry_row[b] and a second, tiny kernel adds the carries of every row that
+//    crosses block boundaries, in block order.  No floating-point atomics are
+//    used anywhere, so y is bit-reproducible run to run (the reference's
+//    cross-block atomicAdd, merbit.cuh:386-406, is order-dependent);
+//  * blocks whose items all belong to one row (no row end) reduce straight
+//    from registers (the reference's fast track, merbit.cuh:264-288);
+//  * the hardware dispatches workgroups round-robin over the 8 XCDs; block ids
+//    are remapped so each XCD walks a contiguous range of the merge path and
+//    its private L2 sees the x window of that row range.
+#include "dtypes.hpp"
+#include "launch.hpp"
+
+namespace mr {
+namespace k {
+
+namespace {
+
+constexpr int kLanes = 256;
+constexpr int kWaves = kLanes / 64;
+
+// blk_x/blk_y at diagonal d (merge_search, include/merbit.cuh:24-53)
+__device__ __forceinline__ void merge_search(const int32_t* rp, int64_t n_rows, int64_t nnz,
+                                             int64_t diag, int64_t& cx, int64_t& cy)
+{
+    int64_t y_min = diag - nnz > 0 ? diag - nnz : 0;
+    int64_t y_max = diag < n_rows ? diag : n_rows;
+    while (y_min < y_max) {
+        const int64_t mid = (y_min + y_max) >> 1;
+        if (rp[mid + 1] <= diag - mid - 1) y_min = mid + 1;
+        else y_max = mid;
+    }
+    cx = diag - y_min;
+    cy = y_min < n_rows ? y_min : n_rows;
+}
+
+__global__ void flags_kernel(int64_t n_rows, const int32_t* __restrict__ rp, uint32_t* __restrict__ flags)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = (int64_t)rp[r + 1] + r;
+        atomicOr(&flags[p >> 5], 1u << (p & 31));
+    }
+}
+
+__global__ void blocks_kernel(int64_t n_rows, int64_t nnz, int64_t items, int64_t T, int64_t n_blocks,
+                              const int32_t* __restrict__ rp, int32_t* __restrict__ blk_x,
+                              int32_t* __restrict__ blk_y)
+{
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b > n_blocks) return;
+    int64_t d = b * T;
+    if (d > items) d = items;
+    int64_t cx, cy;
+    merge_search(rp, n_rows, nnz, d, cx, cy);
+    blk_x[b] = (int32_t)cx;
+    blk_y[b] = (int32_t)cy;
+}
+
+// logical block of hardware block i: XCD (i % 8) takes a contiguous range
+__device__ __forceinline__ int64_t remap_block(int64_t i, int64_t n)
+{
+    const int64_t full = (n / 8) * 8;
+    if (i >= full) return i;
+    return (i % 8) * (n / 8) + (i / 8);
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename VT, int S>
+__global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
+    int64_t n_blocks, const int32_t* __restrict__ ci, const VT* __restrict__ val,
+    const uint32_t* __restrict__ flags, const int32_t* __restrict__ blk_x,
+    const int32_t* __restrict__ blk_y, const typename Acc<VT>::type* __restrict__ x,
+    typename Acc<VT>::type* __restrict__ y, typename Acc<VT>::type* __restrict__ carry,
+    int32_t* __restrict__ carry_row)
+{
+    using AT = typename Acc<VT>::type;
+    constexpr int T = kLanes * S;
+    __shared__ AT s[T];
+    __shared__ AT s_wv[kWaves];
+    __shared__ int s_wf[kWaves];
+    __shared__ int s_wc[kWaves];
+
+    const int64_t b = remap_block(blockIdx.x, n_blocks);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int32_t x0 = blk_x[b], x1 = blk_x[b + 1];
+    const int32_t y0 = blk_y[b], y1 = blk_y[b + 1];
+    const int nnzb = x1 - x0, rowsb = y1 - y0;
+    const int32_t* cib = ci + x0;
+    const VT* vb = val + x0;
+
+    if (rowsb == 0) {
+        // every item of the block consumes a nonzero of row y0
+        AT acc = 0;
+        int32_t c[S];
+        VT v[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int e = u * kLanes + tid;
+            c[u] = e < nnzb ? cib[e] : 0;
+            v[u] = e < nnzb ? vb[e] : VT(0);
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int e = u * kLanes + tid;
+            if (e < nnzb) acc += (AT)to_f(v[u]) * x[c[u]];
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) s_wv[wid] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            AT t = s_wv[0];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) t += s_wv[w];
+            carry[b] = t;
+            carry_row[b] = y0;
+        }
+        return;
+    }
+
+    // ---- phase 1: products of the block's nonzeros into LDS ---------------
+    const int pad_end = T - rowsb;   // products [0,nnzb), zeros up to pad_end
+    {
+        int32_t c[S];
+        VT v[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int e = u * kLanes + tid;
+            c[u] = e < nnzb ? cib[e] : 0;
+            v[u] = e < nnzb ? vb[e] : VT(0);
+        }
+        AT xv[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int e = u * kLanes + tid;
+            xv[u] = e < nnzb ? x[c[u]] : AT(0);
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int e = u * kLanes + tid;
+            if (e < pad_end) s[e] = (AT)to_f(v[u]) * xv[u];
+        }
+    }
+
+    // ---- lane coordinates: exclusive scan of row-end popcounts -------------
+    const int64_t item0 = b * (int64_t)T + (int64_t)tid * S;
+    uint32_t f = flags[item0 >> 5] >> (item0 & 31);
+    if (S < 32) f &= (1u << S) - 1u;
+    const int cnt = __popc(f);
+    int inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) s_wc[wid] = inc;
+    __syncthreads();
+    int woff = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) woff += (w < wid) ? s_wc[w] : 0;
+    const int dy = woff + inc - cnt;
+    int dx = tid * S - dy;
+
+    // ---- phase 2: walk the lane's items -------------------------------------
+    AT* sy = s + pad_end;
+    AT acc = 0, head = 0;
+    bool has_end = false;
+    int head_row = 0, row = dy;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        if ((f >> j) & 1u) {
+            if (!has_end) { head = acc; head_row = row; has_end = true; }
+            else sy[row] = acc;
+            acc = 0;
+            ++row;
+        } else {
+            acc += s[dx];
+            ++dx;
+        }
+    }
+
+    // ---- segmented scan of (tail, has_end) over the 256 lanes ----------------
+    AT v = acc;
+    int fl = has_end ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const AT vu = __shfl_up(v, d, 64);
+        const int fu = __shfl_up(fl, d, 64);
+        if (lane >= d) {
+            if (!fl) v += vu;
+            fl |= fu;
+        }
+    }
+    if (lane == 63) { s_wv[wid] = v; s_wf[wid] = fl; }
+    __syncthreads();
+    AT pv = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves - 1; ++w)
+        if (w < wid) pv = s_wf[w] ? s_wv[w] : pv + s_wv[w];
+    const AT incl = fl ? v : pv + v;
+    const AT up = __shfl_up(incl, 1, 64);
+    const AT carry_in = lane == 0 ? pv : up;
+    if (has_end) sy[head_row] = carry_in + head;
+    __syncthreads();
+
+    // ---- phase 3: finished rows out, open row to the carry buffer -------------
+    for (int i = tid; i < rowsb; i += kLanes) y[y0 + i] = sy[i];
+    if (tid == kLanes - 1) {
+        carry[b] = incl;
+        carry_row[b] = y1;
+    }
+}
+
+template <typename AT>
+__global__ void merbit_fixup_kernel(int64_t n_blocks, int64_t n_rows, const AT* __restrict__ carry,
+                                    const int32_t* __restrict__ carry_row, AT* __restrict__ y)
+{
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= n_blocks) return;
+    const int32_t r = carry_row[b];
+    if (r >= n_rows) return;
+    if (b > 0 && carry_row[b - 1] == r) return;   // not the first block of the run
+    AT s = carry[b];
+    for (int64_t j = b + 1; j < n_blocks && carry_row[j] == r; ++j) s += carry[j];
+    y[r] = y[r] + s;
+}
+
+template <typename VT, int S>
+hipError_t launch_spmv(int64_t n_rows, int64_t n_blocks, const int32_t* ci, const void* val,
+                       const uint32_t* flags, const int32_t* blk_x, const int32_t* blk_y,
+                       const void* x, void* y, void* carry, int32_t* carry_row, hipStream_t s)
+{
+    using AT = typename Acc<VT>::type;
+    merbit_spmv_kernel<VT, S><<<(unsigned)n_blocks, kLanes, 0, s>>>(
+        n_blocks, ci, static_cast<const VT*>(val), flags, blk_x, blk_y, static_cast<const AT*>(x),
+        static_cast<AT*>(y), static_cast<AT*>(carry), carry_row);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    merbit_fixup_kernel<AT><<<(unsigned)((n_blocks + 255) / 256), 256, 0, s>>>(
+        n_blocks, n_rows, static_cast<const AT*>(carry), carry_row, static_cast<AT*>(y));
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t merbit_build(int64_t n_rows, int64_t nnz, const int32_t* rp, int sigma, int64_t n_blocks,
+                        uint32_t* flags, int64_t flag_words, int32_t* blk_x, int32_t* blk_y,
+                        hipStream_t s)
+{
+    hipError_t e = fill_u32(flags, 0u, flag_words, s);
+    if (e != hipSuccess) return e;
+    if (n_rows > 0) {
+        int64_t g = (n_rows + 255) / 256;
+        if (g > 65536) g = 65536;
+        flags_kernel<<<(unsigned)g, 256, 0, s>>>(n_rows, rp, flags);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const int64_t items = n_rows + nnz;
+    const int64_t T = (int64_t)kLanes * sigma;
+    blocks_kernel<<<(unsigned)((n_blocks + 1 + 255) / 256), 256, 0, s>>>(n_rows, nnz, items, T, n_blocks,
+                                                                          rp, blk_x, blk_y);
+    return hipGetLastError();
+}
+
+hipError_t merbit_spmv(int dt, int sigma, int64_t n_rows, int64_t n_blocks, const int32_t* ci,
+                       const void* val, const uint32_t* flags, const int32_t* blk_x,
+                       const int32_t* blk_y, const void* x, void* y, void* carry,
+                       int32_t* carry_row, hipStream_t s)
+{
+    if (n_blocks <= 0 || n_rows <= 0) return hipSuccess;
+    return dispatch_dt(dt, [&](auto v) -> hipError_t {
+        using VT = decltype(v);
+        switch (sigma) {
+        case 8: return launch_spmv<VT, 8>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+        case 16: return launch_spmv<VT, 16>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+        case 32:
+            // 256 x 32 doubles would exceed the 64 KB LDS of one workgroup
+            if constexpr (sizeof(typename Acc<VT>::type) == 4)
+                return launch_spmv<VT, 32>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+            return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
+        }
+    });
+}
+
+}  // namespace k
+}  // namespace mr

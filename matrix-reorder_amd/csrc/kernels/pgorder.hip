@@ -1,0 +1,462 @@
+// pgorder.hip -- the GPU greedy-placement loop of pGOrder on gfx950.
+//
+// Reference: GRAPH::gorder6 GPU part (src/class.cu:3033-3062) with
+// gorder_gpu_cpu_kernel1 (:2357-2369), gorder6_kernel2 (:2897-2913),
+// gorder6_kernel3 (:2917-2953) and a full CUB stable descending radix sort of
+// all n (score, id) pairs every batch (:3046, 3052).
+//
+// score[v] is a u32: bit 31 = not yet placed, low bits = count.  Batch b places
+// the |P(b)| unplaced vertices that come first by (count descending, id
+// ascending) -- the order the stable radix sort gives over identity ids -- then
+// every score is reset to its status bit and the window O(b) adds one per
+// 2-hop path x -> y -> z (y in cols(x), z in rows(y), hub caps 10000) to every
+// unplaced z.
+//
+// The reference's per-batch full sort and full reset are O(n) per batch
+// (quadratic overall).  After a reset only vertices touched by the window have
+// a non-zero count, so this build keeps the touched set T of the last window
+// (appended on the 0 -> 1 transition of a count) and selects exactly:
+//   |T| <= 4096       : one workgroup sorts T in LDS;
+//   otherwise         : a count histogram (exact below 65536) finds the
+//                       threshold bin, candidates above it are collected, and
+//                       the tie bin is resolved by an id histogram when it is
+//                       too large to sort;
+//   |T| < |P(b)|      : the rest comes from untouched unplaced vertices in
+//                       ascending id order (count 0 ties), found by a
+//                       workgroup scan of per-1024-id unplaced counters.
+// Only T is reset between batches.  Batch 0 starts from count = row length
+// with T = all vertices with a non-empty row.  The placement is identical to
+// the reference's for every input (the oracle restates the full-sort loop and
+// the tests compare the two bit for bit).
+#include "dtypes.hpp"
+#include "launch_pg.hpp"
+
+namespace mr {
+namespace k {
+
+namespace {
+
+constexpr uint32_t kUnplaced = 0x80000000u;
+constexpr int kExactBins = 65536;
+constexpr int kBins = kExactBins + 16 * 64;
+constexpr int kSmallT = 4096;     // sort T directly at or below this size
+constexpr int kChunk = 1024;      // ids per unplaced counter
+constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
+constexpr int kSureCap = 4096;    // capacity of the candidates-above-threshold list
+
+__device__ __forceinline__ int count_bin(uint32_t c)
+{
+    if (c < (uint32_t)kExactBins) return (int)c;
+    const int e = 31 - __clz(c);              // 16..30
+    const int mant = (int)((c >> (e - 6)) & 63u);
+    return kExactBins + (e - 16) * 64 + mant;
+}
+
+// ascending order of this key == (count descending, id ascending)
+__device__ __forceinline__ uint64_t sel_key(uint32_t count, uint32_t id)
+{
+    return ((uint64_t)(~count) << 32) | id;
+}
+
+// wave-aggregated append of `v` when `pred`; entries past `cap` are dropped
+__device__ __forceinline__ void wave_append(bool pred, int32_t v, int32_t* list, int32_t* counter,
+                                            int64_t cap = INT64_MAX)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    const int64_t at = (int64_t)base + __popcll(m & ((1ull << lane) - 1ull));
+    if (pred && at < cap) list[at] = v;
+}
+
+__device__ void lds_bitonic_asc(uint64_t* a, int P)
+{
+    for (int k2 = 2; k2 <= P; k2 <<= 1) {
+        for (int j = k2 >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P / 2; i += blockDim.x) {
+                const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const int hi = lo | j;
+                const uint64_t x = a[lo], y = a[hi];
+                const bool up = (lo & k2) == 0;
+                if ((x > y) == up) { a[lo] = y; a[hi] = x; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// block-wide exclusive prefix count of a bool (256 threads); returns total
+__device__ __forceinline__ int block_rank(bool f, int& rank, int* s_w)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t m = __ballot(f);
+    const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_w[wid] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        off += (w < wid) ? s_w[w] : 0;
+        tot += s_w[w];
+    }
+    __syncthreads();
+    rank = off + in_wave;
+    return tot;
+}
+
+__global__ void init_kernel(int64_t n, const int32_t* __restrict__ rp, uint32_t* __restrict__ score,
+                            int32_t* __restrict__ t0, PgState* __restrict__ st, int32_t* __restrict__ chunk_left)
+{
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < ((n + 63) / 64) * 64;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        bool pred = false;
+        if (v < n) {
+            const int32_t len = rp[v + 1] - rp[v];
+            score[v] = (uint32_t)len | kUnplaced;
+            pred = len > 0;
+            if ((v & (kChunk - 1)) == 0) {
+                const int64_t left = n - v;
+                chunk_left[v / kChunk] = (int32_t)(left < kChunk ? left : kChunk);
+            }
+        }
+        wave_append(pred, (int32_t)v, t0, &st->tcount[0]);
+    }
+}
+
+__global__ __launch_bounds__(256) void hist_kernel(const int32_t* __restrict__ T, const PgState* __restrict__ st,
+                                                   int cur, const uint32_t* __restrict__ score,
+                                                   int32_t* __restrict__ hist)
+{
+    const int32_t tn = st->tcount[cur];
+    if (tn <= kSmallT) return;
+    __shared__ int32_t h[4096];
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tn; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = count_bin(score[T[i]] & ~kUnplaced);
+        if (b < 4096) atomicAdd(&h[b], 1);
+        else atomicAdd(&hist[b], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+__global__ __launch_bounds__(256) void thresh_kernel(const int32_t* __restrict__ hist, PgState* __restrict__ st,
+                                                     int cur, int32_t pl)
+{
+    const int32_t tn = st->tcount[cur];
+    if (tn <= kSmallT) return;
+    __shared__ int64_t s_sum[256];
+    __shared__ int64_t s_above[256];
+    constexpr int seg = (kBins + 255) / 256;
+    const int t = threadIdx.x;
+    const int lo = t * seg, hi = min(kBins, lo + seg);
+    int64_t sum = 0;
+    for (int bnum = lo; bnum < hi; ++bnum) sum += hist[bnum];
+    s_sum[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0;
+        for (int i = 255; i >= 0; --i) { s_above[i] = run; run += s_sum[i]; }
+    }
+    __syncthreads();
+    if (pl <= 0 || tn < pl) {
+        if (t == 0) { st->tb = -1; st->above = tn; st->need = 0; }
+        return;
+    }
+    const int64_t above = s_above[t];
+    if (above < pl && above + sum >= pl) {
+        int64_t run = above;
+        for (int bnum = hi - 1; bnum >= lo; --bnum) {
+            const int32_t c = hist[bnum];
+            if (run + c >= pl) {
+                st->tb = bnum;
+                st->above = (int32_t)run;
+                st->need = (int32_t)(pl - run);
+                break;
+            }
+            run += c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void collect_kernel(const int32_t* __restrict__ T, PgState* __restrict__ st,
+                                                      int cur, const uint32_t* __restrict__ score,
+                                                      int32_t* __restrict__ sure, int32_t* __restrict__ tie,
+                                                      int32_t pl)
+{
+    const int32_t tn = st->tcount[cur];
+    if (tn <= kSmallT || pl <= 0) return;
+    const int tb = st->tb;
+    const int64_t lim = ((int64_t)tn + 63) / 64 * 64;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t v = 0;
+        int b = -2;
+        if (i < tn) {
+            v = T[i];
+            b = count_bin(score[v] & ~kUnplaced);
+        }
+        wave_append(i < tn && b > tb, v, sure, &st->nsure, kSureCap);
+        wave_append(i < tn && b == tb, v, tie, &st->ntie);
+    }
+}
+
+// one workgroup: final selection, fill with untouched vertices, placement,
+// and clearing of the per-batch scratch
+__global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __restrict__ T, int cur,
+                                                    PgState* __restrict__ st, uint32_t* __restrict__ score,
+                                                    const int32_t* __restrict__ sure, const int32_t* __restrict__ tie,
+                                                    int32_t* __restrict__ hist, int32_t* __restrict__ chunk_left,
+                                                    uint32_t* __restrict__ placement, int32_t ps, int32_t pl)
+{
+    __shared__ uint64_t keys[kSmallT];
+    __shared__ int32_t sel[1024];
+    __shared__ int32_t s_w[8];
+    __shared__ int32_t s_nsel;
+    __shared__ int32_t s_bins[4096];
+    __shared__ int32_t s_misc[4];
+    const int tid = threadIdx.x;
+    const int32_t tn = st->tcount[cur];
+    const int32_t want = pl > 0 ? (pl < 1024 ? pl : 1024) : 0;
+    if (tid == 0) {
+        s_nsel = 0;
+        s_misc[0] = 0;
+        s_misc[1] = 0;
+    }
+    __syncthreads();
+
+    auto sort_take = [&](int cnt, int take) {
+        int P = 2;
+        while (P < cnt) P <<= 1;
+        for (int i = cnt + tid; i < P; i += blockDim.x) keys[i] = ~0ull;
+        __syncthreads();
+        lds_bitonic_asc(keys, P);
+        const int base = s_nsel;
+        for (int i = tid; i < take; i += blockDim.x) sel[base + i] = (int32_t)(keys[i] & 0xffffffffu);
+        __syncthreads();
+        if (tid == 0) s_nsel = base + take;
+        __syncthreads();
+    };
+
+    if (want > 0) {
+        if (tn <= kSmallT) {
+            for (int i = tid; i < tn; i += blockDim.x) {
+                const int32_t v = T[i];
+                keys[i] = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
+            }
+            __syncthreads();
+            sort_take(tn, tn < want ? tn : want);
+        } else {
+            const int32_t ns = min(st->nsure, kSureCap), nt = st->ntie, need = st->need;
+            if (ns + nt <= kSmallT) {
+                for (int i = tid; i < ns + nt; i += blockDim.x) {
+                    const int32_t v = i < ns ? sure[i] : tie[i - ns];
+                    keys[i] = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
+                }
+                __syncthreads();
+                const int take = ns + nt < want ? ns + nt : want;
+                sort_take(ns + nt, take);
+            } else {
+                // all sure candidates first (they beat every tie)
+                for (int i = tid; i < ns; i += blockDim.x) {
+                    const int32_t v = sure[i];
+                    keys[i] = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
+                }
+                __syncthreads();
+                sort_take(ns, ns);
+                // the `need` best ties
+                if (st->tb < kExactBins) {
+                    // equal counts: the `need` smallest ids, via an id histogram
+                    int sh = 0;
+                    while (((n - 1) >> sh) >= 4096) ++sh;
+                    for (int i = tid; i < 4096; i += blockDim.x) s_bins[i] = 0;
+                    __syncthreads();
+                    for (int i = tid; i < nt; i += blockDim.x) atomicAdd(&s_bins[tie[i] >> sh], 1);
+                    __syncthreads();
+                    if (tid == 0) {
+                        int run = 0, tbin = 4095;
+                        for (int bb = 0; bb < 4096; ++bb) {
+                            if (run + s_bins[bb] >= need) { tbin = bb; break; }
+                            run += s_bins[bb];
+                        }
+                        s_misc[0] = tbin;
+                        s_misc[1] = 0;
+                    }
+                    __syncthreads();
+                    const int tbin = s_misc[0];
+                    // ids in bins <= tbin: fewer than need + 2^sh; sort them by id
+                    for (int i = tid; i < nt; i += blockDim.x) {
+                        const int32_t v = tie[i];
+                        if ((v >> sh) <= tbin) {
+                            const int pos = atomicAdd(&s_misc[1], 1);
+                            if (pos < kSmallT) keys[pos] = (uint64_t)(uint32_t)v;
+                        }
+                    }
+                    __syncthreads();
+                    const int cnt = s_misc[1];
+                    if (cnt <= kSmallT) sort_take(cnt, need);
+                }
+                if (st->tb >= kExactBins || (st->tb < kExactBins && s_misc[1] > kSmallT)) {
+                    // rare general case: repeated maximum over the tie list
+                    uint64_t last = 0;
+                    for (int r = 0; r < need; ++r) {
+                        uint64_t best = ~0ull;
+                        for (int i = tid; i < nt; i += blockDim.x) {
+                            const int32_t v = tie[i];
+                            const uint64_t kk = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
+                            if ((r == 0 || kk > last) && kk < best) best = kk;
+                        }
+                        for (int off = 32; off > 0; off >>= 1) {
+                            const uint64_t o = __shfl_xor(best, off, 64);
+                            best = o < best ? o : best;
+                        }
+                        if ((tid & 63) == 0) keys[tid >> 6] = best;
+                        __syncthreads();
+                        if (tid == 0) {
+                            uint64_t bm = keys[0];
+                            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = keys[w] < bm ? keys[w] : bm;
+                            sel[s_nsel] = (int32_t)(bm & 0xffffffffu);
+                            s_nsel = s_nsel + 1;
+                            keys[8] = bm;
+                        }
+                        __syncthreads();
+                        last = keys[8];
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // place the selected touched vertices
+    const int nsel_t = s_nsel;
+    for (int i = tid; i < nsel_t; i += blockDim.x) {
+        const int32_t v = sel[i];
+        score[v] = 0u;
+        placement[ps + i] = (uint32_t)v;
+        atomicSub(&chunk_left[v / kChunk], 1);
+    }
+    __syncthreads();
+    // fill with untouched unplaced vertices in ascending id order
+    int missing = want - nsel_t;
+    if (missing > 0) {
+        const int64_t nchunks = (n + kChunk - 1) / kChunk;
+        int64_t c = st->lo_chunk;
+        int filled = 0;
+        while (missing > 0 && c < nchunks) {
+            const int left = chunk_left[c];
+            __syncthreads();
+            if (left == 0) {
+                if (tid == 0 && st->lo_chunk == c) st->lo_chunk = (int32_t)(c + 1);
+                __syncthreads();
+                ++c;
+                continue;
+            }
+            const int64_t base = c * kChunk;
+            for (int sub = 0; sub < kChunk && missing > 0; sub += 256) {
+                const int64_t id = base + sub + tid;
+                const bool f = id < n && score[id] == kUnplaced;
+                int rank = 0;
+                const int tot = block_rank(f, rank, s_w);
+                if (f && rank < missing) sel[nsel_t + filled + rank] = (int32_t)id;
+                const int got = tot < missing ? tot : missing;
+                filled += got;
+                missing -= got;
+            }
+            ++c;
+        }
+        __syncthreads();
+        for (int i = tid; i < filled; i += blockDim.x) {
+            const int32_t v = sel[nsel_t + i];
+            score[v] = 0u;
+            placement[ps + nsel_t + i] = (uint32_t)v;
+            atomicSub(&chunk_left[v / kChunk], 1);
+        }
+    }
+    __syncthreads();
+    // clear per-batch scratch for the next batch
+    if (tn > kSmallT)
+        for (int i = tid; i < kBins; i += blockDim.x) hist[i] = 0;
+    if (tid == 0) {
+        st->nsure = 0;
+        st->ntie = 0;
+        st->tcount[cur ^ 1] = 0;
+    }
+}
+
+__global__ void reset_kernel(const int32_t* __restrict__ T, const PgState* __restrict__ st, int cur,
+                             uint32_t* __restrict__ score)
+{
+    const int32_t tn = st->tcount[cur];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tn; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = T[i];
+        if (score[v] != 0u) score[v] = kUnplaced;
+    }
+}
+
+// window score: one workgroup per window vertex x, a wave per y in cols(x),
+// lanes over z in rows(y) (gorder6_kernel3 with wave64)
+__global__ __launch_bounds__(256) void window_kernel(int64_t n, const uint32_t* __restrict__ window,
+                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                     const int32_t* __restrict__ tp, const int32_t* __restrict__ ti,
+                                                     uint32_t* __restrict__ score, int32_t* __restrict__ T,
+                                                     PgState* __restrict__ st, int nxt)
+{
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t xu = window[blockIdx.x];
+    if (xu >= (uint32_t)n) return;   // never taken when every position is placed
+    const int32_t x = (int32_t)xu;
+    const int32_t yb = rp[x], ye = rp[x + 1];
+    if (ye - yb >= 10000) return;
+    for (int32_t a = yb + wid; a < ye; a += 4) {
+        const int32_t y = ci[a];
+        const int32_t zb = tp[y], ze = tp[y + 1];
+        if (ze - zb >= 10000) continue;
+        for (int32_t c0 = zb; c0 < ze; c0 += 64) {
+            const int32_t c = c0 + lane;
+            bool first = false;
+            int32_t z = 0;
+            if (c < ze) {
+                z = ti[c];
+                if (score[z] & kUnplaced) first = atomicAdd(&score[z], 1u) == kUnplaced;
+            }
+            wave_append(first, z, T, &st->tcount[nxt]);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, PgState* st, int32_t* chunk_left,
+                   hipStream_t s)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    init_kernel<<<(unsigned)g, 256, 0, s>>>(n, rp, score, t0, st, chunk_left);
+    return hipGetLastError();
+}
+
+hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
+                    uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
+                    int32_t* chunk_left, uint32_t* placement, int32_t ps, int32_t pl, int32_t os, int32_t ol,
+                    hipStream_t s)
+{
+    const int nxt = cur ^ 1;
+    hist_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, hist);
+    thresh_kernel<<<1, 256, 0, s>>>(hist, st, cur, pl);
+    collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl);
+    final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, hist, chunk_left, placement, ps, pl);
+    reset_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score);
+    if (ol > 0) window_kernel<<<(unsigned)ol, 256, 0, s>>>(n, placement + os, rp, ci, tp, ti, score, Tl[nxt], st, nxt);
+    return hipGetLastError();
+}
+
+int pg_hist_bins() { return kBins; }
+
+}  // namespace k
+}  // namespace mr

@@ -1,0 +1,494 @@
+// reorder.cpp -- reordering entry points of libmr: the GPU relabel
+// (P A P^T), pGOrder (GPU greedy placement + per-batch CPU Gorder), the
+// sequential CPU Gorder and Reverse Cuthill-McKee baselines.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "bucket_heap.hpp"
+#include "common.hpp"
+#include "kernels/launch.hpp"
+#include "kernels/launch_perm.hpp"
+#include "kernels/launch_pg.hpp"
+#include "parallel.hpp"
+#include "reorder.hpp"
+
+namespace mr {
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+// segmented row sort of a CSR-shaped structure on the device:
+// new row p = old row inv[p], entries mapped through perm and sorted.
+int sort_rows(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const void* val, int val_bytes,
+              const int32_t* perm, const int32_t* inv, const int32_t* new_rp, int32_t* new_ci, void* new_val,
+              hipStream_t s)
+{
+    DevBuf d_len, d_counts, d_offs, d_cursor, d_lists;
+    MR_TRY(d_len.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(n, 1)));
+    MR_TRY(d_counts.alloc(sizeof(int32_t) * k::kPermClasses));
+    MR_TRY(d_offs.alloc(sizeof(int32_t) * k::kPermClasses));
+    MR_TRY(d_cursor.alloc(sizeof(int32_t) * k::kPermClasses));
+    MR_TRY(d_lists.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(n, 1)));
+    MR_HIP_TRY(k::perm_row_lengths(n, rp, inv, d_len.as<int32_t>(), s));
+    MR_HIP_TRY(hipMemsetAsync(d_counts.p, 0, d_counts.bytes, s));
+    MR_HIP_TRY(hipMemsetAsync(d_cursor.p, 0, d_cursor.bytes, s));
+    MR_HIP_TRY(k::perm_class_count(n, d_len.as<int32_t>(), d_counts.as<int32_t>(), s));
+    int32_t counts[k::kPermClasses], offs[k::kPermClasses];
+    MR_HIP_TRY(hipMemcpyAsync(counts, d_counts.p, sizeof(counts), hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    int32_t run = 0;
+    for (int c = 0; c < k::kPermClasses; ++c) { offs[c] = run; run += counts[c]; }
+    MR_HIP_TRY(hipMemcpyAsync(d_offs.p, offs, sizeof(offs), hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(k::perm_class_fill(n, d_len.as<int32_t>(), d_offs.as<int32_t>(), d_cursor.as<int32_t>(),
+                                  d_lists.as<int32_t>(), s));
+    // scratch rows for the longest class
+    DevBuf d_scratch, d_soff;
+    const int32_t nbig = counts[k::kPermClasses - 1];
+    if (nbig > 0) {
+        std::vector<int32_t> rows((size_t)nbig), lens((size_t)n);
+        MR_HIP_TRY(hipMemcpyAsync(rows.data(), d_lists.as<int32_t>() + offs[k::kPermClasses - 1],
+                                  sizeof(int32_t) * (size_t)nbig, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipMemcpyAsync(lens.data(), d_len.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+        std::vector<int64_t> soff((size_t)nbig);
+        int64_t tot = 0;
+        for (int32_t i = 0; i < nbig; ++i) {
+            int64_t P = 8192;
+            while (P < lens[(size_t)rows[(size_t)i]]) P <<= 1;
+            soff[(size_t)i] = tot;
+            tot += P;
+        }
+        MR_TRY(d_scratch.alloc(sizeof(uint64_t) * (size_t)tot));
+        MR_TRY(d_soff.alloc(sizeof(int64_t) * (size_t)nbig));
+        MR_HIP_TRY(hipMemcpyAsync(d_soff.p, soff.data(), sizeof(int64_t) * (size_t)nbig, hipMemcpyHostToDevice, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+    }
+    for (int c = 0; c < k::kPermClasses; ++c)
+        MR_HIP_TRY(k::perm_sort_class(c, counts[c], d_lists.as<int32_t>() + offs[c], rp, ci, val, val_bytes, perm,
+                                      inv, new_rp, new_ci, new_val, d_scratch.as<uint64_t>(),
+                                      d_soff.as<int64_t>(), s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    (void)nnz;
+    return MR_OK;
+}
+
+}  // namespace
+
+int gpu_transpose(mr_mat m, DevBuf& tp, DevBuf& ti, hipStream_t s)
+{
+    const int64_t n = m->n_rows, nc = m->n_cols, nnz = m->nnz;
+    MR_TRY(tp.alloc(sizeof(int32_t) * (size_t)(nc + 1)));
+    MR_TRY(ti.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    DevBuf cnt, cursor, raw, ident, dummy, ws;
+    MR_TRY(cnt.alloc(sizeof(int32_t) * (size_t)(nc + 1)));
+    MR_TRY(cursor.alloc(sizeof(int32_t) * (size_t)(nc + 1)));
+    MR_TRY(raw.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    MR_TRY(ident.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(std::max(n, nc), 1)));
+    MR_TRY(dummy.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    MR_TRY(ws.alloc(k::scan_workspace_bytes(nc + 1)));
+    MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
+    MR_HIP_TRY(hipMemsetAsync(cursor.p, 0, cursor.bytes, s));
+    MR_HIP_TRY(k::transpose_count(nnz, m->ci.as<int32_t>(), cnt.as<int32_t>(), s));
+    MR_HIP_TRY(k::exclusive_scan_i32(cnt.as<int32_t>(), nc + 1, tp.as<int32_t>(), nullptr, ws.p, s));
+    MR_HIP_TRY(k::transpose_fill(n, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(),
+                                 cursor.as<int32_t>(), raw.as<int32_t>(), s));
+    MR_HIP_TRY(k::iota_i32(std::max(n, nc), ident.as<int32_t>(), s));
+    // sort each column's row list: rows of (tp, raw) mapped through identity
+    MR_TRY(sort_rows(nc, nnz, tp.as<int32_t>(), raw.as<int32_t>(), raw.p, 4, ident.as<int32_t>(),
+                     ident.as<int32_t>(), tp.as<int32_t>(), ti.as<int32_t>(), dummy.p, s));
+    return MR_OK;
+}
+
+int relabel(mr_mat m, const std::vector<int32_t>& perm, double* ms_out)
+{
+    const auto t0 = clk::now();
+    const int64_t n = m->n_rows, nnz = m->nnz;
+    hipStream_t s = m->ctx->stream;
+    if (n == 0) return MR_OK;
+    std::vector<int32_t> inv((size_t)n);
+    for (int64_t i = 0; i < n; ++i) inv[(size_t)perm[(size_t)i]] = (int32_t)i;
+    DevBuf d_perm, d_inv, d_len, d_newrp, d_newci, d_newval, ws;
+    MR_TRY(d_perm.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(d_inv.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(d_len.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(d_newrp.alloc(sizeof(int32_t) * (size_t)(n + 1)));
+    MR_TRY(d_newci.alloc(sizeof(int32_t) * (size_t)(nnz + kPad)));
+    MR_TRY(d_newval.alloc(dtype_size(m->dtype) * (size_t)(nnz + kPad)));
+    MR_TRY(ws.alloc(k::scan_workspace_bytes(n)));
+    MR_HIP_TRY(hipMemcpyAsync(d_perm.p, perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(hipMemcpyAsync(d_inv.p, inv.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(hipMemsetAsync(d_newci.p, 0, d_newci.bytes, s));
+    MR_HIP_TRY(hipMemsetAsync(d_newval.p, 0, d_newval.bytes, s));
+    MR_HIP_TRY(k::perm_row_lengths(n, m->rp.as<int32_t>(), d_inv.as<int32_t>(), d_len.as<int32_t>(), s));
+    MR_HIP_TRY(k::exclusive_scan_i32(d_len.as<int32_t>(), n, d_newrp.as<int32_t>(), nullptr, ws.p, s));
+    const int32_t tail = (int32_t)nnz;
+    MR_HIP_TRY(hipMemcpyAsync(d_newrp.as<int32_t>() + n, &tail, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    MR_TRY(sort_rows(n, nnz, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p, (int)dtype_size(m->dtype),
+                     d_perm.as<int32_t>(), d_inv.as<int32_t>(), d_newrp.as<int32_t>(), d_newci.as<int32_t>(),
+                     d_newval.p, s));
+    m->rp = std::move(d_newrp);
+    m->ci = std::move(d_newci);
+    m->val = std::move(d_newval);
+    for (auto& p : m->perm) p = perm[(size_t)p];
+    if (ms_out) *ms_out = ms_since(t0);
+    return merbit_build(m, s);
+}
+
+// ---------------------------------------------------------------------------
+// pGOrder batch schedule (src/class.cu:2995-3016)
+int32_t pg_schedule(int32_t n, int32_t W, std::vector<int32_t>& Ps, std::vector<int32_t>& Pl,
+                    std::vector<int32_t>& Os, std::vector<int32_t>& Ol)
+{
+    const int base = 5;
+    const int32_t kk = (int32_t)std::log2((double)W / std::pow(2.0, base));
+    const int32_t B = (n + W - 1) / W + kk;
+    Ps.assign((size_t)B, 0); Pl.assign((size_t)B, 0); Os.assign((size_t)B, 0); Ol.assign((size_t)B, 0);
+    for (int32_t b = 0; b < B; ++b) {
+        if (b == 0) { Ps[0] = 0; Pl[0] = 32; Os[0] = 0; Ol[0] = 32; continue; }
+        const double p1 = std::pow(2.0, b + base - 1), p2 = std::pow(2.0, b + base);
+        int32_t start = (p1 < W) ? (int32_t)p1 : (b - kk) * W;
+        int32_t leng = (p1 < W) ? (int32_t)p1 : W;
+        Ps[(size_t)b] = start;
+        Pl[(size_t)b] = (start + leng < n) ? leng : n - start;
+        start = (p1 < W) ? 0 : (b - kk) * W;
+        leng = (p2 < W) ? (int32_t)p2 : W;
+        Os[(size_t)b] = start;
+        Ol[(size_t)b] = (start + leng < n) ? leng : n - start;
+    }
+    return B;
+}
+
+// ---------------------------------------------------------------------------
+// order_on_cpu (src/class.cu:2220-2314) for one batch.  In-batch siblings of a
+// vertex are reached through y -> z lists restricted to the batch, built in
+// ascending z order, which issues exactly the reference's sequence of heap
+// updates (it walks all of rows(y) in ascending order and skips vertices of
+// other batches).
+struct PgCpuShared {
+    const int32_t *rp, *ci, *tp, *ti;
+    const int32_t* local_idx;  // position of a vertex inside its batch slice
+    int32_t window, degree;
+};
+
+struct PgCpuWorker {
+    BucketHeap heap;
+    std::vector<int32_t> yslot;      // per vertex, -1 when not a y of this batch
+    std::vector<int32_t> ys, ycnt, ystart, zlist, sorted, out;
+    std::vector<uint8_t> sts;
+};
+
+void order_batch(const PgCpuShared& g, PgCpuWorker& w, uint32_t* slice, int32_t len)
+{
+    if (len <= 0) return;
+    const int32_t* rp = g.rp;
+    const int32_t* ci = g.ci;
+    const int32_t* tp = g.tp;
+    const int32_t L = g.degree;
+    auto collen = [&](int32_t v) { return tp[v + 1] - tp[v]; };
+    auto rowlen = [&](int32_t v) { return rp[v + 1] - rp[v]; };
+    // start: first vertex with strictly maximal out-list length
+    int32_t start = 0, deg = collen((int32_t)slice[0]);
+    for (int32_t i = 0; i < len; ++i)
+        if (collen((int32_t)slice[i]) > deg) { deg = collen((int32_t)slice[i]); start = i; }
+    // y -> in-batch z lists, z ascending
+    w.sorted.assign(slice, slice + len);
+    std::sort(w.sorted.begin(), w.sorted.end());
+    w.ys.clear();
+    w.ycnt.clear();
+    for (int32_t z : w.sorted)
+        for (int32_t a = rp[z]; a < rp[z + 1]; ++a) {
+            const int32_t y = ci[a];
+            if (collen(y) >= L) continue;
+            int32_t& s = w.yslot[(size_t)y];
+            if (s < 0) { s = (int32_t)w.ys.size(); w.ys.push_back(y); w.ycnt.push_back(0); }
+            w.ycnt[(size_t)s]++;
+        }
+    w.ystart.assign(w.ys.size() + 1, 0);
+    for (size_t i = 0; i < w.ys.size(); ++i) w.ystart[i + 1] = w.ystart[i] + w.ycnt[i];
+    w.zlist.resize((size_t)w.ystart.back());
+    std::fill(w.ycnt.begin(), w.ycnt.end(), 0);
+    for (int32_t z : w.sorted) {
+        const int32_t lz = g.local_idx[z];
+        for (int32_t a = rp[z]; a < rp[z + 1]; ++a) {
+            const int32_t y = ci[a];
+            if (collen(y) >= L) continue;
+            const int32_t s = w.yslot[(size_t)y];
+            w.zlist[(size_t)(w.ystart[(size_t)s] + w.ycnt[(size_t)s]++)] = lz;
+        }
+    }
+    w.heap.reset(len);
+    for (int32_t i = 0; i < len; ++i) w.heap.push(i);
+    w.heap.modify(start, 1);
+    w.sts.assign((size_t)len, 1);
+    w.out.resize((size_t)len);
+    auto apply = [&](int32_t x, int32_t d) {
+        if (rowlen(x) >= L) return;
+        for (int32_t a = rp[x]; a < rp[x + 1]; ++a) {
+            const int32_t y = ci[a];
+            if (collen(y) >= L) continue;
+            const int32_t s = w.yslot[(size_t)y];
+            if (s < 0) continue;
+            for (int32_t q = w.ystart[(size_t)s]; q < w.ystart[(size_t)s + 1]; ++q) {
+                const int32_t z = w.zlist[(size_t)q];
+                if (w.sts[(size_t)z]) w.heap.modify(z, d);
+            }
+        }
+    };
+    for (int32_t id = 0; id < len; ++id) {
+        const int32_t li = w.heap.pop();
+        w.sts[(size_t)li] = 0;
+        const int32_t node = (int32_t)slice[li];
+        w.out[(size_t)id] = node;
+        apply(node, 1);
+        if (id >= g.window) apply(w.out[(size_t)(id - g.window)], -1);
+    }
+    for (int32_t i = 0; i < len; ++i) slice[i] = (uint32_t)w.out[(size_t)i];
+    for (int32_t y : w.ys) w.yslot[(size_t)y] = -1;
+}
+
+}  // namespace mr
+
+using namespace mr;
+
+extern "C" {
+
+int mr_get_width(int64_t n, int64_t nnz, int* width)
+{
+    if (!width || n <= 0 || nnz < 0) return set_error(MR_ERR_INVALID_ARG, "bad arguments");
+    // GRAPH::get_width (src/class.cu:733-750): float arithmetic, integer nnz/n
+    const int32_t nrow = (int32_t)n, nz = (int32_t)nnz;
+    const float q = (float)nrow / (float)(nz / nrow);
+    const float bound = std::round(std::log2(std::sqrt(q)));
+    *width = bound < 10.0f ? 1024 : (bound < 12.0f ? 16384 : 262144);
+    return MR_OK;
+}
+
+int mr_permute(mr_mat m, const int32_t* perm)
+{
+    if (!m || !perm) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "symmetric relabel needs a square matrix");
+    if (m->n_rows == 0) return MR_OK;
+    MR_TRY(mr_check_perm(perm, m->n_rows));
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    std::vector<int32_t> p(perm, perm + m->n_rows);
+    return relabel(m, p, nullptr);
+}
+
+int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
+                     mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "pGOrder needs a square matrix");
+    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (n64 == 0) return MR_OK;
+    int W = width;
+    if (W <= 0) MR_TRY(mr_get_width(n64, nnz, &W));
+    if (W < 32 || W > 1024 || (W & (W - 1)))
+        return set_error(MR_ERR_UNSUPPORTED, "pGOrder GPU loop supports power-of-two widths 32..1024 (got %d)", W);
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    const auto t_all = clk::now();
+    const int32_t n = (int32_t)n64;
+    hipStream_t s = m->ctx->stream;
+    std::vector<int32_t> Ps, Pl, Os, Ol;
+    const int32_t B = pg_schedule(n, W, Ps, Pl, Os, Ol);
+    // the reference never clips the first window (class.cu:3003); for n < 32 it
+    // would read past the placement, here it is clipped to n
+    if (Ol[0] > n) Ol[0] = n;
+    for (int32_t b = 0; b < B; ++b)
+        if (Os[(size_t)b] + Ol[(size_t)b] > n) Ol[(size_t)b] = n - Os[(size_t)b];
+
+    // ---- GPU loop ----
+    const auto t_gpu = clk::now();
+    DevBuf tp, ti;
+    MR_TRY(gpu_transpose(m, tp, ti, s));
+    DevBuf score, T0, T1, sure, tie, hist, chunk, placement, state;
+    MR_TRY(score.alloc(sizeof(uint32_t) * (size_t)n));
+    MR_TRY(T0.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(T1.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(sure.alloc(sizeof(int32_t) * 4096));
+    MR_TRY(tie.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(hist.alloc(sizeof(int32_t) * (size_t)k::pg_hist_bins()));
+    MR_TRY(chunk.alloc(sizeof(int32_t) * (size_t)((n + 1023) / 1024 + 1)));
+    MR_TRY(placement.alloc(sizeof(uint32_t) * (size_t)n));
+    MR_TRY(state.alloc(sizeof(k::PgState)));
+    MR_HIP_TRY(hipMemsetAsync(hist.p, 0, hist.bytes, s));
+    MR_HIP_TRY(hipMemsetAsync(state.p, 0, state.bytes, s));
+    MR_HIP_TRY(hipMemsetAsync(placement.p, 0xff, placement.bytes, s));
+    k::PgState* st = state.as<k::PgState>();
+    MR_HIP_TRY(k::pg_init(n, m->rp.as<int32_t>(), score.as<uint32_t>(), T0.as<int32_t>(), st, chunk.as<int32_t>(), s));
+    int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
+    int cur = 0;
+    for (int32_t b = 0; b < B; ++b) {
+        MR_HIP_TRY(k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(), ti.as<int32_t>(),
+                               score.as<uint32_t>(), Tl, st, sure.as<int32_t>(), tie.as<int32_t>(),
+                               hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(), Ps[(size_t)b],
+                               Pl[(size_t)b], Os[(size_t)b], Ol[(size_t)b], s));
+        cur ^= 1;
+    }
+    std::vector<uint32_t> order((size_t)n);
+    MR_HIP_TRY(hipMemcpyAsync(order.data(), placement.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    const double gpu_ms = ms_since(t_gpu);
+    if (gpu_order) std::memcpy(gpu_order, order.data(), sizeof(uint32_t) * (size_t)n);
+    {
+        std::vector<int32_t> chk(order.begin(), order.end());
+        if (mr_check_perm(chk.data(), n) != MR_OK)
+            return set_error(MR_ERR_NUMERIC, "pGOrder GPU loop produced an invalid placement");
+    }
+
+    // ---- CPU phase: order_on_cpu per batch from log2(W) ----
+    const auto t_cpu = clk::now();
+    std::vector<int32_t> hrp((size_t)n + 1), hci((size_t)std::max<int64_t>(nnz, 1)), htp((size_t)n + 1),
+        hti((size_t)std::max<int64_t>(nnz, 1));
+    MR_HIP_TRY(hipMemcpyAsync(hrp.data(), m->rp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipMemcpyAsync(htp.data(), tp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    if (nnz > 0) {
+        MR_HIP_TRY(hipMemcpyAsync(hci.data(), m->ci.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipMemcpyAsync(hti.data(), ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+    }
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    const int32_t first = (int32_t)std::log2((double)W);
+    std::vector<int32_t> local_idx((size_t)n, -1);
+    for (int32_t b = first; b < B; ++b)
+        for (int32_t i = 0; i < Ol[(size_t)b]; ++i) local_idx[(size_t)order[(size_t)(Os[(size_t)b] + i)]] = i;
+    PgCpuShared g{hrp.data(), hci.data(), htp.data(), hti.data(), local_idx.data(), window,
+                  (int32_t)(std::sqrt((double)n) / 4)};
+    std::atomic<int32_t> next{first};
+    const int nth = std::max(1, std::min<int>(host_threads(), B - first));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth && first < B; ++t)
+        pool.emplace_back([&] {
+            PgCpuWorker w;
+            w.yslot.assign((size_t)n, -1);
+            for (int32_t b = next++; b < B; b = next++)
+                order_batch(g, w, order.data() + Os[(size_t)b], Ol[(size_t)b]);
+        });
+    for (auto& th : pool) th.join();
+    const double cpu_ms = ms_since(t_cpu);
+    if (final_order) std::memcpy(final_order, order.data(), sizeof(uint32_t) * (size_t)n);
+    if (stats) {
+        stats->width = W;
+        stats->window = window;
+        stats->batches = B;
+        stats->gpu_ms = gpu_ms;
+        stats->cpu_ms = cpu_ms;
+        stats->total_ms = ms_since(t_all);
+    }
+    return MR_OK;
+}
+
+int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    const int64_t n = m->n_rows;
+    if (n == 0) return MR_OK;
+    const auto t0 = clk::now();
+    std::vector<uint32_t> order((size_t)n);
+    mr_reorder_stats st{};
+    MR_TRY(mr_pgorder_order(m, width, window, nullptr, order.data(), &st));
+    std::vector<int32_t> perm((size_t)n);
+    for (int64_t i = 0; i < n; ++i) perm[(size_t)order[(size_t)i]] = (int32_t)i;
+    double rl = 0;
+    MR_TRY(relabel(m, perm, &rl));
+    if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+    if (stats) {
+        *stats = st;
+        stats->relabel_ms = rl;
+        stats->total_ms = ms_since(t0);
+    }
+    return MR_OK;
+}
+
+int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "Gorder needs a square matrix");
+    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (n64 == 0) return MR_OK;
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    const auto t0 = clk::now();
+    const int32_t n = (int32_t)n64;
+    hipStream_t s = m->ctx->stream;
+    DevBuf tp, ti;
+    MR_TRY(gpu_transpose(m, tp, ti, s));
+    std::vector<int32_t> rp((size_t)n + 1), ci((size_t)std::max<int64_t>(nnz, 1)), tpp((size_t)n + 1),
+        tii((size_t)std::max<int64_t>(nnz, 1));
+    MR_HIP_TRY(hipMemcpyAsync(rp.data(), m->rp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipMemcpyAsync(tpp.data(), tp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    if (nnz > 0) {
+        MR_HIP_TRY(hipMemcpyAsync(ci.data(), m->ci.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipMemcpyAsync(tii.data(), ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+    }
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<uint32_t> order((size_t)n);
+    gorder_cpu(n, rp.data(), ci.data(), tpp.data(), tii.data(), window, order.data());
+    std::vector<int32_t> perm((size_t)n);
+    for (int32_t i = 0; i < n; ++i) perm[(size_t)order[(size_t)i]] = i;
+    const double order_ms = ms_since(t0);
+    double rl = 0;
+    MR_TRY(relabel(m, perm, &rl));
+    if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+    if (stats) {
+        stats->width = 0;
+        stats->window = window;
+        stats->batches = 1;
+        stats->gpu_ms = 0;
+        stats->cpu_ms = order_ms;
+        stats->relabel_ms = rl;
+        stats->total_ms = ms_since(t0);
+    }
+    return MR_OK;
+}
+
+int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "RCM needs a square matrix");
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (n64 == 0) return MR_OK;
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    const auto t0 = clk::now();
+    const int32_t n = (int32_t)n64;
+    std::vector<int32_t> rp((size_t)n + 1), ci((size_t)std::max<int64_t>(nnz, 1));
+    MR_TRY(mr_mat_export(m, rp.data(), ci.data(), nullptr));
+    // the reference passes its COO (row_ind, col_ind); as a CSR of M the pair
+    // is (col, row) -- RCM symmetrises, so the entry order is what matters:
+    // the reference COO is sorted by (col, row) = rows of M in order.
+    std::vector<int32_t> rows((size_t)std::max<int64_t>(nnz, 1)), cols((size_t)std::max<int64_t>(nnz, 1));
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t q = rp[(size_t)i]; q < rp[(size_t)i + 1]; ++q) {
+            rows[(size_t)q] = ci[(size_t)q];
+            cols[(size_t)q] = i;
+        }
+    std::vector<uint32_t> order((size_t)n);
+    rcm_order(n, nnz, rows.data(), cols.data(), order.data());
+    std::vector<int32_t> perm((size_t)n);
+    for (int32_t i = 0; i < n; ++i) perm[(size_t)order[(size_t)i]] = i;
+    const double order_ms = ms_since(t0);
+    double rl = 0;
+    MR_TRY(relabel(m, perm, &rl));
+    if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->cpu_ms = order_ms;
+        stats->relabel_ms = rl;
+        stats->total_ms = ms_since(t0);
+    }
+    return MR_OK;
+}
+
+}  // extern "C"

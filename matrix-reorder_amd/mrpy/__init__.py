@@ -1,0 +1,568 @@
+"""ctypes binding of libmr (include/mr.h) for tests, the benchmark and scripts.
+
+Mirrors the reference's GRAPH<V> operator surface (include/class.cuh:59-99):
+``Graph`` wraps a matrix handle with ``gorder6`` / ``gorder1`` / ``relabel_rcm``
+/ ``locality_predict`` / ``spmv``; the lower-level ``Mat`` exposes every
+C-ABI entry point.  Device memory is handled through the library's own
+plumbing (``Ctx.malloc`` / ``DeviceArray``), so no other HIP runtime has to be
+loaded into the process.  Loading fails loudly when libmr.so is missing: there
+is no CPU fallback for any product entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "lib" / "libmr.so"
+
+F64, F32, F16, BF16 = 0, 1, 2, 3
+MERBIT, CSR_SCALAR, CSR_VECTOR = 0, 1, 2
+H2D, D2H, D2D = 0, 1, 2
+RD_EXACT, RD_SAMPLED = 0, 1
+MODEL_MERBIT, MODEL_CSR = 0, 1
+
+_NP_OF = {F64: np.float64, F32: np.float32, F16: np.float16, BF16: np.uint16}
+STATUS = {
+    0: "MR_OK", 1: "MR_ERR_INVALID_ARG", 2: "MR_ERR_HIP", 3: "MR_ERR_OUT_OF_MEMORY",
+    4: "MR_ERR_NOT_PERMUTATION", 5: "MR_ERR_UNSUPPORTED", 6: "MR_ERR_MALFORMED_CSR", 7: "MR_ERR_IO",
+    8: "MR_ERR_ADMISSION", 9: "MR_ERR_NO_DEVICE", 10: "MR_ERR_NUMERIC", 11: "MR_ERR_TIMEOUT",
+    12: "MR_ERR_CANCELLED",
+}
+
+
+class MRError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ReorderStats(C.Structure):
+    _fields_ = [("width", C.c_int32), ("window", C.c_int32), ("batches", C.c_int32),
+                ("gpu_ms", C.c_double), ("cpu_ms", C.c_double), ("relabel_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class RDParams(C.Structure):
+    _fields_ = [("sector_size", C.c_int32), ("samples", C.c_int32), ("rd_cap", C.c_int32),
+                ("hll_p", C.c_int32), ("max_scan", C.c_int32), ("stream_type", C.c_int32),
+                ("graph_id", C.c_int32), ("n_edges", C.c_int32), ("bin_edges", C.c_int32 * 32)]
+
+
+class RDStats(C.Structure):
+    _fields_ = [("reuses", C.c_int64), ("cold", C.c_int64), ("found", C.c_int32),
+                ("cold_samples", C.c_int32), ("large", C.c_int32), ("unresolved", C.c_int32),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_U32P = C.POINTER(C.c_uint32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+
+# name -> argtypes (every entry point returns int unless listed in _RESTYPE)
+SIGNATURES = {
+    "mr_last_error": [], "mr_version": [],
+    "mr_device_count": [C.POINTER(C.c_int)],
+    "mr_ctx_create": [C.c_int, C.POINTER(_P)], "mr_ctx_destroy": [_P],
+    "mr_ctx_stream": [_P, C.POINTER(_P)],
+    "mr_ctx_mem_info": [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "mr_ctx_arch": [_P, C.c_char_p, C.c_int, C.POINTER(C.c_int)],
+    "mr_malloc": [_P, C.c_size_t, C.POINTER(_P)], "mr_free": [_P, _P],
+    "mr_memcpy": [_P, _P, _P, C.c_size_t, C.c_int], "mr_memset": [_P, _P, C.c_int, C.c_size_t],
+    "mr_stream_create": [_P, C.POINTER(_P)], "mr_stream_destroy": [_P, _P], "mr_stream_sync": [_P, _P],
+    "mr_event_create": [_P, C.POINTER(_P)], "mr_event_destroy": [_P, _P],
+    "mr_event_record": [_P, _P, _P], "mr_event_elapsed_ms": [_P, _P, _P, _FP],
+    "mr_gen_rmat_edges": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
+                          _I32P, _I32P],
+    "mr_coo_to_csr": [C.c_int64, C.c_int64, _I32P, _I32P, C.c_int, _I32P, _I32P, _I64P],
+    "mr_gen_values": [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_int64, _DP],
+    "mr_gen_stencil27": [C.c_int, C.c_int, C.c_int, _I32P, _I32P, _I64P, _I64P],
+    "mr_gen_banded": [C.c_int64, C.c_int, C.c_int, C.c_uint64, _I32P, _I32P, _I64P],
+    "mr_csr_create": [_P, C.c_int64, C.c_int64, C.c_int64, _I32P, _I32P, _P, C.c_int, C.c_int, C.POINTER(_P)],
+    "mr_mtx_load": [_P, C.c_char_p, C.c_int, C.c_int, C.POINTER(_P)],
+    "mr_mat_destroy": [_P],
+    "mr_mat_info": [_P, _I64P, _I64P, _I64P, C.POINTER(C.c_int)],
+    "mr_mat_export": [_P, _I32P, _I32P, _P],
+    "mr_mat_convert": [_P, C.c_int, C.POINTER(_P)],
+    "mr_mat_perm": [_P, _I32P],
+    "mr_check_perm": [_I32P, C.c_int64],
+    "mr_permute": [_P, _I32P],
+    "mr_permute_vec": [_P, C.c_int, _P, _P, _P],
+    "mr_get_width": [C.c_int64, C.c_int64, C.POINTER(C.c_int)],
+    "mr_reorder_pgorder": [_P, C.c_int, C.c_int, _I32P, C.POINTER(ReorderStats)],
+    "mr_pgorder_order": [_P, C.c_int, C.c_int, _U32P, _U32P, C.POINTER(ReorderStats)],
+    "mr_reorder_gorder_cpu": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
+    "mr_reorder_rcm": [_P, _I32P, C.POINTER(ReorderStats)],
+    "mr_rd_params_default": [C.POINTER(RDParams)],
+    "mr_rd_exact_stream": [_I32P, C.c_int64, C.c_int64, C.POINTER(RDParams), _I64P, _DP, C.POINTER(RDStats)],
+    "mr_rd_features": [_P, C.c_int, C.POINTER(RDParams), _DP, C.POINTER(RDStats)],
+    "mr_rd_sampled_raw": [_P, C.POINTER(RDParams), _FP, _I32P, _I32P],
+    "mr_predict": [_DP, C.c_int, _DP],
+    "mr_locality_predict": [_P, C.c_int, _DP, _DP, C.POINTER(RDStats)],
+    "mr_spmv": [_P, C.c_int, _P, _P, _P],
+    "mr_merbit_ws_bytes": [_P, C.POINTER(C.c_size_t)],
+    "mr_spmv_ws": [_P, _P, _P, _P, _P],
+    "mr_spmm": [_P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int64, _P],
+    "mr_merbit_configure": [_P, C.c_int],
+    "mr_merbit_info": [_P, C.POINTER(C.c_int), _I64P, _I64P, _DP],
+    "mr_merbit_export": [_P, _U32P, _I32P, _I32P],
+}
+_RESTYPE = {"mr_last_error": C.c_char_p, "mr_version": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load libmr.so once; raise when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no fallback exists)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != 0:
+        raise MRError(code, lib().mr_last_error().decode())
+
+
+def ptr(a: np.ndarray | None, ct=None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    if ct is None:
+        return a.ctypes.data_as(C.c_void_p)
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().mr_device_count(C.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------- host generators
+def gen_rmat(scale: int, edge_factor: int = 16, abc=(0.57, 0.19, 0.19), seed: int = 1, scramble: bool = True,
+             dedup: bool = True):
+    """R-MAT CSR (row_ptr, col_idx) with 2^scale rows; self loops kept."""
+    m = edge_factor << scale
+    src = np.empty(m, np.int32)
+    dst = np.empty(m, np.int32)
+    check(lib().mr_gen_rmat_edges(scale, edge_factor, abc[0], abc[1], abc[2], seed, int(scramble),
+                                  ptr(src, C.c_int32), ptr(dst, C.c_int32)))
+    n = 1 << scale
+    rp = np.empty(n + 1, np.int32)
+    ci = np.empty(m, np.int32)
+    nnz = C.c_int64(0)
+    check(lib().mr_coo_to_csr(n, m, ptr(src, C.c_int32), ptr(dst, C.c_int32), int(dedup), ptr(rp, C.c_int32),
+                              ptr(ci, C.c_int32), C.byref(nnz)))
+    return rp, ci[: nnz.value].copy()
+
+
+def coo_to_csr(n: int, src: np.ndarray, dst: np.ndarray, dedup: bool = True):
+    src = np.ascontiguousarray(src, np.int32)
+    dst = np.ascontiguousarray(dst, np.int32)
+    rp = np.empty(n + 1, np.int32)
+    ci = np.empty(max(len(src), 1), np.int32)
+    nnz = C.c_int64(0)
+    check(lib().mr_coo_to_csr(n, len(src), ptr(src, C.c_int32), ptr(dst, C.c_int32), int(dedup),
+                              ptr(rp, C.c_int32), ptr(ci, C.c_int32), C.byref(nnz)))
+    return rp, ci[: nnz.value].copy()
+
+
+def gen_values(seed: int, count: int, kind: int = 0, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+    out = np.empty(max(count, 1), np.float64)
+    check(lib().mr_gen_values(seed, kind, lo, hi, count, ptr(out, C.c_double)))
+    return out[:count]
+
+
+def gen_stencil27(nx: int, ny: int, nz: int):
+    n = C.c_int64(0)
+    nnz = C.c_int64(0)
+    check(lib().mr_gen_stencil27(nx, ny, nz, None, None, C.byref(n), C.byref(nnz)))
+    rp = np.empty(n.value + 1, np.int32)
+    ci = np.empty(nnz.value, np.int32)
+    check(lib().mr_gen_stencil27(nx, ny, nz, ptr(rp, C.c_int32), ptr(ci, C.c_int32), None, None))
+    return rp, ci
+
+
+def gen_banded(n: int, bandwidth: int, nnz_per_row: int, seed: int):
+    nnz = C.c_int64(0)
+    check(lib().mr_gen_banded(n, bandwidth, nnz_per_row, seed, None, None, C.byref(nnz)))
+    rp = np.empty(n + 1, np.int32)
+    ci = np.empty(max(nnz.value, 1), np.int32)
+    check(lib().mr_gen_banded(n, bandwidth, nnz_per_row, seed, ptr(rp, C.c_int32), ptr(ci, C.c_int32), None))
+    return rp, ci[: nnz.value]
+
+
+def get_width(n: int, nnz: int) -> int:
+    w = C.c_int(0)
+    check(lib().mr_get_width(n, nnz, C.byref(w)))
+    return w.value
+
+
+def check_perm(perm: np.ndarray) -> bool:
+    perm = np.ascontiguousarray(perm, np.int32)
+    code = lib().mr_check_perm(ptr(perm, C.c_int32), len(perm))
+    if code == 4:
+        return False
+    check(code)
+    return True
+
+
+def predict(feat, model: int = MODEL_MERBIT) -> float:
+    f = np.ascontiguousarray(feat, np.float64)
+    out = C.c_double(0)
+    check(lib().mr_predict(ptr(f, C.c_double), model, C.byref(out)))
+    return out.value
+
+
+def rd_params(**kw) -> RDParams:
+    p = RDParams()
+    check(lib().mr_rd_params_default(C.byref(p)))
+    for k, v in kw.items():
+        if k == "bin_edges":
+            p.n_edges = len(v)
+            for i, e in enumerate(v):
+                p.bin_edges[i] = e
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def rd_exact_stream(stream: np.ndarray, n: int, params: RDParams | None = None):
+    p = params or rd_params()
+    s = np.ascontiguousarray(stream, np.int32)
+    nb = p.n_edges + 1
+    counts = np.zeros(nb, np.int64)
+    feat = np.zeros(nb, np.float64)
+    st = RDStats()
+    check(lib().mr_rd_exact_stream(ptr(s, C.c_int32), n, len(s), C.byref(p), ptr(counts, C.c_int64),
+                                   ptr(feat, C.c_double), C.byref(st)))
+    return counts, feat, st.as_dict()
+
+
+# ---------------------------------------------------------------- device plumbing
+class DeviceArray:
+    """A device buffer owned through the library's own allocator."""
+
+    def __init__(self, ctx: "Ctx", nbytes: int, dtype=np.float32, shape=None):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        p = C.c_void_p()
+        check(lib().mr_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            check(lib().mr_memcpy(ctx.h, d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, H2D))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().mr_memcpy(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, D2H))
+        return out
+
+    def zero(self):
+        check(lib().mr_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().mr_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Ctx:
+    """A device context (replaces the reference's ENV, include/class.cuh:13-36)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().mr_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().mr_ctx_destroy(self.h)
+            self.h = None
+
+    def arch(self):
+        buf = C.create_string_buffer(64)
+        cu = C.c_int(0)
+        check(lib().mr_ctx_arch(self.h, buf, 64, C.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    def mem_info(self):
+        f = C.c_size_t(0)
+        t = C.c_size_t(0)
+        check(lib().mr_ctx_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().mr_ctx_stream(self.h, C.byref(s)))
+        return s
+
+    def new_stream(self):
+        s = C.c_void_p()
+        check(lib().mr_stream_create(self.h, C.byref(s)))
+        return s
+
+    def sync(self, stream=None):
+        check(lib().mr_stream_sync(self.h, stream))
+
+    def event(self):
+        e = C.c_void_p()
+        check(lib().mr_event_create(self.h, C.byref(e)))
+        return e
+
+    def record(self, ev, stream=None):
+        check(lib().mr_event_record(self.h, ev, stream))
+
+    def elapsed(self, e0, e1) -> float:
+        ms = C.c_float(0)
+        check(lib().mr_event_elapsed_ms(self.h, e0, e1, C.byref(ms)))
+        return ms.value
+
+    def upload(self, a: np.ndarray) -> DeviceArray:
+        return DeviceArray.from_host(self, a)
+
+    def empty(self, shape, dtype) -> DeviceArray:
+        shape = (shape,) if isinstance(shape, int) else tuple(shape)
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        return DeviceArray(self, n, dtype, shape)
+
+
+class Mat:
+    """A device CSR handle (mr_mat)."""
+
+    def __init__(self, ctx: Ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    @classmethod
+    def from_csr(cls, ctx: Ctx, rp, ci, val=None, n_cols=None, store=F32):
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        n = len(rp) - 1
+        nnz = int(rp[-1]) if n >= 0 else 0
+        n_cols = n if n_cols is None else n_cols
+        vp, vdt = None, F64
+        if val is not None:
+            val = np.ascontiguousarray(val)
+            vdt = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}[val.dtype]
+            vp = val.ctypes.data_as(C.c_void_p)
+        h = C.c_void_p()
+        ci_arg = ci if len(ci) else np.zeros(1, np.int32)
+        check(lib().mr_csr_create(ctx.h, n, n_cols, nnz, ptr(rp, C.c_int32), ptr(ci_arg, C.c_int32), vp, vdt, store,
+                                  C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load_mtx(cls, ctx: Ctx, path: str, store=F32, force_ones=True):
+        h = C.c_void_p()
+        check(lib().mr_mtx_load(ctx.h, str(path).encode(), store, int(force_ones), C.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().mr_mat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        r, c, z, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        check(lib().mr_mat_info(self.h, C.byref(r), C.byref(c), C.byref(z), C.byref(d)))
+        return r.value, c.value, z.value, d.value
+
+    @property
+    def vec_dtype(self):
+        return np.float64 if self.info()[3] == F64 else np.float32
+
+    def export(self):
+        n, _, nnz, dt = self.info()
+        rp = np.empty(n + 1, np.int32)
+        ci = np.empty(max(nnz, 1), np.int32)
+        val = np.empty(max(nnz, 1), _NP_OF[dt])
+        check(lib().mr_mat_export(self.h, ptr(rp, C.c_int32), ptr(ci, C.c_int32), ptr(val)))
+        return rp, ci[:nnz], val[:nnz]
+
+    def convert(self, dtype: int) -> "Mat":
+        h = C.c_void_p()
+        check(lib().mr_mat_convert(self.h, dtype, C.byref(h)))
+        return Mat(self.ctx, h)
+
+    def perm(self) -> np.ndarray:
+        n = self.info()[0]
+        p = np.empty(max(n, 1), np.int32)
+        check(lib().mr_mat_perm(self.h, ptr(p, C.c_int32)))
+        return p[:n]
+
+    def permute(self, perm):
+        perm = np.ascontiguousarray(perm, np.int32)
+        check(lib().mr_permute(self.h, ptr(perm, C.c_int32)))
+
+    def permute_vec(self, direction: int, src: DeviceArray, dst: DeviceArray, stream=None):
+        check(lib().mr_permute_vec(self.h, direction, src.ptr, dst.ptr, stream))
+
+    # reorder
+    def reorder_pgorder(self, width: int = 0, window: int = 5):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(lib().mr_reorder_pgorder(self.h, width, window, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
+    def pgorder_order(self, width: int = 0, window: int = 5):
+        n = self.info()[0]
+        g = np.empty(max(n, 1), np.uint32)
+        f = np.empty(max(n, 1), np.uint32)
+        st = ReorderStats()
+        check(lib().mr_pgorder_order(self.h, width, window, ptr(g, C.c_uint32), ptr(f, C.c_uint32), C.byref(st)))
+        return g[:n], f[:n], st.as_dict()
+
+    def reorder_gorder_cpu(self, window: int = 5):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(lib().mr_reorder_gorder_cpu(self.h, window, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
+    def reorder_rcm(self):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(lib().mr_reorder_rcm(self.h, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
+    # reuse distance
+    def rd_features(self, mode=RD_SAMPLED, params: RDParams | None = None):
+        p = params or rd_params()
+        feat = np.zeros(p.n_edges + 1, np.float64)
+        st = RDStats()
+        check(lib().mr_rd_features(self.h, mode, C.byref(p), ptr(feat, C.c_double), C.byref(st)))
+        return feat, st.as_dict()
+
+    def rd_sampled_raw(self, params: RDParams | None = None):
+        p = params or rd_params()
+        s = p.samples
+        est = np.zeros(s, np.float32)
+        status = np.zeros(s, np.int32)
+        pos = np.zeros(s, np.int32)
+        check(lib().mr_rd_sampled_raw(self.h, C.byref(p), ptr(est, C.c_float), ptr(status, C.c_int32),
+                                      ptr(pos, C.c_int32)))
+        return est, status, pos
+
+    def locality_predict(self, model=MODEL_MERBIT):
+        sp = C.c_double(0)
+        feat = np.zeros(10, np.float64)
+        st = RDStats()
+        check(lib().mr_locality_predict(self.h, model, C.byref(sp), ptr(feat, C.c_double), C.byref(st)))
+        return sp.value, feat, st.as_dict()
+
+    # multiply
+    def spmv(self, x: DeviceArray, y: DeviceArray, kernel=MERBIT, stream=None):
+        check(lib().mr_spmv(self.h, kernel, x.ptr, y.ptr, stream))
+
+    def spmv_ws_bytes(self) -> int:
+        b = C.c_size_t(0)
+        check(lib().mr_merbit_ws_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def spmv_ws(self, x: DeviceArray, y: DeviceArray, ws: DeviceArray, stream=None):
+        check(lib().mr_spmv_ws(self.h, x.ptr, y.ptr, ws.ptr, stream))
+
+    def spmm(self, k: int, X: DeviceArray, Y: DeviceArray, kernel=MERBIT, ldx=None, ldy=None, stream=None):
+        check(lib().mr_spmm(self.h, kernel, k, X.ptr, ldx or k, Y.ptr, ldy or k, stream))
+
+    def merbit_configure(self, sigma: int):
+        check(lib().mr_merbit_configure(self.h, sigma))
+
+    def merbit_info(self):
+        s, it, nb, ms = C.c_int(), C.c_int64(), C.c_int64(), C.c_double()
+        check(lib().mr_merbit_info(self.h, C.byref(s), C.byref(it), C.byref(nb), C.byref(ms)))
+        return {"sigma": s.value, "items": it.value, "n_blocks": nb.value, "build_ms": ms.value}
+
+    def merbit_export(self):
+        info = self.merbit_info()
+        words = info["n_blocks"] * 256 * info["sigma"] // 32
+        flags = np.zeros(max(words, 1), np.uint32)
+        bx = np.zeros(info["n_blocks"] + 1, np.int32)
+        by = np.zeros(info["n_blocks"] + 1, np.int32)
+        check(lib().mr_merbit_export(self.h, ptr(flags, C.c_uint32), ptr(bx, C.c_int32), ptr(by, C.c_int32)))
+        return flags[:words], bx, by, info
+
+
+class Graph:
+    """C++-style mirror of the reference's GRAPH<V> workflow (include/class.cuh:59-99):
+    load, locality_predict, gorder6 / gorder1 / relabel_rcm, spmv."""
+
+    def __init__(self, ctx: Ctx, mat: Mat):
+        self.ctx = ctx
+        self.mat = mat
+
+    def get_width(self) -> int:
+        n, _, nnz, _ = self.mat.info()
+        return get_width(n, nnz)
+
+    def gorder6(self, width: int = 0):
+        return self.mat.reorder_pgorder(width or self.get_width(), 5)
+
+    def gorder1(self, width: int = 5):
+        return self.mat.reorder_gorder_cpu(width)
+
+    def relabel_rcm(self):
+        return self.mat.reorder_rcm()
+
+    def locality_predict(self) -> float:
+        return self.mat.locality_predict(MODEL_MERBIT)[0]
+
+    def spmv(self, x: np.ndarray, method: str = "merbit") -> np.ndarray:
+        """y = A x on the current (relabelled) matrix, x and y in the ORIGINAL
+        vertex order (x is relabelled on the way in, y on the way out, as
+        class.cu:1437-1445 does for its check)."""
+        kernel = {"merbit": MERBIT, "csr": CSR_VECTOR, "csr_scalar": CSR_SCALAR}[method]
+        dt = self.mat.vec_dtype
+        xd = self.ctx.upload(np.ascontiguousarray(x, dt))
+        xp = self.ctx.empty(len(x), dt)
+        self.mat.permute_vec(0, xd, xp)
+        yp = self.ctx.empty(self.mat.info()[0], dt)
+        self.mat.spmv(xp, yp, kernel)
+        y = self.ctx.empty(self.mat.info()[0], dt)
+        self.mat.permute_vec(1, yp, y)
+        return y.to_host()

@@ -1,0 +1,336 @@
+/*
+ * gorder.c -- oracle restatements of the Gorder family.  TEST INFRASTRUCTURE ONLY.
+ *
+ *   oracle_get_width         src/class.cu:733-750  (float arithmetic kept)
+ *   oracle_pgorder_schedule  src/class.cu:2995-3016 (Pstart/Pleng/Ostart/Oleng)
+ *   oracle_pgorder_gpu_loop  src/class.cu:3041-3059 with gorder_gpu_cpu_kernel1
+ *                            (:2357-2369), gorder6_kernel2 (:2897-2913),
+ *                            gorder6_kernel3 (:2917-2953) and the stable
+ *                            descending CUB radix sort (:3046,3052), restated
+ *                            as "score descending, vertex id ascending".
+ *   oracle_order_on_cpu      src/class.cu:2220-2314
+ *   oracle_pgorder           src/class.cu:2955-3140 (CPU phase from batch log2(W),
+ *                            :3077-3093; perm scatter :3101-3102)
+ *   oracle_gorder1           src/class.cu:1825-2013
+ *
+ * Divergence kept on purpose: for n < 32 the reference's first window reads
+ * d_permutation past n (Oleng[0] = 32 is never clipped, class.cu:3003); here the
+ * window is clipped to n.
+ *
+ * Pinned by: heap golden traces; hand-derived schedule for W=1024 and small W;
+ * brute-force top-|P| selection per batch on small graphs (selection equals the
+ * first |P| of all unplaced vertices by naive 2-hop score, id ascending); the
+ * max-score invariant of every gorder1/order_on_cpu pop against a naive score
+ * recomputation; hand-worked tiny graphs (tests/test_oracle_gorder.py).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+int32_t oracle_get_width(int64_t n, int64_t nnz)
+{
+    /* float bound = round(log2(sqrt((float)nrow / (nnz / nrow)))) */
+    int32_t nrow = (int32_t)n, nz = (int32_t)nnz;
+    float q = (float)nrow / (float)(nz / nrow);
+    float bound = roundf(log2f(sqrtf(q)));
+    if (bound < 10.0f) return 1024;
+    if (bound >= 10.0f && bound < 12.0f) return 16384;
+    return 262144;
+}
+
+void oracle_transpose(int32_t n_rows, int32_t n_cols, const int32_t* rp,
+                      const int32_t* ci, int32_t* tp, int32_t* ti)
+{
+    memset(tp, 0, sizeof(int32_t) * ((size_t)n_cols + 1));
+    for (int32_t i = 0; i < n_rows; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) tp[ci[k] + 1]++;
+    for (int32_t j = 0; j < n_cols; ++j) tp[j + 1] += tp[j];
+    int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n_cols + 1));
+    memcpy(fill, tp, sizeof(int32_t) * ((size_t)n_cols + 1));
+    for (int32_t i = 0; i < n_rows; ++i)              /* rows ascending -> sorted */
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) ti[fill[ci[k]]++] = i;
+    free(fill);
+}
+
+/* ------------------------------------------------------------------------ */
+static int32_t ilog2_floor(double v)
+{
+    int32_t r = 0;
+    while (v >= 2.0) { v /= 2.0; ++r; }
+    return r;
+}
+
+int32_t oracle_pgorder_schedule(int32_t n, int32_t W, int32_t cap, int32_t* Ps,
+                                int32_t* Pl, int32_t* Os, int32_t* Ol)
+{
+    /* base = 5; batch_size = ceil(n/W) + log2(W / 2^5) (class.cu:2996-2998) */
+    const int32_t base = 5;
+    const int32_t k = (int32_t)log2((double)W / pow(2.0, base));
+    const int32_t B = (n + W - 1) / W + k;
+    for (int32_t b = 0; b < B && b < cap; ++b) {
+        if (b == 0) { Ps[0] = 0; Pl[0] = 32; Os[0] = 0; Ol[0] = 32; continue; }
+        const double p1 = pow(2.0, b + base - 1), p2 = pow(2.0, b + base);
+        int32_t start = (p1 < W) ? (int32_t)p1 : (b - k) * W;
+        int32_t leng = (p1 < W) ? (int32_t)p1 : W;
+        Ps[b] = start;
+        Pl[b] = (start + leng < n) ? leng : n - start;
+        start = (p1 < W) ? 0 : (b - k) * W;
+        leng = (p2 < W) ? (int32_t)p2 : W;
+        Os[b] = start;
+        Ol[b] = (start + leng < n) ? leng : n - start;
+    }
+    (void)ilog2_floor;
+    return B;
+}
+
+/* ------------------------------------------------------------------------ */
+typedef struct { uint32_t score; uint32_t id; } skey;
+
+static int cmp_desc_stable(const void* a, const void* b)
+{
+    const skey* x = (const skey*)a;
+    const skey* y = (const skey*)b;
+    if (x->score != y->score) return (x->score > y->score) ? -1 : 1;
+    return (x->id < y->id) ? -1 : (x->id > y->id);
+}
+
+int oracle_pgorder_gpu_loop(int32_t n, const int32_t* rp, const int32_t* ci,
+                            const int32_t* tp, const int32_t* ti, int32_t W,
+                            uint32_t* placement)
+{
+    const int32_t cap = (n + W - 1) / W + 64;
+    int32_t *Ps = malloc(sizeof(int32_t) * cap), *Pl = malloc(sizeof(int32_t) * cap);
+    int32_t *Os = malloc(sizeof(int32_t) * cap), *Ol = malloc(sizeof(int32_t) * cap);
+    const int32_t B = oracle_pgorder_schedule(n, W, cap, Ps, Pl, Os, Ol);
+    uint32_t* score = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    skey* keys = (skey*)malloc(sizeof(skey) * (size_t)n);
+    /* kernel1: score = rowlen | 1<<31 (offset2 = SpMV row pointer) */
+    for (int32_t v = 0; v < n; ++v) score[v] = (uint32_t)(rp[v + 1] - rp[v]) | 0x80000000u;
+    for (int32_t b = 0; b < B; ++b) {
+        /* stable descending radix sort of (score, id) with ids in identity order */
+        for (int32_t v = 0; v < n; ++v) { keys[v].score = score[v]; keys[v].id = (uint32_t)v; }
+        qsort(keys, (size_t)n, sizeof(skey), cmp_desc_stable);
+        /* kernel2: keep only the status bit; place the first Pleng */
+        for (int32_t v = 0; v < n; ++v) score[v] &= 0x80000000u;
+        for (int32_t i = 0; i < Pl[b] && i < n; ++i) {
+            score[keys[i].id] = 0u;
+            placement[Ps[b] + i] = keys[i].id;
+        }
+        /* kernel3: window 2-hop sibling counts */
+        int32_t olen = Ol[b];
+        if (Os[b] + olen > n) olen = n - Os[b];
+        for (int32_t w = 0; w < olen; ++w) {
+            const int32_t x = (int32_t)placement[Os[b] + w];
+            if (rp[x + 1] - rp[x] >= 10000) continue;
+            for (int32_t a = rp[x]; a < rp[x + 1]; ++a) {
+                const int32_t y = ci[a];
+                if (tp[y + 1] - tp[y] >= 10000) continue;
+                for (int32_t c = tp[y]; c < tp[y + 1]; ++c) {
+                    const int32_t z = ti[c];
+                    if (score[z] >> 31) score[z] += 1u;
+                }
+            }
+        }
+    }
+    free(Ps); free(Pl); free(Os); free(Ol); free(score); free(keys);
+    return B;
+}
+
+/* ------------------------------------------------------------------------ */
+void oracle_order_on_cpu(int32_t n, const int32_t* rp, const int32_t* ci,
+                         const int32_t* tp, const int32_t* ti, int32_t* batch_of,
+                         uint32_t* slice, int32_t batch_id, int32_t len,
+                         int32_t window, int32_t degree)
+{
+    (void)n;
+    if (len <= 0) return;
+    /* start: first vertex of the slice with strictly maximal out-list length
+     * (offset1 = transpose of M) */
+    int32_t start = (int32_t)slice[0];
+    int32_t deg = tp[start + 1] - tp[start];
+    for (int32_t i = 0; i < len; ++i) {
+        const int32_t v = (int32_t)slice[i];
+        if (tp[v + 1] - tp[v] > deg) { deg = tp[v + 1] - tp[v]; start = v; }
+    }
+    /* local node ids = slice positions; heap over local ids */
+    int32_t* local = (int32_t*)malloc(sizeof(int32_t) * (size_t)len);
+    uint8_t* sts = (uint8_t*)malloc((size_t)len);
+    for (int32_t i = 0; i < len; ++i) local[i] = (int32_t)slice[i];
+    oheap h;
+    oheap_init(&h, len, (int64_t)window * len * 8 + 64);
+    /* position lookup through batch_of: batch_of[v] = batch id, and a map
+     * vertex -> local index kept in a side table */
+    int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)len);
+    for (int32_t i = 0; i < len; ++i) {
+        batch_of[local[i]] = batch_id;
+        sts[i] = 1;
+        pos[i] = i;
+        oheap_push(&h, i);
+    }
+    (void)pos;
+    /* map vertex -> local index: a sorted copy for bsearch */
+    int32_t* sorted = (int32_t*)malloc(sizeof(int32_t) * (size_t)len * 2);
+    for (int32_t i = 0; i < len; ++i) { sorted[2 * i] = local[i]; sorted[2 * i + 1] = i; }
+    /* insertion sort by vertex (len <= W) */
+    for (int32_t i = 1; i < len; ++i) {
+        int32_t kv = sorted[2 * i], ki = sorted[2 * i + 1], j = i - 1;
+        while (j >= 0 && sorted[2 * j] > kv) {
+            sorted[2 * (j + 1)] = sorted[2 * j];
+            sorted[2 * (j + 1) + 1] = sorted[2 * j + 1];
+            --j;
+        }
+        sorted[2 * (j + 1)] = kv; sorted[2 * (j + 1) + 1] = ki;
+    }
+#define LOCAL_OF(v, out) do { int32_t lo_ = 0, hi_ = len - 1; out = -1;          \
+        while (lo_ <= hi_) { int32_t mid_ = (lo_ + hi_) >> 1;                    \
+            if (sorted[2 * mid_] == (v)) { out = sorted[2 * mid_ + 1]; break; } \
+            if (sorted[2 * mid_] < (v)) lo_ = mid_ + 1; else hi_ = mid_ - 1; }  \
+    } while (0)
+    {
+        int32_t ls; LOCAL_OF(start, ls);
+        oheap_modify(&h, ls, 1);
+    }
+    for (int32_t id = 0; h.cnt > 0; ++id) {
+        const int32_t li = oheap_pop(&h);
+        sts[li] = 0;
+        const int32_t node = local[li];
+        slice[id] = (uint32_t)node;
+        /* increase: siblings reached through the node's in-list */
+        if (rp[node + 1] - rp[node] < degree) {
+            for (int32_t a = rp[node]; a < rp[node + 1]; ++a) {
+                const int32_t y = ci[a];
+                if (tp[y + 1] - tp[y] < degree) {
+                    for (int32_t c = tp[y]; c < tp[y + 1]; ++c) {
+                        const int32_t z = ti[c];
+                        if (batch_of[z] == batch_id) {
+                            int32_t lz; LOCAL_OF(z, lz);
+                            if (sts[lz]) oheap_modify(&h, lz, 1);
+                        }
+                    }
+                }
+            }
+        }
+        /* decrease: the vertex leaving the window */
+        if (id >= window) {
+            const int32_t old = (int32_t)slice[id - window];
+            if (rp[old + 1] - rp[old] < degree) {
+                for (int32_t a = rp[old]; a < rp[old + 1]; ++a) {
+                    const int32_t y = ci[a];
+                    if (tp[y + 1] - tp[y] < degree) {
+                        for (int32_t c = tp[y]; c < tp[y + 1]; ++c) {
+                            const int32_t z = ti[c];
+                            if (batch_of[z] == batch_id) {
+                                int32_t lz; LOCAL_OF(z, lz);
+                                if (sts[lz]) oheap_modify(&h, lz, -1);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+#undef LOCAL_OF
+    oheap_free(&h);
+    free(local); free(sts); free(pos); free(sorted);
+}
+
+int oracle_pgorder(int32_t n, const int32_t* rp, const int32_t* ci, int32_t W,
+                   int32_t window, uint32_t* placement_gpu, uint32_t* placement_final,
+                   int32_t* perm)
+{
+    int32_t* tp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* ti = (int32_t*)malloc(sizeof(int32_t) * ((size_t)rp[n] + 1));
+    oracle_transpose(n, n, rp, ci, tp, ti);
+    const int32_t B = oracle_pgorder_gpu_loop(n, rp, ci, tp, ti, W, placement_gpu);
+    memcpy(placement_final, placement_gpu, sizeof(uint32_t) * (size_t)n);
+    const int32_t cap = B + 1;
+    int32_t *Ps = malloc(sizeof(int32_t) * cap), *Pl = malloc(sizeof(int32_t) * cap);
+    int32_t *Os = malloc(sizeof(int32_t) * cap), *Ol = malloc(sizeof(int32_t) * cap);
+    oracle_pgorder_schedule(n, W, cap, Ps, Pl, Os, Ol);
+    int32_t* batch_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    for (int32_t v = 0; v < n; ++v) batch_of[v] = -1;
+    const int32_t L = (int32_t)(sqrt((double)n) / 4);
+    const int32_t first = (int32_t)log2((double)W);
+    for (int32_t b = first; b < B; ++b)
+        oracle_order_on_cpu(n, rp, ci, tp, ti, batch_of, &placement_final[Os[b]], b,
+                            Ol[b], window, L);
+    for (int32_t i = 0; i < n; ++i) perm[placement_final[i]] = i;
+    free(tp); free(ti); free(Ps); free(Pl); free(Os); free(Ol); free(batch_of);
+    return B;
+}
+
+/* ------------------------------------------------------------------------ */
+int oracle_gorder1(int32_t n, const int32_t* rp, const int32_t* ci, int32_t window,
+                   uint32_t* order, int32_t* perm)
+{
+    if (n <= 0) return 0;
+    int32_t* tp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* ti = (int32_t*)malloc(sizeof(int32_t) * ((size_t)rp[n] + 1));
+    oracle_transpose(n, n, rp, ci, tp, ti);
+    const int32_t L = (int32_t)(sqrt((double)n) / 4);
+    /* start vertex: first maximum of the SpMV row length (offset2) */
+    int32_t start = 0;
+    for (int32_t v = 1; v < n; ++v)
+        if (rp[v + 1] - rp[v] > rp[start + 1] - rp[start]) start = v;
+    uint8_t* sts = (uint8_t*)malloc((size_t)n);
+    oheap h;
+    oheap_init(&h, n, (int64_t)rp[n] + 64);
+    for (int32_t v = 0; v < n; ++v) { sts[v] = 1; oheap_push(&h, v); }
+    oheap_modify(&h, start, 1);
+    for (int32_t id = 0; h.cnt > 0; ++id) {
+        int32_t node = oheap_pop(&h);
+        sts[node] = 0;
+        order[id] = (uint32_t)node;
+        for (int pass = 0; pass < 2; ++pass) {
+            int32_t delta;
+            if (pass == 0) { delta = 1; }
+            else {
+                if (id < window) break;
+                node = (int32_t)order[id - window];
+                delta = -1;
+            }
+            /* neighbours through the out-list of node */
+            if (tp[node + 1] - tp[node] < L)
+                for (int32_t c = tp[node]; c < tp[node + 1]; ++c)
+                    if (sts[ti[c]]) oheap_modify(&h, ti[c], delta);
+            /* in-neighbours y, and siblings through y's out-list */
+            for (int32_t a = rp[node]; a < rp[node + 1]; ++a) {
+                const int32_t y = ci[a];
+                if (sts[y]) oheap_modify(&h, y, delta);
+                if (tp[y + 1] - tp[y] < L)
+                    for (int32_t c = tp[y]; c < tp[y + 1]; ++c)
+                        if (sts[ti[c]]) oheap_modify(&h, ti[c], delta);
+            }
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) perm[order[i]] = i;
+    oheap_free(&h);
+    free(tp); free(ti); free(sts);
+    return 0;
+}
+
+int64_t oracle_gorder1_score(int32_t n, const int32_t* rp, const int32_t* ci,
+                             const int32_t* tp, const int32_t* ti, const uint32_t* order,
+                             int32_t id, int32_t window, int32_t z)
+{
+    /* score(z) after `id` pops = sum over window members u = order[id-window..id)
+     * of: [collen(u) < L] * #(u -> z out-edges) + #(y in cols(u) with y == z)
+     *     + sum_{y in cols(u), collen(y) < L} #(z in out-list of y);
+     * plus 1 for the start vertex (never undone). */
+    const int32_t L = (int32_t)(sqrt((double)n) / 4);
+    int64_t s = 0;
+    int32_t lo = id - window; if (lo < 0) lo = 0;
+    for (int32_t w = lo; w < id; ++w) {
+        const int32_t u = (int32_t)order[w];
+        if (tp[u + 1] - tp[u] < L)
+            for (int32_t c = tp[u]; c < tp[u + 1]; ++c) s += (ti[c] == z);
+        for (int32_t a = rp[u]; a < rp[u + 1]; ++a) {
+            const int32_t y = ci[a];
+            s += (y == z);
+            if (tp[y + 1] - tp[y] < L)
+                for (int32_t c = tp[y]; c < tp[y + 1]; ++c) s += (ti[c] == z);
+        }
+    }
+    return s;
+}

@@ -1,0 +1,140 @@
+"""C-ABI checks that need no GPU: libmr.so loads, exports every entry point
+include/mr.h declares, and its host-side logic (generators, CSR assembly,
+width rule, predictor, exact RD, argument validation) matches the oracle."""
+from __future__ import annotations
+
+import json
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import GOLDEN, ROOT
+
+
+def declared_functions():
+    text = (ROOT / "include" / "mr.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char\s*\*)\s+(mr_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol(mr):
+    names = declared_functions()
+    assert len(names) >= 40
+    L = mr.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the binding covers the whole header
+    assert set(names) <= set(mr.SIGNATURES)
+
+
+def test_no_device_is_reported_not_faked(mr):
+    if mr.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(mr.MRError) as e:
+        mr.Ctx(0)
+    assert e.value.code == 9   # MR_ERR_NO_DEVICE
+
+
+def test_rmat_generator_is_deterministic_and_shaped(mr):
+    a = mr.gen_rmat(12, 16, seed=5)
+    b = mr.gen_rmat(12, 16, seed=5)
+    c = mr.gen_rmat(12, 16, seed=6)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[1], c[1])
+    rp, ci = a
+    n = 1 << 12
+    assert rp[0] == 0 and (np.diff(rp) >= 0).all() and ci.min() >= 0 and ci.max() < n
+    for i in range(0, n, 97):   # sorted unique columns per row
+        row = ci[rp[i]:rp[i + 1]]
+        assert (np.diff(row) > 0).all()
+    # duplicates merged: nnz a bit below edge_factor * n
+    assert 0.8 * 16 * n < len(ci) < 16 * n
+
+
+def test_rmat_quadrant_probabilities(mr):
+    # without scrambling the top bit of (src, dst) follows (a, b, c, d)
+    import ctypes as C
+    src = np.empty(16 << 14, np.int32)
+    dst = np.empty_like(src)
+    mr.check(mr.lib().mr_gen_rmat_edges(14, 16, 0.57, 0.19, 0.19, 9, 0, mr.ptr(src, C.c_int32),
+                                        mr.ptr(dst, C.c_int32)))
+    hs, hd = src >> 13, dst >> 13
+    frac = np.array([((hs == 0) & (hd == 0)).mean(), ((hs == 0) & (hd == 1)).mean(),
+                     ((hs == 1) & (hd == 0)).mean(), ((hs == 1) & (hd == 1)).mean()])
+    assert np.allclose(frac, [0.57, 0.19, 0.19, 0.05], atol=0.01)
+
+
+def test_coo_to_csr_matches_scipy(mr):
+    rng = np.random.default_rng(21)
+    n = 300
+    src = rng.integers(0, n, 5000).astype(np.int32)
+    dst = rng.integers(0, n, 5000).astype(np.int32)
+    rp, ci = mr.coo_to_csr(n, src, dst, dedup=True)
+    A = sp.coo_matrix((np.ones(len(src)), (src, dst)), shape=(n, n)).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    assert np.array_equal(rp, A.indptr) and np.array_equal(ci, A.indices)
+
+
+def test_stencil_and_banded(mr):
+    rp, ci = mr.gen_stencil27(5, 4, 3)
+    n = 60
+    assert len(rp) == n + 1 and len(ci) == 13 * 10 * 7
+    A = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(n, n))
+    assert (A != A.T).nnz == 0                       # symmetric pattern
+    assert np.diff(rp).max() == 27 and np.diff(rp).min() == 8
+    rp, ci = mr.gen_banded(500, 20, 7, seed=3)
+    rows = np.repeat(np.arange(500), np.diff(rp))
+    assert (np.abs(ci - rows) <= 20).all() and (np.diff(rp) == 7).all()
+    assert all(i in ci[rp[i]:rp[i + 1]] for i in range(500))
+
+
+def test_value_generators(mr):
+    u = mr.gen_values(3, 100000, 0, -1.0, 1.0)
+    assert u.min() >= -1 and u.max() <= 1 and abs(u.mean()) < 0.01
+    v = mr.gen_values(3, 100000, 1, 1e-3, 1.0)
+    mag = np.abs(v)
+    assert mag.min() >= 1e-3 * (1 - 1e-12) and mag.max() <= 1.0
+    assert abs(np.log10(mag).mean() + 1.5) < 0.02 and abs((v > 0).mean() - 0.5) < 0.01
+    assert np.array_equal(mr.gen_values(3, 1000, 0), u[:1000])   # counter based
+
+
+@pytest.mark.parametrize("n,nnz", [(16384, 228000), (4194304, 65_000_000), (1 << 24, 4 << 24), (1000, 10)])
+def test_width_rule_matches_oracle(mr, oracle, n, nnz):
+    assert mr.get_width(n, nnz) == oracle.get_width(n, nnz)
+
+
+def test_predictor_matches_oracle_and_reference(mr, oracle):
+    for r in json.loads((GOLDEN / "predict_ref.json").read_text()):
+        assert mr.predict(r["feat"], mr.MODEL_MERBIT) == r["merbit"]
+        assert mr.predict(r["feat"], mr.MODEL_CSR) == r["csr"]
+    with pytest.raises(mr.MRError) as e:
+        mr.predict([np.nan] + [0.0] * 9)
+    assert e.value.code == 10
+
+
+def test_exact_rd_matches_oracle(mr, oracle):
+    rng = np.random.default_rng(22)
+    for n, nnz in ((500, 3000), (40000, 60000)):
+        stream = rng.integers(0, n, nnz).astype(np.int32)
+        counts, feat, st = mr.rd_exact_stream(stream, n)
+        oc, reuses, prob = oracle.rd_exact(stream, n)
+        assert counts.tolist() == oc.tolist() and st["reuses"] == reuses
+        assert np.array_equal(feat, prob)
+
+
+def test_rd_parameter_validation(mr):
+    s = np.zeros(10, np.int32)
+    with pytest.raises(mr.MRError) as e:
+        mr.rd_exact_stream(s, 10, mr.rd_params(sector_size=0))
+    assert e.value.code == 1
+    with pytest.raises(mr.MRError):
+        mr.rd_exact_stream(s, 10, mr.rd_params(rd_cap=100))   # below the last edge
+
+
+def test_check_perm(mr):
+    assert mr.check_perm(np.arange(50))
+    assert not mr.check_perm(np.array([0, 2, 2]))
+    assert not mr.check_perm(np.array([0, 5]))

@@ -1,0 +1,139 @@
+"""Reordering on the device against the oracle, bit for bit: the relabel
+P A P^T (row/column permutation with sorted columns), the pGOrder GPU
+placement loop and its per-batch CPU Gorder phase, CPU Gorder (gorder1) and
+RCM, and the GRAPH-style workflow (reorder, multiply, un-permute)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import csr_rows, random_csr, rmat_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def relabel_cases():
+    rng = np.random.default_rng(31)
+    out = []
+    rp, ci = random_csr(rng, 500, empty_frac=0.3, max_row=80)
+    out.append(("random_500", rp, ci))
+    n = 12000
+    rows = []
+    for i in range(n):
+        if i == 7:
+            rows.append(np.sort(rng.choice(n, 10000, replace=False)))   # > 8192: global bitonic
+        elif i % 1000 == 3:
+            rows.append(np.sort(rng.choice(n, 3000, replace=False)))    # LDS bitonic
+        else:
+            rows.append(np.sort(rng.choice(n, int(rng.integers(0, 70)), replace=False)))
+    out.append(("long_rows_12000", *csr_rows(rows, n)))
+    out.append(("rmat_s14", *rmat_csr(14, 16, seed=1)))
+    return out
+
+
+@pytest.mark.parametrize("case", relabel_cases(), ids=lambda c: c[0])
+def test_permute_bit_exact(ctx, mr, oracle, case):
+    name, rp, ci = case
+    n = len(rp) - 1
+    rng = np.random.default_rng(32)
+    val = rng.uniform(-1, 1, len(ci))
+    perm = rng.permutation(n).astype(np.int32)
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F64)
+    mat.permute(perm)
+    grp, gci, gval = mat.export()
+    orp, oci, oval = oracle.permute(n, rp, ci, val, perm)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci) and np.array_equal(gval, oval)
+    assert np.array_equal(mat.perm(), perm)
+    # composition: a second relabel accumulates perm[old] = new
+    p2 = rng.permutation(n).astype(np.int32)
+    mat.permute(p2)
+    assert np.array_equal(mat.perm(), p2[perm])
+    mat.close()
+
+
+def test_permute_rejects_non_permutation(ctx, mr):
+    rp, ci = random_csr(np.random.default_rng(1), 10)
+    mat = mr.Mat.from_csr(ctx, rp, ci)
+    with pytest.raises(mr.MRError) as e:
+        mat.permute(np.zeros(10, np.int32))
+    assert e.value.code == 4
+    mat.close()
+
+
+def pgorder_cases():
+    rng = np.random.default_rng(41)
+    out = [("rmat_s14_w1024", *rmat_csr(14, 16, seed=1), 1024)]
+    rp, ci = random_csr(rng, 3000, empty_frac=0.4, max_row=12)
+    out.append(("random_3000_w64", rp, ci, 64))
+    rp, ci = random_csr(rng, 700, empty_frac=0.1, max_row=30)
+    out.append(("random_700_w32", rp, ci, 32))
+    out.append(("rmat_s12_w128", *rmat_csr(12, 8, seed=4), 128))
+    rp, ci = random_csr(rng, 20, empty_frac=0.2, max_row=5)
+    out.append(("tiny_20_w32", rp, ci, 32))
+    out.append(("no_edges_5000", np.zeros(5001, np.int32), np.zeros(0, np.int32), 1024))
+    # hubs above the 10000 cap of the GPU window score
+    n = 30000
+    rows = [np.sort(rng.choice(n, 12000, replace=False)) if i in (5, 9000) else
+            np.sort(rng.choice(n, int(rng.integers(0, 6)), replace=False)) for i in range(n)]
+    out.append(("hubs_30000_w1024", *csr_rows(rows, n), 1024))
+    return out
+
+
+@pytest.mark.parametrize("case", pgorder_cases(), ids=lambda c: c[0])
+def test_pgorder_bit_exact(ctx, mr, oracle, case):
+    name, rp, ci, W = case
+    n = len(rp) - 1
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    gpu_order, final_order, st = mat.pgorder_order(W, 5)
+    og, of, operm = oracle.pgorder(n, rp, ci, W, 5)
+    assert np.array_equal(gpu_order, og), "GPU placement loop differs"
+    assert np.array_equal(final_order, of), "CPU Gorder phase differs"
+    perm, st2 = mat.reorder_pgorder(W, 5)
+    assert np.array_equal(perm, operm)
+    grp, gci, _ = mat.export()
+    orp, oci, _ = oracle.permute(n, rp, ci, None, operm)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci)
+    mat.close()
+
+
+@pytest.mark.parametrize("scale,ef", [(10, 8), (12, 16)])
+def test_gorder_cpu_bit_exact(ctx, mr, oracle, scale, ef):
+    rp, ci = rmat_csr(scale, ef, seed=2)
+    n = len(rp) - 1
+    mat = mr.Mat.from_csr(ctx, rp, ci)
+    perm, _ = mat.reorder_gorder_cpu(5)
+    _, operm = oracle.gorder1(n, rp, ci, 5)
+    assert np.array_equal(perm, operm)
+    mat.close()
+
+
+def test_rcm_bit_exact(ctx, mr, oracle):
+    rng = np.random.default_rng(51)
+    for n in (30, 400):
+        rp, ci = random_csr(rng, n, empty_frac=0.2, max_row=6)
+        mat = mr.Mat.from_csr(ctx, rp, ci)
+        perm, _ = mat.reorder_rcm()
+        # the reference hands RCM its COO sorted by (col, row) = rows of M in order:
+        # COO row_ind = column of M, col_ind = row of M
+        rows = ci.copy()
+        cols = np.repeat(np.arange(n, dtype=np.int32), np.diff(rp))
+        order = oracle.rcm(n, rows, cols)
+        want = np.empty(n, np.int32)
+        want[order] = np.arange(n, dtype=np.int32)
+        assert np.array_equal(perm, want)
+        mat.close()
+
+
+def test_graph_workflow_unpermutes(ctx, mr, oracle):
+    rp, ci = rmat_csr(13, 16, seed=3)
+    n = len(rp) - 1
+    rng = np.random.default_rng(61)
+    val = rng.uniform(-1, 1, len(ci))
+    x = rng.uniform(-1, 1, n)
+    g = mr.Graph(ctx, mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F64))
+    g.gorder6()
+    yhat, absrow = oracle.spmv_f64(rp, ci, val, x)
+    for method in ("merbit", "csr", "csr_scalar"):
+        y = g.spmv(x, method)
+        worst, l2, _ = oracle.spmv_parity(y, yhat, absrow)
+        assert worst <= 1e-12 and l2 <= 1e-12, (method, worst)

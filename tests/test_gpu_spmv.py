@@ -1,0 +1,142 @@
+"""SpMV/SpMM on the device through the C-ABI against the FP64 oracle:
+MERBIT (all tile widths), CSR scalar and CSR vector, FP64 at 1e-12
+(condition-aware: max_i |y_i - yhat_i| / sum_j |a_ij x_j|, and relative L2),
+FP32 at 1e-5; MERBIT metadata bit-exact; results bit-reproducible."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import spmv_cases
+
+pytestmark = pytest.mark.gpu
+
+CASES = spmv_cases()
+IDS = [c[0] for c in CASES]
+
+
+def spmv(ctx, mr, mat, x, kernel):
+    xd = ctx.upload(x)
+    yd = ctx.empty(mat.info()[0], x.dtype)
+    mat.spmv(xd, yd, kernel)
+    ctx.sync()
+    return yd.to_host()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("kernel", ["merbit", "csr_scalar", "csr_vector"])
+def test_spmv_f64_parity(ctx, mr, oracle, case, kernel):
+    name, rp, ci, nc = case
+    rng = np.random.default_rng(hash(name) % 1000)
+    val = rng.uniform(-1, 1, len(ci))
+    x = rng.uniform(-1, 1, nc)
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=mr.F64)
+    k = {"merbit": mr.MERBIT, "csr_scalar": mr.CSR_SCALAR, "csr_vector": mr.CSR_VECTOR}[kernel]
+    y = spmv(ctx, mr, mat, x, k)
+    yhat, absrow = oracle.spmv_f64(rp, ci, val, x)
+    worst, l2, _ = oracle.spmv_parity(y, yhat, absrow)
+    assert worst <= 1e-12 and l2 <= 1e-12, (worst, l2)
+    mat.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_merbit_f32_all_sigmas(ctx, mr, oracle, case, sigma):
+    name, rp, ci, nc = case
+    rng = np.random.default_rng(11)
+    val = rng.uniform(-1, 1, len(ci))
+    x = rng.uniform(-1, 1, nc)
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=mr.F32)
+    mat.merbit_configure(sigma)
+    y = spmv(ctx, mr, mat, x.astype(np.float32), mr.MERBIT)
+    yhat, absrow = oracle.spmv_f64(rp, ci, val.astype(np.float32).astype(np.float64),
+                                   x.astype(np.float32).astype(np.float64))
+    worst, l2, _ = oracle.spmv_parity(y, yhat, absrow)
+    assert worst <= 1e-5 and l2 <= 1e-5, (worst, l2)
+    mat.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_merbit_metadata_bit_exact(ctx, mr, oracle, case):
+    name, rp, ci, nc = case
+    mat = mr.Mat.from_csr(ctx, rp, ci, None, n_cols=nc, store=mr.F32)
+    for sigma in (8, 16, 32):
+        mat.merbit_configure(sigma)
+        flags, bx, by, info = mat.merbit_export()
+        n, nnz = len(rp) - 1, len(ci)
+        T = 256 * sigma
+        assert info["items"] == n + nnz and info["n_blocks"] == -(-(n + nnz) // T)
+        bits = np.zeros(len(flags) * 32, np.uint8)
+        bits[(rp[1:].astype(np.int64) + np.arange(n))] = 1        # item rp[r+1] + r ends row r
+        want = np.packbits(bits.reshape(-1, 32)[:, ::-1], axis=1, bitorder="big").view(">u4").ravel()
+        assert np.array_equal(flags, want.astype(np.uint32))
+        for b in range(info["n_blocks"] + 1):
+            d = min(b * T, n + nnz)
+            assert (bx[b], by[b]) == oracle.merge_search(rp, nnz, d)
+    mat.close()
+
+
+def test_merbit_bit_reproducible_and_workspace(ctx, mr):
+    name, rp, ci, nc = CASES[0]
+    rng = np.random.default_rng(3)
+    mat = mr.Mat.from_csr(ctx, rp, ci, rng.uniform(-1, 1, len(ci)), n_cols=nc, store=mr.F32)
+    x = rng.uniform(-1, 1, nc).astype(np.float32)
+    y1 = spmv(ctx, mr, mat, x, mr.MERBIT)
+    y2 = spmv(ctx, mr, mat, x, mr.MERBIT)
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+    ws = ctx.empty(mat.spmv_ws_bytes(), np.uint8)
+    xd = ctx.upload(x)
+    yd = ctx.empty(len(y1), np.float32)
+    mat.spmv_ws(xd, yd, ws)
+    ctx.sync()
+    assert np.array_equal(yd.to_host().view(np.uint32), y1.view(np.uint32))
+    mat.close()
+
+
+@pytest.mark.parametrize("k", [2, 4, 8, 32])
+@pytest.mark.parametrize("store", ["f64", "f32"])
+def test_spmm(ctx, mr, oracle, k, store):
+    name, rp, ci, nc = CASES[0]
+    rng = np.random.default_rng(k)
+    val = rng.uniform(-1, 1, len(ci))
+    X = rng.uniform(-1, 1, (nc, k))
+    dt = np.float64 if store == "f64" else np.float32
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=mr.F64 if store == "f64" else mr.F32)
+    Xd = ctx.upload(X.astype(dt))
+    Yd = ctx.empty((len(rp) - 1, k), dt)
+    mat.spmm(k, Xd, Yd, mr.CSR_VECTOR)
+    ctx.sync()
+    Y = Yd.to_host()
+    Yh = oracle.spmm_f64(rp, ci, val.astype(dt).astype(np.float64), X.astype(dt).astype(np.float64))
+    tol = 1e-12 if store == "f64" else 1e-5
+    assert np.max(np.abs(Y - Yh)) <= tol * max(1.0, np.max(np.abs(Yh))) * 16
+    mat.close()
+
+
+def test_value_conversion_rounding(ctx, mr):
+    # values are rounded once from FP64 to the storage type (round to nearest even)
+    rng = np.random.default_rng(5)
+    n = 4096
+    v = np.concatenate([rng.uniform(-1, 1, n - 6), [1 + 2.0 ** -8, 1 + 3 * 2.0 ** -9, 65504.0, 1e-8, -0.0, 1 + 2.0 ** -8 + 2.0 ** -30]])
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.zeros(n, np.int32)
+    m64 = mr.Mat.from_csr(ctx, rp, ci, v, n_cols=1, store=mr.F64)
+    f16 = m64.convert(mr.F16).export()[2]
+    assert np.array_equal(f16.view(np.uint16), v.astype(np.float16).view(np.uint16))
+    bf = m64.convert(mr.BF16).export()[2]
+    # exact single rounding to 8 significant bits, ties to even (np.rint)
+    m, e = np.frexp(v)
+    r = np.ldexp(np.rint(m * 256.0), e - 8)
+    want = (r.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    assert np.array_equal(bf.view(np.uint16), want)
+    for m in (m64,):
+        m.close()
+
+
+def test_malformed_csr_rejected(ctx, mr):
+    with pytest.raises(mr.MRError) as e:
+        mr.Mat.from_csr(ctx, np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32), n_cols=2)
+    assert e.value.code == 6
+    with pytest.raises(mr.MRError) as e:
+        mr.Mat.from_csr(ctx, np.array([0, 1], np.int32), np.array([5], np.int32), n_cols=2)
+    assert e.value.code == 6

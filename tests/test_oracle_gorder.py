@@ -1,0 +1,161 @@
+"""The Gorder-family oracle pinned by hand-derived schedules and by dense
+matrix algebra (numpy products, not the oracle's loops): every batch of the
+pGOrder GPU loop must be the first |P| unplaced vertices by (2-hop window
+count desc, id asc), and every Gorder pop must carry the maximal score."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import random_csr
+
+
+def dense(n, rp, ci):
+    A = np.zeros((n, n), np.int64)
+    for i in range(n):
+        for q in range(rp[i], rp[i + 1]):
+            A[i, ci[q]] += 1
+    return A
+
+
+def test_schedule_hand_derived(oracle):
+    # W = 1024: 32-wide doubling batches, then W-wide batches; O covers the
+    # placed prefix until W, then equals P
+    ps, pl, os_, ol = oracle.pgorder_schedule(5000, 1024)
+    assert len(ps) == 5 + 5   # ceil(5000/1024) + log2(1024/32)
+    assert list(zip(ps[:7], pl[:7])) == [(0, 32), (32, 32), (64, 64), (128, 128), (256, 256), (512, 512), (1024, 1024)]
+    assert list(zip(os_[:7], ol[:7])) == [(0, 32), (0, 64), (0, 128), (0, 256), (0, 512), (0, 1024), (1024, 1024)]
+    assert (ps[9], pl[9], os_[9], ol[9]) == (4096, 904, 4096, 904)
+    # the union of the P ranges is exactly [0, n)
+    cover = np.zeros(5000, int)
+    for s, l in zip(ps, pl):
+        cover[s:s + max(l, 0)] += 1
+    assert (cover == 1).all()
+    ps, pl, os_, ol = oracle.pgorder_schedule(100, 64)
+    assert len(ps) == 2 + 1
+    assert list(zip(ps, pl)) == [(0, 32), (32, 32), (64, 36)]
+    assert list(zip(os_, ol)) == [(0, 32), (0, 64), (64, 36)]
+
+
+def brute_gpu_loop(n, A, W, cap=10000):
+    ps, pl, os_, ol = None, None, None, None
+    import oracle
+    ps, pl, os_, ol = oracle.pgorder_schedule(n, W)
+    rowlen = A.sum(1)
+    collen = A.sum(0)
+    placed = np.zeros(n, bool)
+    placement = np.full(n, -1, np.int64)
+    score = rowlen.copy()
+    for b in range(len(ps)):
+        cand = np.flatnonzero(~placed)
+        order = cand[np.lexsort((cand, -score[cand]))]
+        take = order[: max(min(pl[b], n), 0)]
+        placement[ps[b]: ps[b] + len(take)] = take
+        placed[take] = True
+        win = placement[os_[b]: os_[b] + max(min(ol[b], n - os_[b]), 0)]
+        w = np.zeros(n, np.int64)
+        w[win[rowlen[win] < cap]] = 1
+        c = (collen < cap).astype(np.int64)
+        score = ((w @ A) * c) @ A.T
+        score[placed] = 0
+    return placement
+
+
+@pytest.mark.parametrize("seed,n,W", [(0, 40, 32), (1, 150, 64), (2, 300, 64), (3, 257, 128), (4, 20, 32)])
+def test_gpu_loop_equals_dense_brute_force(oracle, seed, n, W):
+    rng = np.random.default_rng(seed)
+    rp, ci = random_csr(rng, n, max_row=6, empty_frac=0.3)
+    got = oracle.pgorder_gpu_loop(n, rp, ci, W)
+    want = brute_gpu_loop(n, dense(n, rp, ci), W)
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+def test_gpu_loop_ties_and_isolated(oracle):
+    # no edges at all: every count is 0, so placement is the identity
+    n = 100
+    rp = np.zeros(n + 1, np.int32)
+    got = oracle.pgorder_gpu_loop(n, rp, np.zeros(0, np.int32), 32)
+    assert got.tolist() == list(range(n))
+
+
+def gorder1_scores(A, order, pid, window, L):
+    n = A.shape[0]
+    collen = A.sum(0)
+    w = np.zeros(n, np.int64)
+    w[order[max(0, pid - window):pid]] = 1
+    cl = (collen < L).astype(np.int64)
+    return (w * cl) @ A.T + w @ A + ((w @ A) * cl) @ A.T
+
+
+@pytest.mark.parametrize("seed,n", [(5, 60), (6, 200), (7, 121)])
+def test_gorder1_pops_carry_max_score(oracle, seed, n):
+    rng = np.random.default_rng(seed)
+    rp, ci = random_csr(rng, n, max_row=8, empty_frac=0.2)
+    A = dense(n, rp, ci)
+    order, perm = oracle.gorder1(n, rp, ci, 5)
+    assert oracle.check_perm(perm) and np.array_equal(perm[order], np.arange(n))
+    L = int(math.sqrt(n) / 4)
+    rowlen = A.sum(1)
+    assert order[0] == int(np.argmax(rowlen))          # start: first maximal row length
+    placed = np.zeros(n, bool)
+    for pid in range(n):
+        s = gorder1_scores(A, order, pid, 5, L)
+        free = ~placed
+        assert s[order[pid]] == s[free].max(), f"pop {pid}"
+        placed[order[pid]] = True
+
+
+def test_gorder1_empty_graph_hand_worked(oracle):
+    # all scores stay 0: the start (vertex 0) is popped first, then bucket 0
+    # is read from its head: pushes insert right after the head -> n-1, ..., 1
+    n = 7
+    order, _ = oracle.gorder1(n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), 5)
+    assert order.tolist() == [0, 6, 5, 4, 3, 2, 1]
+
+
+@pytest.mark.parametrize("seed,n", [(8, 120), (9, 300)])
+def test_order_on_cpu_pops_carry_max_score(oracle, seed, n):
+    rng = np.random.default_rng(seed)
+    rp, ci = random_csr(rng, n, max_row=10, empty_frac=0.1)
+    A = dense(n, rp, ci)
+    rowlen, collen = A.sum(1), A.sum(0)
+    sl = rng.permutation(n)[: n // 2].astype(np.uint32)
+    degree = max(3, int(math.sqrt(n) / 4) + 2)
+    out = oracle.order_on_cpu(n, rp, ci, sl, 3, 5, degree)
+    assert sorted(out.tolist()) == sorted(sl.tolist())
+    # start: first slice vertex with strictly maximal out-list length
+    first = int(sl[0])
+    for v in sl:
+        if collen[v] > collen[first]:
+            first = int(v)
+    assert out[0] == first
+    inb = np.zeros(n, bool)
+    inb[sl] = True
+    c = (collen < degree).astype(np.int64)
+    placed = np.zeros(n, bool)
+    for pid in range(len(out)):
+        w = np.zeros(n, np.int64)
+        win = out[max(0, pid - 5):pid].astype(np.int64)
+        w[win[rowlen[win] < degree]] = 1
+        s = ((w @ A) * c) @ A.T
+        free = inb & ~placed
+        if pid > 0:
+            assert s[out[pid]] == s[free].max(), f"pop {pid}"
+        placed[out[pid]] = True
+
+
+def test_pgorder_cpu_phase_leaves_prefix(oracle):
+    rng = np.random.default_rng(10)
+    n = 2600
+    rp, ci = random_csr(rng, n, max_row=5, empty_frac=0.3)
+    g, f, perm = oracle.pgorder(n, rp, ci, 64, 5)
+    ps, pl, os_, ol = oracle.pgorder_schedule(n, 64)
+    first = int(math.log2(64))
+    # batches before log2(W) keep the GPU order; later ones are permutations of it
+    assert np.array_equal(f[: os_[first]], g[: os_[first]])
+    for b in range(first, len(ps)):
+        s, l = os_[b], ol[b]
+        assert sorted(f[s:s + l].tolist()) == sorted(g[s:s + l].tolist())
+    assert oracle.check_perm(perm) and np.array_equal(perm[f], np.arange(n))

@@ -1,0 +1,115 @@
+"""Reuse-distance oracle pinned by the reference's own worked example
+(src/test.cu:197-216), by brute force, and by what HyperLogLog must give in
+its exact regimes (0 distinct -> 0, 1 distinct -> bin 1, linear counting)."""
+from __future__ import annotations
+
+import numpy as np
+
+EDGES = np.array([1, 112, 512, 18432, 36864, 73728, 147456, 294912, 589824], np.int32)
+
+
+def test_reference_worked_example(oracle):
+    # stream and sector size of the commented example in src/test.cu: 15 reuse
+    # events; with sector 8 every access but one is to sector 0
+    stream = np.array([0, 1, 2, 0, 3, 4, 5, 0, 1, 2, 3, 4, 0, 6, 7, 8, 0], np.int32)
+    counts, reuses, prob = oracle.rd_exact(stream, 1024, 8, EDGES)
+    assert reuses == 15
+    assert counts.tolist() == [14, 1, 0, 0, 0, 0, 0, 0, 0, 0]
+    rd = oracle.rd_bruteforce(stream, 8)
+    assert (rd >= 0).sum() == 15 and rd.max() == 1
+    assert abs(prob.sum() - 1.0) < 1e-15
+
+
+def test_fenwick_equals_bruteforce(oracle):
+    rng = np.random.default_rng(11)
+    edges = np.array([1, 3, 8, 20], np.int32)
+    for trial in range(12):
+        nnz = int(rng.integers(1, 400))
+        n = int(rng.integers(8, 2000))
+        sector = int(rng.choice([1, 4, 32]))
+        stream = rng.integers(0, n, nnz).astype(np.int32)
+        counts, reuses, prob = oracle.rd_exact(stream, n, sector, edges)
+        rd = oracle.rd_bruteforce(stream, sector)
+        valid = rd[rd >= 0]
+        assert reuses == len(valid)
+        want = np.bincount(np.searchsorted(edges, valid, side="right"), minlength=len(edges) + 1)
+        assert counts.tolist() == want.tolist()
+
+
+def test_hll_exact_regimes(oracle):
+    m = 4096
+    regs = np.zeros(m, np.int32)
+    assert oracle.hll_estimate(regs) == 0.0                  # m ln(m/m)
+    regs[17] = 3
+    e1 = oracle.hll_estimate(regs)                           # one distinct value
+    assert 1.0 <= e1 < 1.01
+    # rank of a hash whose upper bits are zero is the maximal rank 33 - p
+    assert oracle.hll_rank(0x00000fff, 12) == 21
+    assert oracle.hll_rank(0x80000000, 12) == 1
+    assert oracle.hll_rank(0x00001000, 12) == 20
+    # lowbias32 is a bijection: no collisions on 2^16 consecutive keys
+    hs = {oracle.hll_hash(i) for i in range(1 << 16)}
+    assert len(hs) == 1 << 16 and oracle.hll_hash(0) == 0
+
+
+def test_hll_registers_are_order_independent(oracle):
+    rng = np.random.default_rng(12)
+    keys = rng.integers(0, 1 << 20, 3000)
+    def regs_of(ks):
+        r = np.zeros(4096, np.int32)
+        for k in ks:
+            h = oracle.hll_hash(int(k))
+            r[h & 4095] = max(r[h & 4095], oracle.hll_rank(h, 12))
+        return r
+    a = regs_of(keys)
+    b = regs_of(keys[::-1])
+    assert np.array_equal(a, b)
+    est = oracle.hll_estimate(a)
+    true = len(np.unique(keys))
+    assert abs(est - true) < 0.05 * true
+
+
+def test_sampled_matches_bruteforce_distances(oracle):
+    rng = np.random.default_rng(13)
+    n = 4000
+    nnz = 6000
+    # mostly-local stream so that many reuses are short
+    stream = np.clip(np.cumsum(rng.integers(-40, 41, nnz)) % n, 0, n - 1).astype(np.int32)
+    res = oracle.rd_sampled(stream, samples=3000, sector=8)
+    rd = oracle.rd_bruteforce(stream, 8)
+    pos, st, est = res["pos"], res["status"], res["est"]
+    assert pos.min() >= 1 and pos.max() <= nnz - 1
+    for p, s, e in zip(pos, st, est):
+        if rd[p] < 0:
+            assert s == 1
+            continue
+        assert s == 0
+        if rd[p] == 0:
+            assert e == 0.0
+        elif rd[p] == 1:
+            assert 1.0 <= e < 1.01
+        elif rd[p] < 300:
+            assert abs(e - rd[p]) <= 0.05 * rd[p] + 2, (p, rd[p], e)
+    found, cold, large, unres = res["stat4"]
+    assert found + cold + large + unres == 3000 and large == 0 and unres == 0
+    assert res["counts"].sum() == found
+    assert abs(res["prob"].sum() - 1.0) < 1e-12
+
+
+def test_sampled_status2_and_max_scan(oracle):
+    # the target recurs after 5000 distinct sectors: the check after the
+    # first 16 chunks of 256 sees an estimate far above 1.03 * rd_cap
+    n = 6000
+    stream = np.concatenate([[0], np.arange(1, 5001), [0]]).astype(np.int32)
+    edges = np.array([1, 4, 8], np.int32)
+    res = oracle.rd_sampled(stream, samples=20000, sector=1, rd_cap=16, edges=edges)
+    last = len(stream) - 1
+    hits = res["pos"] == last
+    assert hits.any()
+    assert (res["status"][hits] == 2).all()
+    assert (res["est"][hits] > 16 * 1.03).all()
+    # status 2 samples land in the last bin and count as reuses
+    assert res["counts"][-1] >= res["stat4"][2]
+    # max_scan shorter than the gap: unresolved (status 3), excluded
+    res3 = oracle.rd_sampled(stream, samples=20000, sector=1, rd_cap=0, max_scan=100, edges=edges)
+    assert (res3["status"][res3["pos"] == last] == 3).all()
