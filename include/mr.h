@@ -78,6 +78,9 @@ int mr_malloc(mr_ctx ctx, size_t bytes, void** dptr);
 int mr_free(mr_ctx ctx, void* dptr);
 int mr_memcpy(mr_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
 int mr_memset(mr_ctx ctx, void* dptr, int value, size_t bytes);
+/* device-to-device streaming copy kernel (16-byte words; bytes % 16 == 0):
+ * the measured-bandwidth reference of the roofline figures */
+int mr_bench_copy(mr_ctx ctx, void* dst, const void* src, size_t bytes, mr_stream s);
 int mr_stream_create(mr_ctx ctx, mr_stream* out);
 int mr_stream_destroy(mr_ctx ctx, mr_stream s);
 int mr_stream_sync(mr_ctx ctx, mr_stream s);
@@ -232,6 +235,14 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
 /* tile geometry: sigma merge-path items per lane (8, 16 or 32), 256 lanes
  * (4 wave64s) per block; rebuilds the metadata on the device */
 int mr_merbit_configure(mr_mat m, int sigma);
+/* load policy of the MERBIT kernel: bit 0 = non-temporal col/val loads and
+ * y stores, bit 1 = XCD-contiguous block order (default 1); results do not
+ * depend on it */
+int mr_merbit_options(mr_mat m, int policy);
+/* autotuner: times every (sigma, policy) on this matrix (reps launches each,
+ * hipEvents) and keeps the fastest; no reference counterpart (the reference
+ * fixes BLOCK=256, SIGMA=14 at class.cu:1409-1410) */
+int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, float* best_us);
 int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms);
 /* row-end bitmap over the merge path (bit p of word p/32 set when item p ends a
  * row) and the merge coordinates at every block start (+ sentinel) */

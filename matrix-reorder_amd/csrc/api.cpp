@@ -2,6 +2,7 @@
 // matrix handles, SpMV/SpMM dispatch and MERBIT metadata.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -47,7 +48,10 @@ int merbit_build(mr_mat m, hipStream_t s)
         return set_error(MR_ERR_UNSUPPORTED, "merge path of %lld items exceeds int32",
                          (long long)p.items);
     p.n_blocks = (p.items + T - 1) / T;
-    const int64_t words = p.n_blocks * T / 32;
+    // bitmap padded to whole 8192-item blocks: every tile width (sigma 8..32,
+    // the 4096-item SpMM grid) reads inside it
+    const int64_t words = (p.items + 8191) / 8192 * (8192 / 32);
+    p.mm_built = false;
     MR_TRY(p.flags.alloc(sizeof(uint32_t) * (size_t)(words > 0 ? words : 1)));
     MR_TRY(p.blk_x.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
     MR_TRY(p.blk_y.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
@@ -193,6 +197,15 @@ int mr_memset(mr_ctx ctx, void* dptr, int value, size_t bytes)
     MR_TRY(use_device(ctx));
     MR_HIP_TRY(hipMemsetAsync(dptr, value, bytes, ctx->stream));
     MR_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MR_OK;
+}
+
+int mr_bench_copy(mr_ctx ctx, void* dst, const void* src, size_t bytes, mr_stream s)
+{
+    if (!ctx || !dst || !src) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    if (bytes % 16) return set_error(MR_ERR_INVALID_ARG, "bytes must be a multiple of 16");
+    MR_TRY(use_device(ctx));
+    MR_HIP_TRY(k::copy_bytes(dst, src, (int64_t)bytes, ctx->cu_count, stream_of(ctx, s)));
     return MR_OK;
 }
 
@@ -374,6 +387,7 @@ int mr_mat_convert(mr_mat m, int dtype, mr_mat* out)
     c->avg_row = m->avg_row;
     c->perm = m->perm;
     c->merbit.sigma = m->merbit.sigma;
+    c->merbit.policy = m->merbit.policy;
     MR_TRY(c->rp.alloc(m->rp.bytes));
     MR_TRY(c->ci.alloc(m->ci.bytes));
     MR_TRY(c->val.alloc(dtype_size(dtype) * (size_t)(m->nnz + kPad)));
@@ -439,15 +453,28 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
         if (!p.built) MR_TRY(merbit_build(m, st));
         if (m->dtype == MR_F64 && p.sigma > 16)
             return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
-        MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
+        MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, p.policy, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
                                   p.flags.as<uint32_t>(), p.blk_x.as<int32_t>(), p.blk_y.as<int32_t>(), x,
                                   y, p.carry.p, p.carry_row.as<int32_t>(), st));
         return MR_OK;
     }
     case MR_KERNEL_CSR_SCALAR:
     case MR_KERNEL_CSR_VECTOR:
+        if (kernel == MR_KERNEL_CSR_VECTOR && m->n_long < 0) {
+            DevBuf cnt;
+            MR_TRY(cnt.alloc(sizeof(int32_t)));
+            MR_TRY(m->long_rows.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(m->n_rows, 1)));
+            MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(int32_t), st));
+            MR_HIP_TRY(k::csr_long_rows(m->n_rows, m->rp.as<int32_t>(), m->long_rows.as<int32_t>(),
+                                        cnt.as<int32_t>(), st));
+            int32_t nl = 0;
+            MR_HIP_TRY(hipMemcpyAsync(&nl, cnt.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            MR_HIP_TRY(hipStreamSynchronize(st));
+            m->n_long = nl;
+        }
         MR_HIP_TRY(k::csr_spmv(m->dtype, kernel == MR_KERNEL_CSR_VECTOR, m->n_rows, m->rp.as<int32_t>(),
-                               m->ci.as<int32_t>(), m->val.p, x, y, m->avg_row, st));
+                               m->ci.as<int32_t>(), m->val.p, x, y, m->avg_row, m->long_rows.as<int32_t>(),
+                               m->n_long > 0 ? m->n_long : 0, st));
         return MR_OK;
     default:
         return set_error(MR_ERR_INVALID_ARG, "unknown kernel %d", kernel);
@@ -476,7 +503,7 @@ int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s)
     char* base = static_cast<char*>(ws);
     void* carry = base;
     int32_t* crow = reinterpret_cast<int32_t*>(base + ((nb * vec_size(m->dtype) + 255) / 256) * 256);
-    MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
+    MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, p.policy, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
                               p.flags.as<uint32_t>(), p.blk_x.as<int32_t>(), p.blk_y.as<int32_t>(), x, y,
                               carry, crow, st));
     return MR_OK;
@@ -488,10 +515,35 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
     if (k <= 0 || ldx < k || ldy < k) return set_error(MR_ERR_INVALID_ARG, "bad k / leading dimension");
     if ((!X || !Y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null X or Y");
     hipStream_t st = stream_of(m->ctx, s);
-    if (k == 1) return mr_spmv(m, kernel, X, Y, s);
-    (void)kernel;
-    MR_HIP_TRY(k::csr_spmm(m->dtype, m->n_rows, k, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p, X,
-                           ldx, Y, ldy, st));
+    if (k == 1 && ldx == 1 && ldy == 1) return mr_spmv(m, kernel, X, Y, s);
+    if (kernel != MR_KERNEL_MERBIT) {
+        MR_HIP_TRY(k::csr_spmm(m->dtype, m->n_rows, k, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p, X,
+                               ldx, Y, ldy, st));
+        return MR_OK;
+    }
+    if (m->n_rows == 0) return MR_OK;
+    MerbitPlan& p = m->merbit;
+    if (!p.built) MR_TRY(merbit_build(m, st));
+    if (!p.mm_built) {
+        p.mm_blocks = (p.items + 4095) / 4096;
+        MR_TRY(p.mm_x.alloc(sizeof(int32_t) * (size_t)(p.mm_blocks + 1)));
+        MR_TRY(p.mm_y.alloc(sizeof(int32_t) * (size_t)(p.mm_blocks + 1)));
+        MR_TRY(p.mm_carry.alloc(vec_size(m->dtype) * 64 * (size_t)std::max<int64_t>(p.mm_blocks, 1)));
+        MR_TRY(p.mm_carry_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(p.mm_blocks, 1)));
+        MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), 4096, p.mm_blocks, p.mm_x.as<int32_t>(),
+                                    p.mm_y.as<int32_t>(), st));
+        p.mm_built = true;
+    }
+    // column chunks of at most 64 right-hand sides
+    const size_t vb = vec_size(m->dtype);
+    for (int j0 = 0; j0 < k; j0 += 64) {
+        const int kk = std::min(64, k - j0);
+        MR_HIP_TRY(k::merbit_spmm(m->dtype, m->n_rows, m->nnz, p.mm_blocks, m->ci.as<int32_t>(), m->val.p,
+                                  p.flags.as<uint32_t>(), p.mm_x.as<int32_t>(), p.mm_y.as<int32_t>(),
+                                  static_cast<const char*>(X) + vb * (size_t)j0, ldx,
+                                  static_cast<char*>(Y) + vb * (size_t)j0, ldy, kk, p.mm_carry.p,
+                                  p.mm_carry_row.as<int32_t>(), st));
+    }
     return MR_OK;
 }
 
@@ -504,6 +556,62 @@ int mr_merbit_configure(mr_mat m, int sigma)
         return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
     m->merbit.sigma = sigma;
     return merbit_build(m, m->ctx->stream);
+}
+
+int mr_merbit_options(mr_mat m, int policy)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (policy < 0 || policy > 3) return set_error(MR_ERR_INVALID_ARG, "policy must be in [0, 3]");
+    m->merbit.policy = policy;
+    return MR_OK;
+}
+
+int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, float* best_us)
+{
+    MR_TRY(check_mat(m));
+    if (m->n_rows == 0) return MR_OK;
+    if (reps < 1) reps = 5;
+    hipStream_t s = m->ctx->stream;
+    const size_t vb = vec_size(m->dtype);
+    DevBuf x, y;
+    MR_TRY(x.alloc(vb * (size_t)std::max<int64_t>(m->n_cols, 1)));
+    MR_TRY(y.alloc(vb * (size_t)m->n_rows));
+    MR_HIP_TRY(hipMemsetAsync(x.p, 0, x.bytes, s));
+    hipEvent_t e0, e1;
+    MR_HIP_TRY(hipEventCreate(&e0));
+    MR_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bs = m->merbit.sigma, bp = m->merbit.policy;
+    const int sigmas[3] = {8, 16, 32};
+    for (int si = 0; si < 3; ++si) {
+        const int sg = sigmas[si];
+        if (m->dtype == MR_F64 && sg > 16) continue;
+        m->merbit.sigma = sg;
+        MR_TRY(merbit_build(m, s));
+        for (int pol = 0; pol < 4; ++pol) {
+            m->merbit.policy = pol;
+            MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
+            MR_HIP_TRY(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
+            MR_HIP_TRY(hipEventRecord(e1, s));
+            MR_HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            MR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            const float us = ms * 1000.0f / (float)reps;
+            if (us < best) { best = us; bs = sg; bp = pol; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    m->merbit.policy = bp;
+    if (m->merbit.sigma != bs) {
+        m->merbit.sigma = bs;
+        MR_TRY(merbit_build(m, s));
+    }
+    if (best_sigma) *best_sigma = bs;
+    if (best_policy) *best_policy = bp;
+    if (best_us) *best_us = best;
+    return MR_OK;
 }
 
 int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms)

@@ -95,7 +95,9 @@ struct mr_ctx_s {
 
 // MERBIT plan: merge-path tiles of 256 lanes x sigma items
 struct MerbitPlan {
-    int sigma = 16;
+    // defaults = the autotuner's pick on R-MAT s22 after pGOrder (MI355X)
+    int sigma = 8;
+    int policy = 1;          // bit 0 non-temporal streams, bit 1 XCD block remap
     int64_t items = 0;       // n_rows + nnz
     int64_t n_blocks = 0;
     mr::DevBuf flags;        // u32 words, row-end bitmap (n_blocks * 256 * sigma bits)
@@ -103,6 +105,10 @@ struct MerbitPlan {
     mr::DevBuf blk_y;        // int32[n_blocks + 1]
     mr::DevBuf carry;        // accumulator type [n_blocks]
     mr::DevBuf carry_row;    // int32[n_blocks]
+    // SpMM grid: 4096 items per block, k-wide carries (k <= 64)
+    int64_t mm_blocks = 0;
+    mr::DevBuf mm_x, mm_y, mm_carry, mm_carry_row;
+    bool mm_built = false;
     bool built = false;
     double build_ms = 0.0;
 };
@@ -117,6 +123,8 @@ struct mr_mat_s {
     std::vector<int32_t> perm;   // accumulated relabelling perm[old] = new
     MerbitPlan merbit;
     double avg_row = 0.0;
+    mr::DevBuf long_rows;    // rows above 4096 nonzeros (CSR vector kernel)
+    int32_t n_long = -1;     // -1: not computed for the current structure
 };
 
 namespace mr {

@@ -17,22 +17,36 @@ hipError_t fill_ones(void* dst, int dt, int64_t n, hipStream_t s);
 // dir 0: out[perm[j]] = in[j]; dir 1: out[i] = in[perm[i]]; elem 4 or 8 bytes
 hipError_t permute_vec(const int32_t* perm, int64_t n, int dir, const void* in, void* out,
                        int elem, hipStream_t s);
+// 16-byte-word copy (bytes rounded down to a multiple of 16)
+hipError_t copy_bytes(void* dst, const void* src, int64_t bytes, int cu_count, hipStream_t s);
 // CSR validation: returns via flag (0 ok, 1 non-monotone, 2 column out of range)
 hipError_t validate_csr(int64_t n_rows, int64_t n_cols, const int32_t* rp, const int32_t* ci,
                         int64_t nnz, int32_t* flag, hipStream_t s);
 
 // ---- CSR SpMV / SpMM (spmv_csr.hip) ----
+// rows longer than 4096 are listed once (count zeroed by the caller) and
+// multiplied by a workgroup each inside csr_spmv(vector=1)
+hipError_t csr_long_rows(int64_t n_rows, const int32_t* rp, int32_t* list, int32_t* count, hipStream_t s);
 hipError_t csr_spmv(int dt, int vector, int64_t n_rows, const int32_t* rp, const int32_t* ci,
-                    const void* val, const void* x, void* y, double avg_row, hipStream_t s);
+                    const void* val, const void* x, void* y, double avg_row, const int32_t* long_rows,
+                    int32_t n_long, hipStream_t s);
 hipError_t csr_spmm(int dt, int64_t n_rows, int k, const int32_t* rp, const int32_t* ci,
                     const void* val, const void* X, int64_t ldx, void* Y, int64_t ldy,
                     hipStream_t s);
 
 // ---- MERBIT (merbit.hip) ----
+// merge coordinates at every multiple of `items_per_block` (+ sentinel)
+hipError_t merbit_coords(int64_t n_rows, int64_t nnz, const int32_t* rp, int64_t items_per_block, int64_t n_blocks,
+                         int32_t* blk_x, int32_t* blk_y, hipStream_t s);
+// SpMM over 4096-item blocks, k <= 64 right-hand sides (merbit_mm.hip)
+hipError_t merbit_spmm(int dt, int64_t n_rows, int64_t nnz, int64_t n_blocks, const int32_t* ci, const void* val,
+                       const uint32_t* flags, const int32_t* blk_x, const int32_t* blk_y, const void* X,
+                       int64_t ldx, void* Y, int64_t ldy, int kk, void* carry, int32_t* carry_row, hipStream_t s);
 hipError_t merbit_build(int64_t n_rows, int64_t nnz, const int32_t* rp, int sigma,
                         int64_t n_blocks, uint32_t* flags, int64_t flag_words, int32_t* blk_x,
                         int32_t* blk_y, hipStream_t s);
-hipError_t merbit_spmv(int dt, int sigma, int64_t n_rows, int64_t n_blocks, const int32_t* ci,
+// policy bit 0: non-temporal col/val loads and y stores; bit 1: XCD block remap
+hipError_t merbit_spmv(int dt, int sigma, int policy, int64_t n_rows, int64_t n_blocks, const int32_t* ci,
                        const void* val, const uint32_t* flags, const int32_t* blk_x,
                        const int32_t* blk_y, const void* x, void* y, void* carry,
                        int32_t* carry_row, hipStream_t s);

@@ -10,7 +10,8 @@ namespace k {
 struct PgState {
     int32_t tcount[2];   // sizes of the two touched-vertex lists
     int32_t nsure, ntie; // candidate list sizes
-    int32_t tb, above, need, pad;
+    int32_t tb, above, need;
+    int32_t ycount;      // distinct columns listed by the window's first pass
     int32_t lo_chunk;    // first 1024-id chunk that may hold unplaced vertices
     int32_t pad2[7];
 };
@@ -21,8 +22,8 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 // one batch: select |P| (pl) at ps, then score the window [os, os+ol)
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
-                    int32_t* chunk_left, uint32_t* placement, int32_t ps, int32_t pl, int32_t os, int32_t ol,
-                    hipStream_t s);
+                    int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
+                    int32_t ps, int32_t pl, int32_t os, int32_t ol, hipStream_t s);
 
 }  // namespace k
 }  // namespace mr

@@ -100,7 +100,28 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
-template <typename VT, int S>
+// streaming loads/stores: with NT the col/val streams and the y stores carry
+// the non-temporal hint so that they do not displace x from the caches
+template <bool NT, typename T> __device__ __forceinline__ T ld_stream(const T* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT> __device__ __forceinline__ __half ld_stream(const __half* p)
+{
+    return __ushort_as_half(ld_stream<NT>(reinterpret_cast<const unsigned short*>(p)));
+}
+template <bool NT> __device__ __forceinline__ __hip_bfloat16 ld_stream(const __hip_bfloat16* p)
+{
+    return __ushort_as_bfloat16(ld_stream<NT>(reinterpret_cast<const unsigned short*>(p)));
+}
+template <bool NT, typename T> __device__ __forceinline__ void st_stream(T* p, T v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <typename VT, int S, bool NT, bool REMAP>
 __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
     int64_t n_blocks, const int32_t* __restrict__ ci, const VT* __restrict__ val,
     const uint32_t* __restrict__ flags, const int32_t* __restrict__ blk_x,
@@ -115,7 +136,7 @@ __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
     __shared__ int s_wf[kWaves];
     __shared__ int s_wc[kWaves];
 
-    const int64_t b = remap_block(blockIdx.x, n_blocks);
+    const int64_t b = REMAP ? remap_block(blockIdx.x, n_blocks) : (int64_t)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int32_t x0 = blk_x[b], x1 = blk_x[b + 1];
     const int32_t y0 = blk_y[b], y1 = blk_y[b + 1];
@@ -123,17 +144,18 @@ __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
     const int32_t* cib = ci + x0;
     const VT* vb = val + x0;
 
-    if (rowsb == 0) {
-        // every item of the block consumes a nonzero of row y0
-        AT acc = 0;
-        int32_t c[S];
-        VT v[S];
+    // ---- phase 1: the block's nonzeros, all S loads of a lane issued first ---
+    int32_t c[S];
+    VT v[S];
 #pragma unroll
-        for (int u = 0; u < S; ++u) {
-            const int e = u * kLanes + tid;
-            c[u] = e < nnzb ? cib[e] : 0;
-            v[u] = e < nnzb ? vb[e] : VT(0);
-        }
+    for (int u = 0; u < S; ++u) {
+        const int e = u * kLanes + tid;
+        c[u] = e < nnzb ? ld_stream<NT>(cib + e) : 0;
+        v[u] = e < nnzb ? ld_stream<NT>(vb + e) : VT(0);
+    }
+    if (rowsb == 0) {
+        // every item of the block consumes a nonzero of row y0 (fast track)
+        AT acc = 0;
 #pragma unroll
         for (int u = 0; u < S; ++u) {
             const int e = u * kLanes + tid;
@@ -151,18 +173,8 @@ __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
         }
         return;
     }
-
-    // ---- phase 1: products of the block's nonzeros into LDS ---------------
     const int pad_end = T - rowsb;   // products [0,nnzb), zeros up to pad_end
     {
-        int32_t c[S];
-        VT v[S];
-#pragma unroll
-        for (int u = 0; u < S; ++u) {
-            const int e = u * kLanes + tid;
-            c[u] = e < nnzb ? cib[e] : 0;
-            v[u] = e < nnzb ? vb[e] : VT(0);
-        }
         AT xv[S];
 #pragma unroll
         for (int u = 0; u < S; ++u) {
@@ -214,31 +226,31 @@ __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
     }
 
     // ---- segmented scan of (tail, has_end) over the 256 lanes ----------------
-    AT v = acc;
+    AT sv = acc;
     int fl = has_end ? 1 : 0;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const AT vu = __shfl_up(v, d, 64);
+        const AT vu = __shfl_up(sv, d, 64);
         const int fu = __shfl_up(fl, d, 64);
         if (lane >= d) {
-            if (!fl) v += vu;
+            if (!fl) sv += vu;
             fl |= fu;
         }
     }
-    if (lane == 63) { s_wv[wid] = v; s_wf[wid] = fl; }
+    if (lane == 63) { s_wv[wid] = sv; s_wf[wid] = fl; }
     __syncthreads();
     AT pv = 0;
 #pragma unroll
     for (int w = 0; w < kWaves - 1; ++w)
         if (w < wid) pv = s_wf[w] ? s_wv[w] : pv + s_wv[w];
-    const AT incl = fl ? v : pv + v;
+    const AT incl = fl ? sv : pv + sv;
     const AT up = __shfl_up(incl, 1, 64);
     const AT carry_in = lane == 0 ? pv : up;
     if (has_end) sy[head_row] = carry_in + head;
     __syncthreads();
 
     // ---- phase 3: finished rows out, open row to the carry buffer -------------
-    for (int i = tid; i < rowsb; i += kLanes) y[y0 + i] = sy[i];
+    for (int i = tid; i < rowsb; i += kLanes) st_stream<NT>(y + y0 + i, sy[i]);
     if (tid == kLanes - 1) {
         carry[b] = incl;
         carry_row[b] = y1;
@@ -259,13 +271,13 @@ __global__ void merbit_fixup_kernel(int64_t n_blocks, int64_t n_rows, const AT* 
     y[r] = y[r] + s;
 }
 
-template <typename VT, int S>
+template <typename VT, int S, bool NT, bool REMAP>
 hipError_t launch_spmv(int64_t n_rows, int64_t n_blocks, const int32_t* ci, const void* val,
                        const uint32_t* flags, const int32_t* blk_x, const int32_t* blk_y,
                        const void* x, void* y, void* carry, int32_t* carry_row, hipStream_t s)
 {
     using AT = typename Acc<VT>::type;
-    merbit_spmv_kernel<VT, S><<<(unsigned)n_blocks, kLanes, 0, s>>>(
+    merbit_spmv_kernel<VT, S, NT, REMAP><<<(unsigned)n_blocks, kLanes, 0, s>>>(
         n_blocks, ci, static_cast<const VT*>(val), flags, blk_x, blk_y, static_cast<const AT*>(x),
         static_cast<AT*>(y), static_cast<AT*>(carry), carry_row);
     hipError_t e = hipGetLastError();
@@ -273,6 +285,19 @@ hipError_t launch_spmv(int64_t n_rows, int64_t n_blocks, const int32_t* ci, cons
     merbit_fixup_kernel<AT><<<(unsigned)((n_blocks + 255) / 256), 256, 0, s>>>(
         n_blocks, n_rows, static_cast<const AT*>(carry), carry_row, static_cast<AT*>(y));
     return hipGetLastError();
+}
+
+template <typename VT, int S>
+hipError_t launch_policy(int policy, int64_t n_rows, int64_t n_blocks, const int32_t* ci, const void* val,
+                         const uint32_t* flags, const int32_t* blk_x, const int32_t* blk_y, const void* x,
+                         void* y, void* carry, int32_t* carry_row, hipStream_t s)
+{
+    switch (policy & 3) {
+    case 0: return launch_spmv<VT, S, false, false>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+    case 1: return launch_spmv<VT, S, true, false>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+    case 2: return launch_spmv<VT, S, false, true>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+    default: return launch_spmv<VT, S, true, true>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+    }
 }
 
 }  // namespace
@@ -297,7 +322,15 @@ hipError_t merbit_build(int64_t n_rows, int64_t nnz, const int32_t* rp, int sigm
     return hipGetLastError();
 }
 
-hipError_t merbit_spmv(int dt, int sigma, int64_t n_rows, int64_t n_blocks, const int32_t* ci,
+hipError_t merbit_coords(int64_t n_rows, int64_t nnz, const int32_t* rp, int64_t items_per_block, int64_t n_blocks,
+                         int32_t* blk_x, int32_t* blk_y, hipStream_t s)
+{
+    blocks_kernel<<<(unsigned)((n_blocks + 1 + 255) / 256), 256, 0, s>>>(n_rows, nnz, n_rows + nnz, items_per_block,
+                                                                          n_blocks, rp, blk_x, blk_y);
+    return hipGetLastError();
+}
+
+hipError_t merbit_spmv(int dt, int sigma, int policy, int64_t n_rows, int64_t n_blocks, const int32_t* ci,
                        const void* val, const uint32_t* flags, const int32_t* blk_x,
                        const int32_t* blk_y, const void* x, void* y, void* carry,
                        int32_t* carry_row, hipStream_t s)
@@ -306,12 +339,12 @@ hipError_t merbit_spmv(int dt, int sigma, int64_t n_rows, int64_t n_blocks, cons
     return dispatch_dt(dt, [&](auto v) -> hipError_t {
         using VT = decltype(v);
         switch (sigma) {
-        case 8: return launch_spmv<VT, 8>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
-        case 16: return launch_spmv<VT, 16>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+        case 8: return launch_policy<VT, 8>(policy, n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+        case 16: return launch_policy<VT, 16>(policy, n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
         case 32:
             // 256 x 32 doubles would exceed the 64 KB LDS of one workgroup
             if constexpr (sizeof(typename Acc<VT>::type) == 4)
-                return launch_spmv<VT, 32>(n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
+                return launch_policy<VT, 32>(policy, n_rows, n_blocks, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row, s);
             return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
         }

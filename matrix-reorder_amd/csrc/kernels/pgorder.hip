@@ -42,7 +42,9 @@ constexpr int kBins = kExactBins + 16 * 64;
 constexpr int kSmallT = 4096;     // sort T directly at or below this size
 constexpr int kChunk = 1024;      // ids per unplaced counter
 constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
+constexpr int kWindowGrid = 2048; // blocks of the window-score kernel (8192 waves)
 constexpr int kSureCap = 4096;    // capacity of the candidates-above-threshold list
+constexpr int32_t kHubCap = 10000; // gorder6_kernel3's row / column length caps (class.cu:2930,2938)
 
 __device__ __forceinline__ int count_bin(uint32_t c)
 {
@@ -211,7 +213,9 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
                                                     PgState* __restrict__ st, uint32_t* __restrict__ score,
                                                     const int32_t* __restrict__ sure, const int32_t* __restrict__ tie,
                                                     int32_t* __restrict__ hist, int32_t* __restrict__ chunk_left,
-                                                    uint32_t* __restrict__ placement, int32_t ps, int32_t pl)
+                                                    uint32_t* __restrict__ placement, int32_t ps, int32_t pl,
+                                                    const int32_t* __restrict__ rp, int32_t os, int32_t ol,
+                                                    int32_t* __restrict__ wprefix)
 {
     __shared__ uint64_t keys[kSmallT];
     __shared__ int32_t sel[1024];
@@ -384,7 +388,31 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
     if (tid == 0) {
         st->nsure = 0;
         st->ntie = 0;
+        st->ycount = 0;
         st->tcount[cur ^ 1] = 0;
+    }
+    // prefix sums of the (x, y) pair counts of the window O(b): wprefix[w] =
+    // pairs of window entries before w (rows at or above the hub cap count 0)
+    __syncthreads();
+    if (ol > 0) {
+        int32_t* s_len = s_bins;   // reuse: ol <= 1024 entries
+        for (int w = tid; w < ol; w += blockDim.x) {
+            const uint32_t xu = placement[os + w];
+            int32_t len = 0;
+            if (xu < (uint32_t)n) {
+                len = rp[xu + 1] - rp[xu];
+                if (len >= kHubCap) len = 0;
+            }
+            s_len[w] = len;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int32_t run = 0;
+            for (int w = 0; w < ol; ++w) { wprefix[w] = run; run += s_len[w]; }
+            wprefix[ol] = run;
+        }
+    } else if (tid == 0) {
+        wprefix[0] = 0;
     }
 }
 
@@ -398,34 +426,71 @@ __global__ void reset_kernel(const int32_t* __restrict__ T, const PgState* __res
     }
 }
 
-// window score: one workgroup per window vertex x, a wave per y in cols(x),
-// lanes over z in rows(y) (gorder6_kernel3 with wave64)
-__global__ __launch_bounds__(256) void window_kernel(int64_t n, const uint32_t* __restrict__ window,
-                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                                     const int32_t* __restrict__ tp, const int32_t* __restrict__ ti,
-                                                     uint32_t* __restrict__ score, int32_t* __restrict__ T,
-                                                     PgState* __restrict__ st, int nxt)
+// window score (gorder6_kernel3, class.cu:2917-2953).  The reference adds one
+// to every unplaced z for every 2-hop path x -> y -> z (x in the window with a
+// row below the hub cap, y in cols(x) with a column below the cap, z in
+// rows(y)), walking rows(y) once per (x, y) pair.  The sum regroups exactly as
+// count[z] += h[y] * [z in rows(y)] with h[y] = #window rows holding y, so the
+// work is done in two passes: (1) every (x, y) pair adds one to h[y] -- pairs
+// spread over a fixed grid by a prefix sum of the window's row lengths -- and
+// lists y when h[y] leaves 0; (2) one wave per listed y walks rows(y) once and
+// adds h[y].  Integer sums: identical counts, far fewer updates when window
+// rows share columns (hubs).
+__global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t* __restrict__ window, int32_t ol,
+                                                       const int32_t* __restrict__ wprefix,
+                                                       const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                       const int32_t* __restrict__ tp, int32_t* __restrict__ h,
+                                                       int32_t* __restrict__ ylist, PgState* __restrict__ st)
 {
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t xu = window[blockIdx.x];
-    if (xu >= (uint32_t)n) return;   // never taken when every position is placed
-    const int32_t x = (int32_t)xu;
-    const int32_t yb = rp[x], ye = rp[x + 1];
-    if (ye - yb >= 10000) return;
-    for (int32_t a = yb + wid; a < ye; a += 4) {
-        const int32_t y = ci[a];
+    __shared__ int32_t s_pre[1025];
+    for (int i = threadIdx.x; i <= ol; i += blockDim.x) s_pre[i] = wprefix[i];
+    __syncthreads();
+    const int32_t total = s_pre[ol];
+    const int64_t threads = (int64_t)gridDim.x * blockDim.x;
+    const int64_t lim = ((int64_t)total + 63) / 64 * 64;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < lim; p += threads) {
+        bool first = false;
+        int32_t y = 0;
+        if (p < total) {
+            // window entry owning pair p: last w with s_pre[w] <= p
+            int lo = 0, hi = ol - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= p) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t xu = window[lo];
+            if (xu < (uint32_t)n) {   // every position of the window is placed
+                y = ci[rp[xu] + (int32_t)(p - s_pre[lo])];
+                if (tp[y + 1] - tp[y] < kHubCap) first = atomicAdd(&h[y], 1) == 0;
+            }
+        }
+        wave_append(first, y, ylist, &st->ycount);
+    }
+}
+
+__global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict__ tp, const int32_t* __restrict__ ti,
+                                                       int32_t* __restrict__ h, const int32_t* __restrict__ ylist,
+                                                       uint32_t* __restrict__ score, int32_t* __restrict__ T,
+                                                       PgState* __restrict__ st, int nxt)
+{
+    const int32_t ny = st->ycount;
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < ny; q += waves) {
+        const int32_t y = ylist[q];
+        const uint32_t w = (uint32_t)h[y];
         const int32_t zb = tp[y], ze = tp[y + 1];
-        if (ze - zb >= 10000) continue;
         for (int32_t c0 = zb; c0 < ze; c0 += 64) {
             const int32_t c = c0 + lane;
             bool first = false;
             int32_t z = 0;
             if (c < ze) {
                 z = ti[c];
-                if (score[z] & kUnplaced) first = atomicAdd(&score[z], 1u) == kUnplaced;
+                if (score[z] & kUnplaced) first = atomicAdd(&score[z], w) == kUnplaced;
             }
             wave_append(first, z, T, &st->tcount[nxt]);
         }
+        if (lane == 0) h[y] = 0;   // y is listed once: back to 0 for the next window
     }
 }
 
@@ -443,16 +508,20 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
-                    int32_t* chunk_left, uint32_t* placement, int32_t ps, int32_t pl, int32_t os, int32_t ol,
-                    hipStream_t s)
+                    int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
+                    int32_t ps, int32_t pl, int32_t os, int32_t ol, hipStream_t s)
 {
     const int nxt = cur ^ 1;
     hist_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, hist);
     thresh_kernel<<<1, 256, 0, s>>>(hist, st, cur, pl);
     collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl);
-    final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, hist, chunk_left, placement, ps, pl);
+    final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, hist, chunk_left, placement, ps, pl, rp,
+                                   os, ol, wprefix);
     reset_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score);
-    if (ol > 0) window_kernel<<<(unsigned)ol, 256, 0, s>>>(n, placement + os, rp, ci, tp, ti, score, Tl[nxt], st, nxt);
+    if (ol > 0) {
+        window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st);
+        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, h, ylist, score, Tl[nxt], st, nxt);
+    }
     return hipGetLastError();
 }
 

@@ -50,12 +50,16 @@ __device__ __forceinline__ int hll_rank(uint32_t h, int p)
     return r;
 }
 
+// correctly rounded float quotient: the double quotient of two floats rounds
+// to the same float as an exact IEEE float division (53 >= 2*24 + 2)
+__device__ __forceinline__ float div_rn(float a, float b) { return (float)((double)a / (double)b); }
+
 __device__ __forceinline__ float hll_alpha(int m)
 {
     if (m == 16) return 0.673f;
     if (m == 32) return 0.697f;
     if (m == 64) return 0.709f;
-    return 0.7213f / (1.0f + 1.079f / (float)m);
+    return div_rn(0.7213f, 1.0f + div_rn(1.079f, (float)m));
 }
 
 constexpr int kThreads = 256;
@@ -157,8 +161,8 @@ __global__ __launch_bounds__(kThreads) void hll_kernel(const int32_t* __restrict
                     float sum;
                     int zeros;
                     serial(sum, zeros);
-                    float e = alpha * fm * fm / sum;
-                    if (e <= 2.5f * fm && zeros > 0) e = fm * logf(fm / (float)zeros);
+                    float e = div_rn(alpha * fm * fm, sum);
+                    if (e <= 2.5f * fm && zeros > 0) e = fm * logf(div_rn(fm, (float)zeros));
                     s_flag = e > thr ? 1 : 0;
                 }
                 __syncthreads();
@@ -173,8 +177,8 @@ __global__ __launch_bounds__(kThreads) void hll_kernel(const int32_t* __restrict
         int zeros;
         serial(sum, zeros);
         if (status == 2) {
-            est = alpha * fm * fm / sum;
-            if (est <= 2.5f * fm && zeros > 0) est = fm * logf(fm / (float)zeros);
+            est = div_rn(alpha * fm * fm, sum);
+            if (est <= 2.5f * fm && zeros > 0) est = fm * logf(div_rn(fm, (float)zeros));
         }
         s_est = est;
         out_sum[sid] = sum;

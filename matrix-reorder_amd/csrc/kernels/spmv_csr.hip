@@ -15,6 +15,8 @@ namespace k {
 
 namespace {
 
+constexpr int32_t kLongRow = 4096;   // rows above this go to a workgroup of their own
+
 template <typename VT>
 __global__ __launch_bounds__(256) void csr_scalar_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
                                                          const int32_t* __restrict__ ci,
@@ -42,8 +44,9 @@ __global__ __launch_bounds__(256) void csr_vector_kernel(int64_t n_rows, const i
     const int lane = threadIdx.x & (G - 1);
     const int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
     if (r >= n_rows) return;   // whole groups leave together
-    AT s = 0;
     const int32_t e = rp[r + 1];
+    if (e - rp[r] > kLongRow) return;   // written by csr_long_row_kernel
+    AT s = 0;
     for (int32_t k = rp[r] + lane; k < e; k += G) s += (AT)to_f(val[k]) * x[ci[k]];
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
@@ -72,6 +75,34 @@ __global__ __launch_bounds__(256) void csr_spmm_kernel(int64_t n_rows, int kk, c
     }
 }
 
+// one workgroup per row longer than kLongRow: 256 lanes stride the row, then a
+// fixed-order reduction (wave xor tree, waves summed in order)
+template <typename VT>
+__global__ __launch_bounds__(256) void csr_long_row_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci, const VT* __restrict__ val,
+                                                           const typename Acc<VT>::type* __restrict__ x,
+                                                           typename Acc<VT>::type* __restrict__ y)
+{
+    using AT = typename Acc<VT>::type;
+    __shared__ AT s_w[4];
+    const int32_t r = rows[blockIdx.x];
+    const int32_t b = rp[r], e = rp[r + 1];
+    AT s = 0;
+    for (int32_t k = b + (int32_t)threadIdx.x; k < e; k += 256) s += (AT)to_f(val[k]) * x[ci[k]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) y[r] = ((s_w[0] + s_w[1]) + s_w[2]) + s_w[3];
+}
+
+__global__ void long_rows_kernel(int64_t n_rows, const int32_t* __restrict__ rp, int32_t* __restrict__ list,
+                                 int32_t* __restrict__ count)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * blockDim.x)
+        if (rp[r + 1] - rp[r] > kLongRow) list[atomicAdd(count, 1)] = (int32_t)r;
+}
+
 template <typename VT, int G>
 hipError_t launch_vector(int64_t n_rows, const int32_t* rp, const int32_t* ci, const void* val,
                          const void* x, void* y, hipStream_t s)
@@ -86,8 +117,18 @@ hipError_t launch_vector(int64_t n_rows, const int32_t* rp, const int32_t* ci, c
 
 }  // namespace
 
+hipError_t csr_long_rows(int64_t n_rows, const int32_t* rp, int32_t* list, int32_t* count, hipStream_t s)
+{
+    if (n_rows <= 0) return hipSuccess;
+    int64_t g = (n_rows + 255) / 256;
+    if (g > 4096) g = 4096;
+    long_rows_kernel<<<(unsigned)g, 256, 0, s>>>(n_rows, rp, list, count);
+    return hipGetLastError();
+}
+
 hipError_t csr_spmv(int dt, int vector, int64_t n_rows, const int32_t* rp, const int32_t* ci,
-                    const void* val, const void* x, void* y, double avg_row, hipStream_t s)
+                    const void* val, const void* x, void* y, double avg_row, const int32_t* long_rows,
+                    int32_t n_long, hipStream_t s)
 {
     if (n_rows <= 0) return hipSuccess;
     return dispatch_dt(dt, [&](auto v) -> hipError_t {
@@ -99,12 +140,17 @@ hipError_t csr_spmv(int dt, int vector, int64_t n_rows, const int32_t* rp, const
                                                        static_cast<const AT*>(x), static_cast<AT*>(y));
             return hipGetLastError();
         }
-        if (avg_row <= 2) return launch_vector<VT, 2>(n_rows, rp, ci, val, x, y, s);
-        if (avg_row <= 4) return launch_vector<VT, 4>(n_rows, rp, ci, val, x, y, s);
-        if (avg_row <= 8) return launch_vector<VT, 8>(n_rows, rp, ci, val, x, y, s);
-        if (avg_row <= 16) return launch_vector<VT, 16>(n_rows, rp, ci, val, x, y, s);
-        if (avg_row <= 32) return launch_vector<VT, 32>(n_rows, rp, ci, val, x, y, s);
-        return launch_vector<VT, 64>(n_rows, rp, ci, val, x, y, s);
+        hipError_t e;
+        if (avg_row <= 2) e = launch_vector<VT, 2>(n_rows, rp, ci, val, x, y, s);
+        else if (avg_row <= 4) e = launch_vector<VT, 4>(n_rows, rp, ci, val, x, y, s);
+        else if (avg_row <= 8) e = launch_vector<VT, 8>(n_rows, rp, ci, val, x, y, s);
+        else if (avg_row <= 16) e = launch_vector<VT, 16>(n_rows, rp, ci, val, x, y, s);
+        else if (avg_row <= 32) e = launch_vector<VT, 32>(n_rows, rp, ci, val, x, y, s);
+        else e = launch_vector<VT, 64>(n_rows, rp, ci, val, x, y, s);
+        if (e != hipSuccess || n_long <= 0) return e;
+        csr_long_row_kernel<VT><<<(unsigned)n_long, 256, 0, s>>>(long_rows, rp, ci, static_cast<const VT*>(val),
+                                                                 static_cast<const AT*>(x), static_cast<AT*>(y));
+        return hipGetLastError();
     });
 }
 

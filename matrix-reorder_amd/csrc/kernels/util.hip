@@ -84,6 +84,13 @@ __global__ void permute_vec_kernel(const int32_t* perm, int64_t n, int dir, cons
     }
 }
 
+// streaming copy of 16-byte words: the achievable-bandwidth reference
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void validate_rows_kernel(int64_t n_rows, const int32_t* rp, int64_t nnz, int32_t* flag)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_rows;
@@ -148,6 +155,17 @@ hipError_t permute_vec(const int32_t* perm, int64_t n, int dir, const void* in, 
     else
         permute_vec_kernel<uint32_t><<<grid_for(n), kBlock, 0, s>>>(
             perm, n, dir, static_cast<const uint32_t*>(in), static_cast<uint32_t*>(out));
+    return hipGetLastError();
+}
+
+hipError_t copy_bytes(void* dst, const void* src, int64_t bytes, int cu_count, hipStream_t s)
+{
+    const int64_t n16 = bytes / 16;
+    if (n16 <= 0) return hipSuccess;
+    int64_t g = (int64_t)cu_count * 8;
+    if (g > (n16 + 255) / 256) g = (n16 + 255) / 256;
+    if (g < 1) g = 1;
+    copy16_kernel<<<(unsigned)g, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
     return hipGetLastError();
 }
 

@@ -163,8 +163,10 @@ int mr_rd_sampled_raw(mr_mat m, const mr_rd_params* p, float* est, int32_t* stat
     MR_HIP_TRY(hipMemcpyAsync(hpos.data(), d_pos.p, sizeof(int32_t) * (size_t)S, hipMemcpyDeviceToHost, s));
     MR_HIP_TRY(hipStreamSynchronize(s));
     for (int32_t i = 0; i < S; ++i) {
-        float e = he[(size_t)i];
-        if (hst[(size_t)i] != 2) e = hll_estimate(hs[(size_t)i], hz[(size_t)i], 1 << hp);
+        // the estimate is formed on the host from the serial register sum for
+        // every status (the device value only drives the early-stop check)
+        const float e = hll_estimate(hs[(size_t)i], hz[(size_t)i], 1 << hp);
+        (void)he;
         if (est) est[i] = e;
         if (status) status[i] = hst[(size_t)i];
         if (pos) pos[i] = hpos[(size_t)i];

@@ -142,6 +142,7 @@ int relabel(mr_mat m, const std::vector<int32_t>& perm, double* ms_out)
     m->ci = std::move(d_newci);
     m->val = std::move(d_newval);
     for (auto& p : m->perm) p = perm[(size_t)p];
+    m->n_long = -1;   // row lengths moved: the long-row list is rebuilt on next use
     if (ms_out) *ms_out = ms_since(t0);
     return merbit_build(m, s);
 }
@@ -314,7 +315,7 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     const auto t_gpu = clk::now();
     DevBuf tp, ti;
     MR_TRY(gpu_transpose(m, tp, ti, s));
-    DevBuf score, T0, T1, sure, tie, hist, chunk, placement, state;
+    DevBuf score, T0, T1, sure, tie, hist, chunk, placement, state, wprefix, hcol, ylist;
     MR_TRY(score.alloc(sizeof(uint32_t) * (size_t)n));
     MR_TRY(T0.alloc(sizeof(int32_t) * (size_t)n));
     MR_TRY(T1.alloc(sizeof(int32_t) * (size_t)n));
@@ -324,6 +325,10 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     MR_TRY(chunk.alloc(sizeof(int32_t) * (size_t)((n + 1023) / 1024 + 1)));
     MR_TRY(placement.alloc(sizeof(uint32_t) * (size_t)n));
     MR_TRY(state.alloc(sizeof(k::PgState)));
+    MR_TRY(wprefix.alloc(sizeof(int32_t) * 1025));
+    MR_TRY(hcol.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(ylist.alloc(sizeof(int32_t) * (size_t)n));
+    MR_HIP_TRY(hipMemsetAsync(hcol.p, 0, hcol.bytes, s));
     MR_HIP_TRY(hipMemsetAsync(hist.p, 0, hist.bytes, s));
     MR_HIP_TRY(hipMemsetAsync(state.p, 0, state.bytes, s));
     MR_HIP_TRY(hipMemsetAsync(placement.p, 0xff, placement.bytes, s));
@@ -334,7 +339,8 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     for (int32_t b = 0; b < B; ++b) {
         MR_HIP_TRY(k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(), ti.as<int32_t>(),
                                score.as<uint32_t>(), Tl, st, sure.as<int32_t>(), tie.as<int32_t>(),
-                               hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(), Ps[(size_t)b],
+                               hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(),
+                               wprefix.as<int32_t>(), hcol.as<int32_t>(), ylist.as<int32_t>(), Ps[(size_t)b],
                                Pl[(size_t)b], Os[(size_t)b], Ol[(size_t)b], s));
         cur ^= 1;
     }

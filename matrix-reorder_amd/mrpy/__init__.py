@@ -81,6 +81,7 @@ SIGNATURES = {
     "mr_ctx_arch": [_P, C.c_char_p, C.c_int, C.POINTER(C.c_int)],
     "mr_malloc": [_P, C.c_size_t, C.POINTER(_P)], "mr_free": [_P, _P],
     "mr_memcpy": [_P, _P, _P, C.c_size_t, C.c_int], "mr_memset": [_P, _P, C.c_int, C.c_size_t],
+    "mr_bench_copy": [_P, _P, _P, C.c_size_t, _P],
     "mr_stream_create": [_P, C.POINTER(_P)], "mr_stream_destroy": [_P, _P], "mr_stream_sync": [_P, _P],
     "mr_event_create": [_P, C.POINTER(_P)], "mr_event_destroy": [_P, _P],
     "mr_event_record": [_P, _P, _P], "mr_event_elapsed_ms": [_P, _P, _P, _FP],
@@ -116,6 +117,8 @@ SIGNATURES = {
     "mr_spmv_ws": [_P, _P, _P, _P, _P],
     "mr_spmm": [_P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int64, _P],
     "mr_merbit_configure": [_P, C.c_int],
+    "mr_merbit_options": [_P, C.c_int],
+    "mr_merbit_autotune": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _FP],
     "mr_merbit_info": [_P, C.POINTER(C.c_int), _I64P, _I64P, _DP],
     "mr_merbit_export": [_P, _U32P, _I32P, _I32P],
 }
@@ -354,6 +357,9 @@ class Ctx:
         check(lib().mr_event_elapsed_ms(self.h, e0, e1, C.byref(ms)))
         return ms.value
 
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int, stream=None):
+        check(lib().mr_bench_copy(self.h, dst.ptr, src.ptr, nbytes, stream))
+
     def upload(self, a: np.ndarray) -> DeviceArray:
         return DeviceArray.from_host(self, a)
 
@@ -512,6 +518,14 @@ class Mat:
 
     def merbit_configure(self, sigma: int):
         check(lib().mr_merbit_configure(self.h, sigma))
+
+    def merbit_options(self, policy: int):
+        check(lib().mr_merbit_options(self.h, policy))
+
+    def merbit_autotune(self, reps: int = 5):
+        sg, pol, us = C.c_int(), C.c_int(), C.c_float()
+        check(lib().mr_merbit_autotune(self.h, reps, C.byref(sg), C.byref(pol), C.byref(us)))
+        return {"sigma": sg.value, "policy": pol.value, "us": us.value}
 
     def merbit_info(self):
         s, it, nb, ms = C.c_int(), C.c_int64(), C.c_int64(), C.c_double()

@@ -140,3 +140,55 @@ def test_malformed_csr_rejected(ctx, mr):
     with pytest.raises(mr.MRError) as e:
         mr.Mat.from_csr(ctx, np.array([0, 1], np.int32), np.array([5], np.int32), n_cols=2)
     assert e.value.code == 6
+
+
+def test_merbit_policies_bit_identical(ctx, mr):
+    # the load policy (non-temporal streams, XCD block order) changes nothing
+    # in the arithmetic: y is bit-identical for every policy and sigma pair
+    name, rp, ci, nc = CASES[0]
+    rng = np.random.default_rng(17)
+    mat = mr.Mat.from_csr(ctx, rp, ci, rng.uniform(-1, 1, len(ci)), n_cols=nc, store=mr.F32)
+    x = rng.uniform(-1, 1, nc).astype(np.float32)
+    for sigma in (8, 16, 32):
+        mat.merbit_configure(sigma)
+        ys = []
+        for pol in range(4):
+            mat.merbit_options(pol)
+            ys.append(spmv(ctx, mr, mat, x, mr.MERBIT).view(np.uint32))
+        assert all(np.array_equal(ys[0], yy) for yy in ys[1:])
+    best = mat.merbit_autotune(3)
+    assert best["sigma"] in (8, 16, 32) and 0 <= best["policy"] <= 3 and best["us"] > 0
+    mat.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("k", [2, 3, 8, 16, 32, 64, 70])
+def test_spmm_merbit(ctx, mr, oracle, case, k):
+    name, rp, ci, nc = case
+    rng = np.random.default_rng(100 + k)
+    val = rng.uniform(-1, 1, len(ci))
+    X = rng.uniform(-1, 1, (nc, k))
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=mr.F32)
+    n = len(rp) - 1
+    ldy = k + 3   # a padded leading dimension for Y
+    Xd = ctx.upload(X.astype(np.float32))
+    Yd = ctx.upload(np.full((n, ldy), 7.0, np.float32))
+    mat.spmm(k, Xd, Yd, mr.MERBIT, ldx=k, ldy=ldy)
+    ctx.sync()
+    Y = Yd.to_host()
+    Yh = oracle.spmm_f64(rp, ci, val.astype(np.float32).astype(np.float64), X.astype(np.float32).astype(np.float64))
+    absm = np.abs(sp_abs(rp, ci, val, nc)) @ np.abs(X) if n else np.zeros((0, k))
+    err = np.abs(Y[:, :k] - Yh)
+    assert np.all(err <= 1e-5 * np.maximum(absm, 1e-30) + 1e-30), float(np.max(err / np.maximum(absm, 1e-30)))
+    assert np.all(Y[:, k:] == 7.0)   # padding columns untouched
+    # bit-reproducible
+    Yd2 = ctx.upload(np.zeros((n, ldy), np.float32))
+    mat.spmm(k, Xd, Yd2, mr.MERBIT, ldx=k, ldy=ldy)
+    ctx.sync()
+    assert np.array_equal(Yd2.to_host()[:, :k].view(np.uint32), Y[:, :k].view(np.uint32))
+    mat.close()
+
+
+def sp_abs(rp, ci, val, nc):
+    import scipy.sparse as sp
+    return sp.csr_matrix((np.abs(val), ci, rp), shape=(len(rp) - 1, nc))
