@@ -248,6 +248,39 @@ int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, doub
  * row) and the merge coordinates at every block start (+ sentinel) */
 int mr_merbit_export(mr_mat m, uint32_t* flags, int32_t* blk_x, int32_t* blk_y);
 
+/* ---- batched SpMV / SpMM: many matrices, one persistent fused launch -------
+ * The reference multiplies one matrix per GRAPH::spmv call (class.cu:1331-1454,
+ * one merbit_kernel launch per iteration at :1421-1432); a batch replaces a
+ * loop of those calls over many handles.  All matrices share one context and
+ * one storage dtype; each is multiplied by k right-hand sides (k = 1: SpMV on
+ * the MERBIT sigma grid; 2 <= k <= 64: SpMM on the 4096-item grid).  The fused
+ * launch is a persistent kernel over the concatenated MERBIT tiles of all jobs
+ * plus one carry fix-up; results are bit-identical to mr_spmv / mr_spmm with
+ * the same sigma.  The batch owns its carry buffers, so it may run while the
+ * member handles are used elsewhere; x / y buffers must not overlap between
+ * jobs.  At most 1024 jobs. */
+typedef struct mr_batch_s* mr_batch;
+int mr_batch_create(mr_ctx ctx, int n_jobs, const mr_mat* mats, int k, mr_batch* out);
+int mr_batch_destroy(mr_batch b);
+/* x[j], y[j]: device pointers of job j (X[n_cols][ldx], Y[n_rows][ldy]; ldx /
+ * ldy may be NULL = k).  The job table is copied on stream s; launches on the
+ * same stream see it in order, and a captured graph reads the current table. */
+int mr_batch_set_io(mr_batch b, const void* const* x, const int64_t* ldx, void* const* y, const int64_t* ldy,
+                    mr_stream s);
+/* sigma in {8,16,32} (k = 1 only), nt = non-temporal streams (k = 1 only),
+ * grid = persistent workgroups (0 = one workgroup per tile) */
+int mr_batch_configure(mr_batch b, int sigma, int nt, int grid);
+/* times every (sigma, nt, grid) candidate with the current io (overwrites y)
+ * and keeps the fastest */
+int mr_batch_autotune(mr_batch b, int reps, int* sigma, int* nt, int* grid, float* best_us);
+int mr_batch_run(mr_batch b, mr_stream s);
+/* capture the current configuration as a hipGraph on stream s (not NULL) */
+int mr_batch_graph_capture(mr_batch b, mr_stream s);
+int mr_batch_graph_launch(mr_batch b, mr_stream s);
+/* tiles of the fused launch, total nnz / rows / cols, and the algorithmic
+ * bytes and flops of one run (canonical int32 CSR count) */
+int mr_batch_info(mr_batch b, int64_t* tiles, int64_t* nnz, double* bytes, double* flops);
+
 #ifdef __cplusplus
 }
 #endif

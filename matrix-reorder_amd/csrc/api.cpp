@@ -52,6 +52,7 @@ int merbit_build(mr_mat m, hipStream_t s)
     // the 4096-item SpMM grid) reads inside it
     const int64_t words = (p.items + 8191) / 8192 * (8192 / 32);
     p.mm_built = false;
+    for (int i = 0; i < 3; ++i) p.cnb[i] = -1;
     MR_TRY(p.flags.alloc(sizeof(uint32_t) * (size_t)(words > 0 ? words : 1)));
     MR_TRY(p.blk_x.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
     MR_TRY(p.blk_y.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
@@ -72,6 +73,49 @@ int merbit_build(mr_mat m, hipStream_t s)
     hipEventDestroy(e1);
     p.build_ms = ms;
     p.built = true;
+    return MR_OK;
+}
+
+int merbit_coords_sigma(mr_mat m, int sigma, hipStream_t s, const int32_t** bx, const int32_t** by, int64_t* nb)
+{
+    MerbitPlan& p = m->merbit;
+    if (!p.built) MR_TRY(merbit_build(m, s));
+    if (sigma == p.sigma) {
+        *bx = p.blk_x.as<int32_t>();
+        *by = p.blk_y.as<int32_t>();
+        *nb = p.n_blocks;
+        return MR_OK;
+    }
+    const int i = sigma == 8 ? 0 : sigma == 16 ? 1 : sigma == 32 ? 2 : -1;
+    if (i < 0) return set_error(MR_ERR_INVALID_ARG, "sigma must be 8, 16 or 32 (got %d)", sigma);
+    if (p.cnb[i] < 0) {
+        const int64_t T = 256LL * sigma;
+        const int64_t n = (p.items + T - 1) / T;
+        MR_TRY(p.cx[i].alloc(sizeof(int32_t) * (size_t)(n + 1)));
+        MR_TRY(p.cy[i].alloc(sizeof(int32_t) * (size_t)(n + 1)));
+        MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), T, n, p.cx[i].as<int32_t>(),
+                                    p.cy[i].as<int32_t>(), s));
+        p.cnb[i] = n;
+    }
+    *bx = p.cx[i].as<int32_t>();
+    *by = p.cy[i].as<int32_t>();
+    *nb = p.cnb[i];
+    return MR_OK;
+}
+
+int merbit_mm_grid(mr_mat m, hipStream_t s)
+{
+    MerbitPlan& p = m->merbit;
+    if (!p.built) MR_TRY(merbit_build(m, s));
+    if (p.mm_built) return MR_OK;
+    p.mm_blocks = (p.items + 4095) / 4096;
+    MR_TRY(p.mm_x.alloc(sizeof(int32_t) * (size_t)(p.mm_blocks + 1)));
+    MR_TRY(p.mm_y.alloc(sizeof(int32_t) * (size_t)(p.mm_blocks + 1)));
+    MR_TRY(p.mm_carry.alloc(vec_size(m->dtype) * 64 * (size_t)std::max<int64_t>(p.mm_blocks, 1)));
+    MR_TRY(p.mm_carry_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(p.mm_blocks, 1)));
+    MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), 4096, p.mm_blocks, p.mm_x.as<int32_t>(),
+                                p.mm_y.as<int32_t>(), s));
+    p.mm_built = true;
     return MR_OK;
 }
 
@@ -523,17 +567,7 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
     }
     if (m->n_rows == 0) return MR_OK;
     MerbitPlan& p = m->merbit;
-    if (!p.built) MR_TRY(merbit_build(m, st));
-    if (!p.mm_built) {
-        p.mm_blocks = (p.items + 4095) / 4096;
-        MR_TRY(p.mm_x.alloc(sizeof(int32_t) * (size_t)(p.mm_blocks + 1)));
-        MR_TRY(p.mm_y.alloc(sizeof(int32_t) * (size_t)(p.mm_blocks + 1)));
-        MR_TRY(p.mm_carry.alloc(vec_size(m->dtype) * 64 * (size_t)std::max<int64_t>(p.mm_blocks, 1)));
-        MR_TRY(p.mm_carry_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(p.mm_blocks, 1)));
-        MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), 4096, p.mm_blocks, p.mm_x.as<int32_t>(),
-                                    p.mm_y.as<int32_t>(), st));
-        p.mm_built = true;
-    }
+    MR_TRY(merbit_mm_grid(m, st));
     // column chunks of at most 64 right-hand sides
     const size_t vb = vec_size(m->dtype);
     for (int j0 = 0; j0 < k; j0 += 64) {

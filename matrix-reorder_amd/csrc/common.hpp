@@ -109,6 +109,10 @@ struct MerbitPlan {
     int64_t mm_blocks = 0;
     mr::DevBuf mm_x, mm_y, mm_carry, mm_carry_row;
     bool mm_built = false;
+    // merge coordinates at the other tile widths, for batched launches that
+    // pick their own sigma: index 0 / 1 / 2 = sigma 8 / 16 / 32, built on use
+    mr::DevBuf cx[3], cy[3];
+    int64_t cnb[3] = {-1, -1, -1};
     bool built = false;
     double build_ms = 0.0;
 };
@@ -130,4 +134,9 @@ struct mr_mat_s {
 namespace mr {
 constexpr int64_t kPad = 256;  // element padding of col/val arrays
 int merbit_build(mr_mat m, hipStream_t s);
+// merge coordinates of m at tile width 256 * sigma (the plan's own arrays when
+// sigma matches, else a cached copy built on first use)
+int merbit_coords_sigma(mr_mat m, int sigma, hipStream_t s, const int32_t** bx, const int32_t** by, int64_t* nb);
+// the 4096-item SpMM grid of m, built on first use
+int merbit_mm_grid(mr_mat m, hipStream_t s);
 }  // namespace mr

@@ -35,6 +35,29 @@ hipError_t csr_spmm(int dt, int64_t n_rows, int k, const int32_t* rp, const int3
                     hipStream_t s);
 
 // ---- MERBIT (merbit.hip) ----
+// one matrix of a fused batch (device-resident table; pointers typed by the
+// batch's storage dtype): SpMV uses the first ten fields, SpMM all of them
+constexpr int kMaxBatchJobs = 1024;
+struct BatchJob {
+    const void* ci;
+    const void* val;
+    const void* flags;
+    const void* blk_x;
+    const void* blk_y;
+    const void* x;
+    void* y;
+    void* carry;
+    void* carry_row;
+    int64_t n_rows, nnz, ldx, ldy;
+    int32_t kk, pad;
+};
+// persistent fused MERBIT SpMV over all jobs (begin[j] = first global block of
+// job j, begin[n_jobs] = total) followed by a fused carry fix-up
+hipError_t merbit_batch_spmv(int dt, int sigma, int nt, const void* jobs, const int64_t* begin, int n_jobs,
+                             int64_t total, int grid, hipStream_t s);
+// same for SpMM on the 4096-item grid, group width from the batch's k
+hipError_t merbit_batch_spmm(int dt, int kk, const void* jobs, const int64_t* begin, int n_jobs, int64_t total,
+                             int grid, hipStream_t s);
 // merge coordinates at every multiple of `items_per_block` (+ sentinel)
 hipError_t merbit_coords(int64_t n_rows, int64_t nnz, const int32_t* rp, int64_t items_per_block, int64_t n_blocks,
                          int32_t* blk_x, int32_t* blk_y, hipStream_t s);

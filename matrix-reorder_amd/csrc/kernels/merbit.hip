@@ -46,6 +46,7 @@ namespace {
 
 constexpr int kLanes = 256;
 constexpr int kWaves = kLanes / 64;
+constexpr int kMaxJobs = kMaxBatchJobs;
 
 // blk_x/blk_y at diagonal d (merge_search, include/merbit.cuh:24-53)
 __device__ __forceinline__ void merge_search(const int32_t* rp, int64_t n_rows, int64_t nnz,
@@ -121,9 +122,11 @@ template <bool NT, typename T> __device__ __forceinline__ void st_stream(T* p, T
     else *p = v;
 }
 
-template <typename VT, int S, bool NT, bool REMAP>
-__global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
-    int64_t n_blocks, const int32_t* __restrict__ ci, const VT* __restrict__ val,
+// one MERBIT block b of one matrix (shared by the single-matrix kernel and the
+// persistent batched kernel, which calls it for many (matrix, block) pairs)
+template <typename VT, int S, bool NT>
+__device__ __forceinline__ void merbit_block(
+    int64_t b, const int32_t* __restrict__ ci, const VT* __restrict__ val,
     const uint32_t* __restrict__ flags, const int32_t* __restrict__ blk_x,
     const int32_t* __restrict__ blk_y, const typename Acc<VT>::type* __restrict__ x,
     typename Acc<VT>::type* __restrict__ y, typename Acc<VT>::type* __restrict__ carry,
@@ -136,7 +139,6 @@ __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
     __shared__ int s_wf[kWaves];
     __shared__ int s_wc[kWaves];
 
-    const int64_t b = REMAP ? remap_block(blockIdx.x, n_blocks) : (int64_t)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int32_t x0 = blk_x[b], x1 = blk_x[b + 1];
     const int32_t y0 = blk_y[b], y1 = blk_y[b + 1];
@@ -257,6 +259,85 @@ __global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
     }
 }
 
+template <typename VT, int S, bool NT, bool REMAP>
+__global__ __launch_bounds__(kLanes) void merbit_spmv_kernel(
+    int64_t n_blocks, const int32_t* __restrict__ ci, const VT* __restrict__ val,
+    const uint32_t* __restrict__ flags, const int32_t* __restrict__ blk_x,
+    const int32_t* __restrict__ blk_y, const typename Acc<VT>::type* __restrict__ x,
+    typename Acc<VT>::type* __restrict__ y, typename Acc<VT>::type* __restrict__ carry,
+    int32_t* __restrict__ carry_row)
+{
+    const int64_t b = REMAP ? remap_block(blockIdx.x, n_blocks) : (int64_t)blockIdx.x;
+    merbit_block<VT, S, NT>(b, ci, val, flags, blk_x, blk_y, x, y, carry, carry_row);
+}
+
+// ---- batched: many matrices, one persistent launch ------------------------
+// job j owns global blocks [begin[j], begin[j+1]); a block finds its job by
+// binary search over the job table cached in LDS
+template <typename VT>
+struct BatchJobT {   // same layout as BatchJob in launch.hpp
+    const int32_t* ci;
+    const VT* val;
+    const uint32_t* flags;
+    const int32_t* blk_x;
+    const int32_t* blk_y;
+    const typename Acc<VT>::type* x;
+    typename Acc<VT>::type* y;
+    typename Acc<VT>::type* carry;
+    int32_t* carry_row;
+    int64_t n_rows, nnz, ldx, ldy;
+    int32_t kk, pad;
+};
+
+__device__ __forceinline__ int find_job(const int64_t* s_begin, int n_jobs, int64_t g)
+{
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_begin[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename VT, int S, bool NT>
+__global__ __launch_bounds__(kLanes) void merbit_batch_kernel(const BatchJobT<VT>* __restrict__ jobs,
+                                                              const int64_t* __restrict__ begin, int n_jobs,
+                                                              int64_t total)
+{
+    __shared__ int64_t s_begin[kMaxJobs + 1];
+    for (int i = threadIdx.x; i <= n_jobs; i += kLanes) s_begin[i] = begin[i];
+    __syncthreads();
+    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const int j = find_job(s_begin, n_jobs, g);
+        const BatchJobT<VT> J = jobs[j];
+        merbit_block<VT, S, NT>(g - s_begin[j], J.ci, J.val, J.flags, J.blk_x, J.blk_y, J.x, J.y, J.carry,
+                                J.carry_row);
+        __syncthreads();   // LDS is reused by the next block
+    }
+}
+
+template <typename VT>
+__global__ void merbit_batch_fixup_kernel(const BatchJobT<VT>* __restrict__ jobs, const int64_t* __restrict__ begin,
+                                          int n_jobs, int64_t total)
+{
+    using AT = typename Acc<VT>::type;
+    __shared__ int64_t s_begin[kMaxJobs + 1];
+    for (int i = threadIdx.x; i <= n_jobs; i += blockDim.x) s_begin[i] = begin[i];
+    __syncthreads();
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int j = find_job(s_begin, n_jobs, g);
+    const int64_t b = g - s_begin[j], nb = s_begin[j + 1] - s_begin[j];
+    const AT* carry = jobs[j].carry;
+    const int32_t* crow = jobs[j].carry_row;
+    const int32_t r = crow[b];
+    if (r >= jobs[j].n_rows) return;
+    if (b > 0 && crow[b - 1] == r) return;
+    AT s = carry[b];
+    for (int64_t q = b + 1; q < nb && crow[q] == r; ++q) s += carry[q];
+    jobs[j].y[r] = jobs[j].y[r] + s;
+}
+
 template <typename AT>
 __global__ void merbit_fixup_kernel(int64_t n_blocks, int64_t n_rows, const AT* __restrict__ carry,
                                     const int32_t* __restrict__ carry_row, AT* __restrict__ y)
@@ -284,6 +365,17 @@ hipError_t launch_spmv(int64_t n_rows, int64_t n_blocks, const int32_t* ci, cons
     if (e != hipSuccess) return e;
     merbit_fixup_kernel<AT><<<(unsigned)((n_blocks + 255) / 256), 256, 0, s>>>(
         n_blocks, n_rows, static_cast<const AT*>(carry), carry_row, static_cast<AT*>(y));
+    return hipGetLastError();
+}
+
+template <typename VT, int S, bool NT>
+hipError_t launch_batch(const void* jobs, const int64_t* begin, int n_jobs, int64_t total, int grid, hipStream_t s)
+{
+    const BatchJobT<VT>* J = static_cast<const BatchJobT<VT>*>(jobs);
+    merbit_batch_kernel<VT, S, NT><<<(unsigned)grid, kLanes, 0, s>>>(J, begin, n_jobs, total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    merbit_batch_fixup_kernel<VT><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(J, begin, n_jobs, total);
     return hipGetLastError();
 }
 
@@ -320,6 +412,28 @@ hipError_t merbit_build(int64_t n_rows, int64_t nnz, const int32_t* rp, int sigm
     blocks_kernel<<<(unsigned)((n_blocks + 1 + 255) / 256), 256, 0, s>>>(n_rows, nnz, items, T, n_blocks,
                                                                           rp, blk_x, blk_y);
     return hipGetLastError();
+}
+
+hipError_t merbit_batch_spmv(int dt, int sigma, int nt, const void* jobs, const int64_t* begin, int n_jobs,
+                             int64_t total, int grid, hipStream_t s)
+{
+    if (total <= 0 || n_jobs <= 0) return hipSuccess;
+    if (n_jobs > kMaxJobs) return hipErrorInvalidValue;
+    return dispatch_dt(dt, [&](auto v) -> hipError_t {
+        using VT = decltype(v);
+        auto go = [&](auto stag) -> hipError_t {
+            constexpr int S = decltype(stag)::value;
+            if constexpr (S == 32 && sizeof(typename Acc<VT>::type) == 8) return hipErrorInvalidValue;
+            else return nt ? launch_batch<VT, S, true>(jobs, begin, n_jobs, total, grid, s)
+                           : launch_batch<VT, S, false>(jobs, begin, n_jobs, total, grid, s);
+        };
+        switch (sigma) {
+        case 8: return go(std::integral_constant<int, 8>{});
+        case 16: return go(std::integral_constant<int, 16>{});
+        case 32: return go(std::integral_constant<int, 32>{});
+        default: return hipErrorInvalidValue;
+        }
+    });
 }
 
 hipError_t merbit_coords(int64_t n_rows, int64_t nnz, const int32_t* rp, int64_t items_per_block, int64_t n_blocks,

@@ -27,8 +27,8 @@ constexpr int kLanes = 256;
 constexpr int kItems = 4096;   // merge-path items per block
 
 template <typename VT, int G>
-__global__ __launch_bounds__(kLanes) void merbit_mm_kernel(
-    int64_t n_blocks, const int32_t* __restrict__ ci, const VT* __restrict__ val,
+__device__ __forceinline__ void merbit_mm_block(
+    int64_t b, const int32_t* __restrict__ ci, const VT* __restrict__ val,
     const uint32_t* __restrict__ flags, const int32_t* __restrict__ blk_x, const int32_t* __restrict__ blk_y,
     const typename Acc<VT>::type* __restrict__ X, int64_t ldx, typename Acc<VT>::type* __restrict__ Y,
     int64_t ldy, int kk, int64_t nnz_total, typename Acc<VT>::type* __restrict__ carry,
@@ -43,7 +43,6 @@ __global__ __launch_bounds__(kLanes) void merbit_mm_kernel(
     __shared__ AT s_tail[NS][G];
     __shared__ int s_has[NS];
 
-    const int64_t b = blockIdx.x;
     const int tid = threadIdx.x, j = tid & (G - 1), seg = tid / G;
     const int32_t x0 = blk_x[b];
     const int32_t y0 = blk_y[b], y1 = blk_y[b + 1];
@@ -136,6 +135,87 @@ __global__ __launch_bounds__(kLanes) void merbit_mm_kernel(
     }
 }
 
+template <typename VT, int G>
+__global__ __launch_bounds__(kLanes) void merbit_mm_kernel(
+    int64_t n_blocks, const int32_t* __restrict__ ci, const VT* __restrict__ val,
+    const uint32_t* __restrict__ flags, const int32_t* __restrict__ blk_x, const int32_t* __restrict__ blk_y,
+    const typename Acc<VT>::type* __restrict__ X, int64_t ldx, typename Acc<VT>::type* __restrict__ Y,
+    int64_t ldy, int kk, int64_t nnz_total, typename Acc<VT>::type* __restrict__ carry,
+    int32_t* __restrict__ carry_row)
+{
+    merbit_mm_block<VT, G>(blockIdx.x, ci, val, flags, blk_x, blk_y, X, ldx, Y, ldy, kk, nnz_total, carry, carry_row);
+}
+
+// ---- batched SpMM: many matrices (same k), one persistent launch ------------
+template <typename VT>
+struct MmJobT {   // same layout as BatchJob in launch.hpp
+    const int32_t* ci;
+    const VT* val;
+    const uint32_t* flags;
+    const int32_t* blk_x;
+    const int32_t* blk_y;
+    const typename Acc<VT>::type* x;
+    typename Acc<VT>::type* y;
+    typename Acc<VT>::type* carry;
+    int32_t* carry_row;
+    int64_t n_rows, nnz, ldx, ldy;
+    int32_t kk, pad;
+};
+
+__device__ __forceinline__ int find_job_mm(const int64_t* s_begin, int n_jobs, int64_t g)
+{
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_begin[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename VT, int G>
+__global__ __launch_bounds__(kLanes) void merbit_mm_batch_kernel(const MmJobT<VT>* __restrict__ jobs,
+                                                                 const int64_t* __restrict__ begin, int n_jobs,
+                                                                 int64_t total)
+{
+    __shared__ int64_t s_begin[kMaxBatchJobs + 1];
+    for (int i = threadIdx.x; i <= n_jobs; i += kLanes) s_begin[i] = begin[i];
+    __syncthreads();
+    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const int j = find_job_mm(s_begin, n_jobs, g);
+        const MmJobT<VT> J = jobs[j];
+        merbit_mm_block<VT, G>(g - s_begin[j], J.ci, J.val, J.flags, J.blk_x, J.blk_y, J.x, J.ldx, J.y, J.ldy, J.kk,
+                               J.nnz, J.carry, J.carry_row);
+        __syncthreads();
+    }
+}
+
+template <typename VT, int G>
+__global__ void merbit_mm_batch_fixup_kernel(const MmJobT<VT>* __restrict__ jobs, const int64_t* __restrict__ begin,
+                                             int n_jobs, int64_t total)
+{
+    using AT = typename Acc<VT>::type;
+    __shared__ int64_t s_begin[kMaxBatchJobs + 1];
+    for (int i = threadIdx.x; i <= n_jobs; i += blockDim.x) s_begin[i] = begin[i];
+    __syncthreads();
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t g = t / G;
+    const int jc = (int)(t % G);
+    if (g >= total) return;
+    const int j = find_job_mm(s_begin, n_jobs, g);
+    if (jc >= jobs[j].kk) return;
+    const int64_t b = g - s_begin[j], nb = s_begin[j + 1] - s_begin[j];
+    const AT* carry = jobs[j].carry;
+    const int32_t* crow = jobs[j].carry_row;
+    const int32_t r = crow[b];
+    if (r >= jobs[j].n_rows) return;
+    if (b > 0 && crow[b - 1] == r) return;
+    AT s = carry[b * G + jc];
+    for (int64_t q = b + 1; q < nb && crow[q] == r; ++q) s += carry[q * G + jc];
+    AT* Y = jobs[j].y;
+    const int64_t ldy = jobs[j].ldy;
+    Y[(int64_t)r * ldy + jc] = Y[(int64_t)r * ldy + jc] + s;
+}
+
 template <typename AT, int G>
 __global__ void merbit_mm_fixup_kernel(int64_t n_blocks, int64_t n_rows, int kk, const AT* __restrict__ carry,
                                        const int32_t* __restrict__ carry_row, AT* __restrict__ Y, int64_t ldy)
@@ -170,6 +250,37 @@ hipError_t launch_mm(int64_t n_rows, int64_t n_blocks, const int32_t* ci, const 
 }
 
 }  // namespace
+
+hipError_t merbit_batch_spmm(int dt, int kk, const void* jobs, const int64_t* begin, int n_jobs, int64_t total,
+                             int grid, hipStream_t s)
+{
+    if (total <= 0 || n_jobs <= 0) return hipSuccess;
+    if (kk > 64 || n_jobs > kMaxBatchJobs) return hipErrorInvalidValue;
+    return dispatch_dt(dt, [&](auto v) -> hipError_t {
+        using VT = decltype(v);
+        auto go = [&](auto gtag) -> hipError_t {
+            constexpr int G = decltype(gtag)::value;
+            const MmJobT<VT>* J = static_cast<const MmJobT<VT>*>(jobs);
+            merbit_mm_batch_kernel<VT, G><<<(unsigned)grid, kLanes, 0, s>>>(J, begin, n_jobs, total);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            const int64_t threads = total * G;
+            merbit_mm_batch_fixup_kernel<VT, G><<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(J, begin, n_jobs,
+                                                                                                  total);
+            return hipGetLastError();
+        };
+        int g = 2;
+        while (g < kk && g < 64) g <<= 1;
+        switch (g) {
+        case 2: return go(std::integral_constant<int, 2>{});
+        case 4: return go(std::integral_constant<int, 4>{});
+        case 8: return go(std::integral_constant<int, 8>{});
+        case 16: return go(std::integral_constant<int, 16>{});
+        case 32: return go(std::integral_constant<int, 32>{});
+        default: return go(std::integral_constant<int, 64>{});
+        }
+    });
+}
 
 int merbit_mm_group(int kk)
 {

@@ -159,11 +159,20 @@ __global__ __launch_bounds__(256) void thresh_kernel(const int32_t* __restrict__
     const int lo = t * seg, hi = min(kBins, lo + seg);
     int64_t sum = 0;
     for (int bnum = lo; bnum < hi; ++bnum) sum += hist[bnum];
-    s_sum[t] = sum;
-    __syncthreads();
-    if (t == 0) {
-        int64_t run = 0;
-        for (int i = 255; i >= 0; --i) { s_above[i] = run; run += s_sum[i]; }
+    // exclusive suffix sums over the 256 segments: wave scans from the top
+    {
+        const int lane = t & 63, wid = t >> 6;
+        int64_t v = sum;   // inclusive suffix inside the wave (lanes above this one)
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t o = __shfl_down(v, d, 64);
+            if (lane + d < 64) v += o;
+        }
+        if (lane == 0) s_sum[wid] = v;   // wave total
+        __syncthreads();
+        int64_t higher = 0;
+        for (int w = wid + 1; w < 4; ++w) higher += s_sum[w];
+        s_above[t] = higher + v - sum;
     }
     __syncthreads();
     if (pl <= 0 || tn < pl) {
@@ -469,19 +478,28 @@ __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t
 }
 
 __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict__ tp, const int32_t* __restrict__ ti,
+                                                       const int32_t* __restrict__ live,
                                                        int32_t* __restrict__ h, const int32_t* __restrict__ ylist,
                                                        uint32_t* __restrict__ score, int32_t* __restrict__ T,
                                                        PgState* __restrict__ st, int nxt)
 {
+    // groups of 16 lanes, four listed columns per wave; rows(y) holds only
+    // vertices still unplaced at the last compaction (live[y] entries)
     const int32_t ny = st->ycount;
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < ny; q += waves) {
-        const int32_t y = ylist[q];
-        const uint32_t w = (uint32_t)h[y];
-        const int32_t zb = tp[y], ze = tp[y + 1];
-        for (int32_t c0 = zb; c0 < ze; c0 += 64) {
-            const int32_t c = c0 + lane;
+    const int lane = threadIdx.x & 63, li = lane & 15;
+    const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 4);
+    for (int64_t q0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 4; q0 < ny; q0 += groups) {
+        const int64_t q = q0 + (lane >> 4);
+        int32_t zb = 0, ze = 0, y = 0;
+        uint32_t w = 0;
+        if (q < ny) {
+            y = ylist[q];
+            w = (uint32_t)h[y];
+            zb = tp[y];
+            ze = zb + live[y];
+        }
+        for (int32_t c0 = zb; c0 < ze; c0 += 16) {
+            const int32_t c = c0 + li;
             bool first = false;
             int32_t z = 0;
             if (c < ze) {
@@ -490,7 +508,39 @@ __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict
             }
             wave_append(first, z, T, &st->tcount[nxt]);
         }
-        if (lane == 0) h[y] = 0;   // y is listed once: back to 0 for the next window
+        if (q < ny && li == 0) h[y] = 0;   // y is listed once: back to 0 for the next window
+    }
+}
+
+__global__ void live_init_kernel(int64_t n, const int32_t* __restrict__ tp, int32_t* __restrict__ live)
+{
+    for (int64_t y = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; y < n; y += (int64_t)gridDim.x * blockDim.x)
+        live[y] = tp[y + 1] - tp[y];
+}
+
+// drop placed vertices from every column's row list (a placed vertex is never
+// counted again, so the counts do not change); one wave per column, order kept
+__global__ __launch_bounds__(256) void compact_kernel(int64_t n, const int32_t* __restrict__ tp, int32_t* __restrict__ ti,
+                                                      int32_t* __restrict__ live, const uint32_t* __restrict__ score)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t y = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); y < n; y += waves) {
+        const int32_t zb = tp[y], ze = zb + live[y];
+        int32_t wpos = zb;
+        for (int32_t c0 = zb; c0 < ze; c0 += 64) {
+            const int32_t c = c0 + lane;
+            int32_t z = 0;
+            bool keep = false;
+            if (c < ze) {
+                z = ti[c];
+                keep = (score[z] & kUnplaced) != 0;
+            }
+            const uint64_t m = __ballot(keep);
+            if (keep) ti[wpos + __popcll(m & ((1ull << lane) - 1ull))] = z;
+            wpos += __popcll(m);
+        }
+        if (lane == 0) live[y] = wpos - zb;
     }
 }
 
@@ -509,7 +559,7 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
                     int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
-                    int32_t ps, int32_t pl, int32_t os, int32_t ol, hipStream_t s)
+                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, hipStream_t s)
 {
     const int nxt = cur ^ 1;
     hist_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, hist);
@@ -520,8 +570,27 @@ hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, co
     reset_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score);
     if (ol > 0) {
         window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st);
-        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, h, ylist, score, Tl[nxt], st, nxt);
+        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, ylist, score, Tl[nxt], st, nxt);
     }
+    return hipGetLastError();
+}
+
+hipError_t pg_compact(int64_t n, const int32_t* tp, int32_t* ti, int32_t* live, const uint32_t* score,
+                      hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + 3) / 4;
+    if (g > 8192) g = 8192;
+    compact_kernel<<<(unsigned)g, 256, 0, s>>>(n, tp, ti, live, score);
+    return hipGetLastError();
+}
+
+hipError_t pg_live_init(int64_t n, const int32_t* tp, int32_t* live, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    live_init_kernel<<<(unsigned)g, 256, 0, s>>>(n, tp, live);
     return hipGetLastError();
 }
 

@@ -333,15 +333,27 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     MR_HIP_TRY(hipMemsetAsync(state.p, 0, state.bytes, s));
     MR_HIP_TRY(hipMemsetAsync(placement.p, 0xff, placement.bytes, s));
     k::PgState* st = state.as<k::PgState>();
+    // working copy of the column lists: placed vertices are dropped from it
+    // every `compact_every` batches (the host phase needs the full lists)
+    DevBuf ti_live, live;
+    MR_TRY(ti_live.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    MR_TRY(live.alloc(sizeof(int32_t) * (size_t)n));
+    if (nnz > 0)
+        MR_HIP_TRY(hipMemcpyAsync(ti_live.p, ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToDevice, s));
+    MR_HIP_TRY(k::pg_live_init(n, tp.as<int32_t>(), live.as<int32_t>(), s));
+    const int32_t compact_every = std::max<int32_t>(32, B / 32);
     MR_HIP_TRY(k::pg_init(n, m->rp.as<int32_t>(), score.as<uint32_t>(), T0.as<int32_t>(), st, chunk.as<int32_t>(), s));
     int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
     int cur = 0;
     for (int32_t b = 0; b < B; ++b) {
-        MR_HIP_TRY(k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(), ti.as<int32_t>(),
-                               score.as<uint32_t>(), Tl, st, sure.as<int32_t>(), tie.as<int32_t>(),
-                               hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(),
-                               wprefix.as<int32_t>(), hcol.as<int32_t>(), ylist.as<int32_t>(), Ps[(size_t)b],
-                               Pl[(size_t)b], Os[(size_t)b], Ol[(size_t)b], s));
+        if (b > 0 && b % compact_every == 0)
+            MR_HIP_TRY(k::pg_compact(n, tp.as<int32_t>(), ti_live.as<int32_t>(), live.as<int32_t>(),
+                                     score.as<uint32_t>(), s));
+        MR_HIP_TRY(k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(),
+                               ti_live.as<int32_t>(), score.as<uint32_t>(), Tl, st, sure.as<int32_t>(),
+                               tie.as<int32_t>(), hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(),
+                               wprefix.as<int32_t>(), hcol.as<int32_t>(), ylist.as<int32_t>(), live.as<int32_t>(),
+                               Ps[(size_t)b], Pl[(size_t)b], Os[(size_t)b], Ol[(size_t)b], s));
         cur ^= 1;
     }
     std::vector<uint32_t> order((size_t)n);

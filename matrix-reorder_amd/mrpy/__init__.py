@@ -121,6 +121,15 @@ SIGNATURES = {
     "mr_merbit_autotune": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _FP],
     "mr_merbit_info": [_P, C.POINTER(C.c_int), _I64P, _I64P, _DP],
     "mr_merbit_export": [_P, _U32P, _I32P, _I32P],
+    "mr_batch_create": [_P, C.c_int, C.POINTER(_P), C.c_int, C.POINTER(_P)],
+    "mr_batch_destroy": [_P],
+    "mr_batch_set_io": [_P, C.POINTER(_P), _I64P, C.POINTER(_P), _I64P, _P],
+    "mr_batch_configure": [_P, C.c_int, C.c_int, C.c_int],
+    "mr_batch_autotune": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _FP],
+    "mr_batch_run": [_P, _P],
+    "mr_batch_graph_capture": [_P, _P],
+    "mr_batch_graph_launch": [_P, _P],
+    "mr_batch_info": [_P, _I64P, _I64P, _DP, _DP],
 }
 _RESTYPE = {"mr_last_error": C.c_char_p, "mr_version": C.c_char_p}
 
@@ -540,6 +549,61 @@ class Mat:
         by = np.zeros(info["n_blocks"] + 1, np.int32)
         check(lib().mr_merbit_export(self.h, ptr(flags, C.c_uint32), ptr(bx, C.c_int32), ptr(by, C.c_int32)))
         return flags[:words], bx, by, info
+
+
+class Batch:
+    """Many matrices multiplied in one persistent fused launch (mr_batch_*)."""
+
+    def __init__(self, ctx: Ctx, mats, k: int = 1):
+        self.ctx = ctx
+        self.mats = list(mats)
+        self.k = k
+        arr = (_P * len(self.mats))(*[m.h for m in self.mats])
+        h = _P()
+        check(lib().mr_batch_create(ctx.h, len(self.mats), arr, k, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_io(self, xs, ys, ldx=None, ldy=None, stream=None):
+        n = len(self.mats)
+        xa = (_P * n)(*[x.ptr if x is not None else None for x in xs])
+        ya = (_P * n)(*[y.ptr if y is not None else None for y in ys])
+        lx = np.asarray(ldx, np.int64) if ldx is not None else None
+        ly = np.asarray(ldy, np.int64) if ldy is not None else None
+        check(lib().mr_batch_set_io(self.h, xa, ptr(lx, C.c_int64), ya, ptr(ly, C.c_int64), stream))
+        self._io = (list(xs), list(ys))   # keep the buffers alive
+
+    def configure(self, sigma: int = 8, nt: int = 1, grid: int = 0):
+        check(lib().mr_batch_configure(self.h, sigma, nt, grid))
+
+    def autotune(self, reps: int = 5):
+        s, t, g, us = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        check(lib().mr_batch_autotune(self.h, reps, C.byref(s), C.byref(t), C.byref(g), C.byref(us)))
+        return {"sigma": s.value, "nt": t.value, "grid": g.value, "us": us.value}
+
+    def run(self, stream=None):
+        check(lib().mr_batch_run(self.h, stream))
+
+    def capture(self, stream):
+        check(lib().mr_batch_graph_capture(self.h, stream))
+
+    def launch(self, stream=None):
+        check(lib().mr_batch_graph_launch(self.h, stream))
+
+    def info(self):
+        t, z, b, f = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
+        check(lib().mr_batch_info(self.h, C.byref(t), C.byref(z), C.byref(b), C.byref(f)))
+        return {"tiles": t.value, "nnz": z.value, "bytes": b.value, "flops": f.value}
 
 
 class Graph:
