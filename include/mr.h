@@ -281,6 +281,81 @@ int mr_batch_graph_launch(mr_batch b, mr_stream s);
  * bytes and flops of one run (canonical int32 CSR count) */
 int mr_batch_info(mr_batch b, int64_t* tiles, int64_t* nnz, double* bytes, double* flops);
 
+/* ---- serving loop ----------------------------------------------------------
+ * No reference counterpart (main.cu:122-140 runs one matrix per process).
+ * Reordered matrices stay resident; requests (matrix, k right-hand sides,
+ * seed) are queued and a dispatcher thread batches whatever is queued into
+ * fused MERBIT launches as soon as one of `slots` streams is free (continuous
+ * batching: batches overlap on the device).  Requests on one matrix within a
+ * batch are merged into one SpMM when `coalesce` is set.  A request is admitted
+ * into a batch when its X / Y / carry bytes fit the slot's arena; one that can
+ * never fit fails with MR_ERR_ADMISSION.  Right-hand sides are synthesised on
+ * the device from the request seed (mr_serve_rhs gives the same values on the
+ * host); results stay in the slot's arena until the slot is reused, so the
+ * server reports a 64-bit hash (verify >= 1, mr_serve_hash on the host) or a
+ * host copy (verify = 2).  Latency = request arrival -> the dispatcher sees its
+ * batch complete.  Matrices must use one storage dtype (F32, F16 or BF16) and
+ * must not be reconfigured or relabelled while served. */
+typedef struct mr_server_s* mr_server;
+typedef struct {
+    int slots;              /* batches in flight, one HIP stream each (4)          */
+    int max_jobs;           /* requests per batch (256, at most 1024)               */
+    int coalesce;           /* merge requests on one matrix into one SpMM (1)       */
+    int verify;             /* 0, 1 = keep result hashes, 2 = also host copies (0) */
+    int sigma;              /* MERBIT tile width of single-vector jobs (8)          */
+    int grid;               /* persistent workgroups per launch (0 = one per tile)  */
+    int upload_threads;     /* host threads of a background reorder (0 = half)      */
+    int pad;
+    int64_t slot_bytes;     /* device arena per slot (1 GiB)                        */
+    int64_t hbm_budget;     /* bytes uploaded matrices may hold (0 = free HBM only) */
+    int64_t hbm_reserve;    /* device bytes left free when admitting uploads (4 GiB) */
+    double timeout_ms;      /* queued longer -> MR_ERR_TIMEOUT (0 = never)          */
+} mr_serve_config;
+typedef struct {
+    double rate;            /* Poisson arrivals per second                          */
+    int64_t n_requests;     /* trace length, uploads included                       */
+    uint64_t seed;
+    int k_choices[4];       /* k drawn uniformly from the first n_k entries         */
+    int n_k;
+    int upload_every;       /* every n-th arrival uploads a new R-MAT (0 = none)    */
+    int upload_scale;       /* R-MAT scale of an upload (18)                        */
+    int upload_edge_factor; /* (16)                                                 */
+} mr_serve_load;
+typedef struct {
+    int64_t completed, rejected, timed_out, cancelled;
+    int64_t uploads_admitted, uploads_rejected, evictions;
+    double p50_us, p90_us, p99_us, mean_us, max_us;
+    double wall_s;          /* first arrival -> last completion                     */
+    double gflops;          /* 2 nnz k summed over completed requests / wall_s      */
+    double gbytes_s;        /* per-request algorithmic bytes / wall_s               */
+    double offered_rate, achieved_rate;
+    double batches, mean_jobs_per_batch, mean_requests_per_batch;
+    double upload_ms_mean;  /* generate + admit + pGOrder per admitted upload       */
+} mr_serve_report;
+int mr_serve_config_default(mr_serve_config* c);
+int mr_serve_create(mr_ctx ctx, const mr_serve_config* c, mr_server* out);
+/* drains queued and running requests, then frees the server's own matrices */
+int mr_serve_destroy(mr_server s);
+/* a resident matrix owned by the caller (kept alive until the server ends) */
+int mr_serve_add(mr_server s, mr_mat m, int* id);
+/* asynchronous: generate an R-MAT (a,b,c = 0.57, 0.19, 0.19, scrambled ids,
+ * uniform[-1,1] values), admit it by free HBM and the budget (evicting the
+ * least recently used idle upload if needed), reorder it with pGOrder on the
+ * context's stream and make it resident; mr_serve_upload_wait gives its id */
+int mr_serve_upload_rmat(mr_server s, int scale, int edge_factor, uint64_t seed, int* ticket);
+int mr_serve_upload_wait(mr_server s, int ticket, int* status, int* mat_id, double* ms);
+int mr_serve_submit(mr_server s, int mat_id, int k, uint64_t seed, int64_t* req);
+int mr_serve_cancel(mr_server s, int64_t req);   /* only while queued */
+int mr_serve_wait(mr_server s, int64_t req, int* status, double* latency_us, uint64_t* hash);
+/* y[n_rows][k] of a completed request (verify = 2) */
+int mr_serve_result(mr_server s, int64_t req, float* y);
+/* open-loop Poisson load from the calling thread; blocks until every request
+ * and upload of the trace has finished */
+int mr_serve_run_load(mr_server s, const mr_serve_load* load, mr_serve_report* rep);
+/* host copies of the device right-hand side generator and result hash */
+int mr_serve_rhs(uint64_t seed, int64_t n_cols, int k, float* out);
+int mr_serve_hash(const float* y, int64_t n_rows, int k, uint64_t* hash);
+
 #ifdef __cplusplus
 }
 #endif

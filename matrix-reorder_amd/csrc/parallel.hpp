@@ -10,9 +10,17 @@
 
 namespace mr {
 
+// per-thread cap on the worker count of the helpers below (0 = no cap), so a
+// background task (a reorder beside a serving loop) can leave cores free
+inline int& host_thread_cap()
+{
+    static thread_local int cap = 0;
+    return cap;
+}
+
 inline int host_threads()
 {
-    static int n = [] {
+    static const int n = [] {
         int h = (int)std::thread::hardware_concurrency();
         if (h <= 0) h = 4;
         if (const char* e = std::getenv("MR_HOST_THREADS")) {
@@ -21,7 +29,8 @@ inline int host_threads()
         }
         return std::min(h, 64);
     }();
-    return n;
+    const int cap = host_thread_cap();
+    return cap > 0 ? std::min(n, cap) : n;
 }
 
 // fn(t, begin, end) on contiguous chunks of [0, n), one chunk per thread

@@ -64,6 +64,32 @@ class RDStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int), ("max_jobs", C.c_int), ("coalesce", C.c_int), ("verify", C.c_int),
+                ("sigma", C.c_int), ("grid", C.c_int), ("upload_threads", C.c_int), ("pad", C.c_int),
+                ("slot_bytes", C.c_int64), ("hbm_budget", C.c_int64), ("hbm_reserve", C.c_int64),
+                ("timeout_ms", C.c_double)]
+
+
+class ServeLoad(C.Structure):
+    _fields_ = [("rate", C.c_double), ("n_requests", C.c_int64), ("seed", C.c_uint64),
+                ("k_choices", C.c_int * 4), ("n_k", C.c_int), ("upload_every", C.c_int),
+                ("upload_scale", C.c_int), ("upload_edge_factor", C.c_int)]
+
+
+class ServeReport(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("rejected", C.c_int64), ("timed_out", C.c_int64),
+                ("cancelled", C.c_int64), ("uploads_admitted", C.c_int64), ("uploads_rejected", C.c_int64),
+                ("evictions", C.c_int64), ("p50_us", C.c_double), ("p90_us", C.c_double), ("p99_us", C.c_double),
+                ("mean_us", C.c_double), ("max_us", C.c_double), ("wall_s", C.c_double), ("gflops", C.c_double),
+                ("gbytes_s", C.c_double), ("offered_rate", C.c_double), ("achieved_rate", C.c_double),
+                ("batches", C.c_double), ("mean_jobs_per_batch", C.c_double),
+                ("mean_requests_per_batch", C.c_double), ("upload_ms_mean", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
 _U32P = C.POINTER(C.c_uint32)
@@ -130,6 +156,19 @@ SIGNATURES = {
     "mr_batch_graph_capture": [_P, _P],
     "mr_batch_graph_launch": [_P, _P],
     "mr_batch_info": [_P, _I64P, _I64P, _DP, _DP],
+    "mr_serve_config_default": [C.POINTER(ServeConfig)],
+    "mr_serve_create": [_P, C.POINTER(ServeConfig), C.POINTER(_P)],
+    "mr_serve_destroy": [_P],
+    "mr_serve_add": [_P, _P, C.POINTER(C.c_int)],
+    "mr_serve_upload_rmat": [_P, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_int)],
+    "mr_serve_upload_wait": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP],
+    "mr_serve_submit": [_P, C.c_int, C.c_int, C.c_uint64, _I64P],
+    "mr_serve_cancel": [_P, C.c_int64],
+    "mr_serve_wait": [_P, C.c_int64, C.POINTER(C.c_int), _DP, C.POINTER(C.c_uint64)],
+    "mr_serve_result": [_P, C.c_int64, _FP],
+    "mr_serve_run_load": [_P, C.POINTER(ServeLoad), C.POINTER(ServeReport)],
+    "mr_serve_rhs": [C.c_uint64, C.c_int64, C.c_int, _FP],
+    "mr_serve_hash": [_FP, C.c_int64, C.c_int, C.POINTER(C.c_uint64)],
 }
 _RESTYPE = {"mr_last_error": C.c_char_p, "mr_version": C.c_char_p}
 
@@ -604,6 +643,95 @@ class Batch:
         t, z, b, f = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
         check(lib().mr_batch_info(self.h, C.byref(t), C.byref(z), C.byref(b), C.byref(f)))
         return {"tiles": t.value, "nnz": z.value, "bytes": b.value, "flops": f.value}
+
+
+def serve_rhs(seed: int, n_cols: int, k: int) -> np.ndarray:
+    """Host copy of the serving loop's right-hand sides: X[n_cols][k]."""
+    out = np.empty((max(n_cols, 0), k), np.float32)
+    check(lib().mr_serve_rhs(seed, n_cols, k, ptr(out, C.c_float)))
+    return out
+
+
+def serve_hash(y: np.ndarray) -> int:
+    """Host copy of the serving loop's result hash of Y[n_rows][k]."""
+    y = np.ascontiguousarray(y, np.float32)
+    y2 = y.reshape(y.shape[0], -1) if y.ndim > 1 else y.reshape(-1, 1)
+    h = C.c_uint64(0)
+    check(lib().mr_serve_hash(ptr(y2, C.c_float), y2.shape[0], y2.shape[1], C.byref(h)))
+    return h.value
+
+
+class Server:
+    """The serving loop (mr_serve_*)."""
+
+    def __init__(self, ctx: Ctx, **cfg):
+        self.ctx = ctx
+        c = ServeConfig()
+        check(lib().mr_serve_config_default(C.byref(c)))
+        for key, v in cfg.items():
+            setattr(c, key, v)
+        self.cfg = c
+        h = _P()
+        check(lib().mr_serve_create(ctx.h, C.byref(c), C.byref(h)))
+        self.h = h
+        self._mats = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mr_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, mat: "Mat") -> int:
+        i = C.c_int(-1)
+        check(lib().mr_serve_add(self.h, mat.h, C.byref(i)))
+        self._mats.append(mat)
+        return i.value
+
+    def upload_rmat(self, scale: int, edge_factor: int = 16, seed: int = 1) -> int:
+        t = C.c_int(-1)
+        check(lib().mr_serve_upload_rmat(self.h, scale, edge_factor, seed, C.byref(t)))
+        return t.value
+
+    def upload_wait(self, ticket: int):
+        st, mid, ms = C.c_int(), C.c_int(), C.c_double()
+        check(lib().mr_serve_upload_wait(self.h, ticket, C.byref(st), C.byref(mid), C.byref(ms)))
+        return st.value, mid.value, ms.value
+
+    def submit(self, mat_id: int, k: int, seed: int) -> int:
+        r = C.c_int64(-1)
+        check(lib().mr_serve_submit(self.h, mat_id, k, seed, C.byref(r)))
+        return r.value
+
+    def cancel(self, req: int):
+        check(lib().mr_serve_cancel(self.h, req))
+
+    def wait(self, req: int):
+        st, lat, h = C.c_int(), C.c_double(), C.c_uint64()
+        check(lib().mr_serve_wait(self.h, req, C.byref(st), C.byref(lat), C.byref(h)))
+        return st.value, lat.value, h.value
+
+    def result(self, req: int, n_rows: int, k: int) -> np.ndarray:
+        y = np.empty((n_rows, k), np.float32)
+        check(lib().mr_serve_result(self.h, req, ptr(y, C.c_float)))
+        return y
+
+    def run_load(self, rate: float, n_requests: int, seed: int = 1, ks=(1, 4, 16), upload_every: int = 0,
+                 upload_scale: int = 18, upload_edge_factor: int = 16) -> dict:
+        ld = ServeLoad()
+        ld.rate, ld.n_requests, ld.seed = rate, n_requests, seed
+        for i, k in enumerate(ks):
+            ld.k_choices[i] = k
+        ld.n_k = len(ks)
+        ld.upload_every, ld.upload_scale, ld.upload_edge_factor = upload_every, upload_scale, upload_edge_factor
+        rep = ServeReport()
+        check(lib().mr_serve_run_load(self.h, C.byref(ld), C.byref(rep)))
+        return rep.as_dict()
 
 
 class Graph:

@@ -138,3 +138,26 @@ def test_check_perm(mr):
     assert mr.check_perm(np.arange(50))
     assert not mr.check_perm(np.array([0, 2, 2]))
     assert not mr.check_perm(np.array([0, 5]))
+
+
+def test_serve_rhs_and_hash_host_side(mr):
+    """Right-hand sides of the serving loop: deterministic per (seed, index),
+    on the 2^-23 grid of [-1, 1), independent across seeds; the result hash
+    changes with any single element and depends on the element position."""
+    a = mr.serve_rhs(123, 5000, 4)
+    assert a.shape == (5000, 4) and np.array_equal(a, mr.serve_rhs(123, 5000, 4))
+    assert a.min() >= -1.0 and a.max() < 1.0
+    assert np.all(np.ldexp(a.astype(np.float64) + 1.0, 23) == np.floor(np.ldexp(a.astype(np.float64) + 1.0, 23)))
+    assert abs(float(a.mean())) < 0.02 and abs(float(a.var()) - 1.0 / 3.0) < 0.01
+    b = mr.serve_rhs(124, 5000, 4)
+    assert np.mean(a == b) < 0.01
+    # element j of (n, k) is element j of the flattened (n * k, 1) stream
+    assert np.array_equal(a.ravel(), mr.serve_rhs(123, 20000, 1).ravel())
+    h = mr.serve_hash(a)
+    assert h == mr.serve_hash(a.copy())
+    c = a.copy()
+    c[1234, 2] = np.nextafter(c[1234, 2], np.float32(2.0))
+    assert mr.serve_hash(c) != h
+    d = a.copy()
+    d[[0, 1]] = d[[1, 0]]
+    assert mr.serve_hash(d) != h

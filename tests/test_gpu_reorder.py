@@ -76,6 +76,9 @@ def pgorder_cases():
     rows = [np.sort(rng.choice(n, 12000, replace=False)) if i in (5, 9000) else
             np.sort(rng.choice(n, int(rng.integers(0, 6)), replace=False)) for i in range(n)]
     out.append(("hubs_30000_w1024", *csr_rows(rows, n), 1024))
+    # hundreds of batches: the column lists are compacted every 32 batches
+    out.append(("rmat_s13_w32_compacted", *rmat_csr(13, 16, seed=13), 32))
+    out.append(("rmat_s15_w64_compacted", *rmat_csr(15, 16, seed=15), 64))
     return out
 
 
