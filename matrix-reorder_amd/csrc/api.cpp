@@ -9,6 +9,7 @@
 
 #include "common.hpp"
 #include "kernels/launch.hpp"
+#include "kernels/launch_perm.hpp"
 
 namespace mr {
 
@@ -52,6 +53,7 @@ int merbit_build(mr_mat m, hipStream_t s)
     // the 4096-item SpMM grid) reads inside it
     const int64_t words = (p.items + 8191) / 8192 * (8192 / 32);
     p.mm_built = false;
+    p.xc_built = false;
     for (int i = 0; i < 3; ++i) p.cnb[i] = -1;
     MR_TRY(p.flags.alloc(sizeof(uint32_t) * (size_t)(words > 0 ? words : 1)));
     MR_TRY(p.blk_x.alloc(sizeof(int32_t) * (size_t)(p.n_blocks + 1)));
@@ -116,6 +118,67 @@ int merbit_mm_grid(mr_mat m, hipStream_t s)
     MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), 4096, p.mm_blocks, p.mm_x.as<int32_t>(),
                                 p.mm_y.as<int32_t>(), s));
     p.mm_built = true;
+    return MR_OK;
+}
+
+int merbit_xc_build(mr_mat m, hipStream_t s)
+{
+    MerbitPlan& p = m->merbit;
+    if (m->dtype == MR_F64) return set_error(MR_ERR_UNSUPPORTED, "the x-cache MERBIT variant needs F32/F16/BF16 values");
+    if (!p.built) MR_TRY(merbit_build(m, s));
+    if (p.xc_built) return MR_OK;
+    int max_lds = 0;
+    MR_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->ctx->device));
+    // as many hot columns as the workgroup's LDS holds
+    int64_t cap = max_lds > 512 ? ((int64_t)max_lds / 4 - 64) / 64 * 64 : 0;
+    if (p.xc_req >= 0) cap = std::min<int64_t>(cap, p.xc_req);
+    cap = std::max<int64_t>(0, std::min<int64_t>(cap, m->n_cols));
+    const int64_t nc = std::max<int64_t>(m->n_cols, 1);
+    // column degrees -> the `cap` columns with most nonzeros (ties: lower id)
+    std::vector<int32_t> deg((size_t)nc, 0);
+    if (m->nnz > 0 && cap > 0) {
+        DevBuf cnt;
+        MR_TRY(cnt.alloc(sizeof(int32_t) * (size_t)nc));
+        MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
+        MR_HIP_TRY(k::transpose_count(m->nnz, m->ci.as<int32_t>(), cnt.as<int32_t>(), s));
+        MR_HIP_TRY(hipMemcpyAsync(deg.data(), cnt.p, sizeof(int32_t) * (size_t)nc, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+    }
+    std::vector<int32_t> idx((size_t)nc);
+    for (int64_t i = 0; i < nc; ++i) idx[(size_t)i] = (int32_t)i;
+    auto hotter = [&](int32_t a, int32_t b) {
+        return deg[(size_t)a] != deg[(size_t)b] ? deg[(size_t)a] > deg[(size_t)b] : a < b;
+    };
+    int64_t xc = cap;
+    if (xc > 0) {
+        std::partial_sort(idx.begin(), idx.begin() + xc, idx.end(), hotter);
+        while (xc > 0 && deg[(size_t)idx[(size_t)xc - 1]] == 0) --xc;   // never cache an unused column
+    }
+    std::vector<int32_t> map((size_t)nc);
+    for (int64_t i = 0; i < nc; ++i) map[(size_t)i] = (int32_t)i;
+    for (int64_t sl = 0; sl < xc; ++sl) map[(size_t)idx[(size_t)sl]] = (int32_t)(0x80000000u | (uint32_t)sl);
+    const int64_t T = 64LL * p.xs;
+    p.xtiles = (p.items + T - 1) / T;
+    const int64_t padded = std::max<int64_t>(p.xtiles * T, T);
+    DevBuf dmap, tx;
+    MR_TRY(dmap.alloc(sizeof(int32_t) * (size_t)nc));
+    MR_TRY(tx.alloc(sizeof(int32_t) * (size_t)(p.xtiles + 1)));
+    MR_TRY(p.hot.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(xc, 1)));
+    MR_TRY(p.pc.alloc(sizeof(int32_t) * (size_t)padded));
+    MR_TRY(p.pv.alloc(dtype_size(m->dtype) * (size_t)padded));
+    MR_TRY(p.xty.alloc(sizeof(int32_t) * (size_t)(p.xtiles + 1)));
+    MR_TRY(p.xcarry.alloc(sizeof(float) * (size_t)std::max<int64_t>(p.xtiles, 1)));
+    MR_TRY(p.xcarry_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(p.xtiles, 1)));
+    MR_HIP_TRY(hipMemcpyAsync(dmap.p, map.data(), sizeof(int32_t) * (size_t)nc, hipMemcpyHostToDevice, s));
+    if (xc > 0)
+        MR_HIP_TRY(hipMemcpyAsync(p.hot.p, idx.data(), sizeof(int32_t) * (size_t)xc, hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(k::merbit_lp_build(m->dtype, m->n_rows, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p,
+                                  dmap.as<int32_t>(), p.items, padded, p.pc.as<int32_t>(), p.pv.p, s));
+    MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), T, p.xtiles, tx.as<int32_t>(),
+                                p.xty.as<int32_t>(), s));
+    MR_HIP_TRY(hipStreamSynchronize(s));   // dmap and tx are freed on return
+    p.xc = (int)xc;
+    p.xc_built = true;
     return MR_OK;
 }
 
@@ -495,6 +558,14 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
     case MR_KERNEL_MERBIT: {
         MerbitPlan& p = m->merbit;
         if (!p.built) MR_TRY(merbit_build(m, st));
+        if ((p.policy & 4) && m->dtype != MR_F64) {
+            MR_TRY(merbit_xc_build(m, st));
+            MR_HIP_TRY(k::merbit_lp_spmv(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles,
+                                         p.pc.as<int32_t>(), p.pv.p, p.xty.as<int32_t>(), x, p.hot.as<int32_t>(),
+                                         p.xc, y, p.xcarry.p, p.xcarry_row.as<int32_t>(),
+                                         std::max(m->ctx->cu_count, 1), st));
+            return MR_OK;
+        }
         if (m->dtype == MR_F64 && p.sigma > 16)
             return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
         MR_HIP_TRY(k::merbit_spmv(m->dtype, p.sigma, p.policy, m->n_rows, p.n_blocks, m->ci.as<int32_t>(), m->val.p,
@@ -528,7 +599,11 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
 int mr_merbit_ws_bytes(mr_mat m, size_t* bytes)
 {
     if (!m || !bytes) return set_error(MR_ERR_INVALID_ARG, "null argument");
-    const size_t nb = (size_t)(m->merbit.n_blocks > 0 ? m->merbit.n_blocks : 1);
+    const MerbitPlan& p = m->merbit;
+    size_t nb = (size_t)(p.n_blocks > 0 ? p.n_blocks : 1);
+    // the x-cache variant needs one carry per tile of 64 x xs items
+    const int64_t xt = (p.items + 64LL * p.xs - 1) / (64LL * p.xs);
+    if ((p.policy & 4) && m->dtype != MR_F64) nb = std::max<size_t>(nb, (size_t)std::max<int64_t>(xt, 1));
     *bytes = ((nb * vec_size(m->dtype) + 255) / 256) * 256 + nb * sizeof(int32_t);
     return MR_OK;
 }
@@ -541,6 +616,16 @@ int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s)
     hipStream_t st = stream_of(m->ctx, s);
     MerbitPlan& p = m->merbit;
     if (!p.built) MR_TRY(merbit_build(m, st));
+    if ((p.policy & 4) && m->dtype != MR_F64) {
+        MR_TRY(merbit_xc_build(m, st));
+        const size_t nt = (size_t)std::max<int64_t>(p.xtiles, 1);
+        char* base = static_cast<char*>(ws);
+        int32_t* crow = reinterpret_cast<int32_t*>(base + ((nt * sizeof(float) + 255) / 256) * 256);
+        MR_HIP_TRY(k::merbit_lp_spmv(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles, p.pc.as<int32_t>(),
+                                     p.pv.p, p.xty.as<int32_t>(), x, p.hot.as<int32_t>(), p.xc, y, base, crow,
+                                     std::max(m->ctx->cu_count, 1), st));
+        return MR_OK;
+    }
     if (m->dtype == MR_F64 && p.sigma > 16)
         return set_error(MR_ERR_UNSUPPORTED, "F64 MERBIT supports sigma <= 16");
     const size_t nb = (size_t)(p.n_blocks > 0 ? p.n_blocks : 1);
@@ -595,7 +680,9 @@ int mr_merbit_configure(mr_mat m, int sigma)
 int mr_merbit_options(mr_mat m, int policy)
 {
     if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
-    if (policy < 0 || policy > 3) return set_error(MR_ERR_INVALID_ARG, "policy must be in [0, 3]");
+    if (policy < 0 || policy > 7) return set_error(MR_ERR_INVALID_ARG, "policy must be in [0, 7]");
+    if ((policy & 4) && m->dtype == MR_F64)
+        return set_error(MR_ERR_UNSUPPORTED, "the x-cache MERBIT variant needs F32/F16/BF16 values");
     m->merbit.policy = policy;
     return MR_OK;
 }
@@ -615,36 +702,96 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
     MR_HIP_TRY(hipEventCreate(&e0));
     MR_HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    int bs = m->merbit.sigma, bp = m->merbit.policy;
+    int bs = m->merbit.sigma, bp = m->merbit.policy, bxs = m->merbit.xs, bxw = m->merbit.xw;
+    auto time_it = [&](float* us) -> int {
+        MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
+        MR_HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
+        MR_HIP_TRY(hipEventRecord(e1, s));
+        MR_HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        MR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *us = ms * 1000.0f / (float)reps;
+        return MR_OK;
+    };
+    // workgroup tiles: every sigma with the four load policies
     const int sigmas[3] = {8, 16, 32};
-    for (int si = 0; si < 3; ++si) {
+    int rc = MR_OK;
+    for (int si = 0; si < 3 && rc == MR_OK; ++si) {
         const int sg = sigmas[si];
         if (m->dtype == MR_F64 && sg > 16) continue;
         m->merbit.sigma = sg;
-        MR_TRY(merbit_build(m, s));
-        for (int pol = 0; pol < 4; ++pol) {
+        rc = merbit_build(m, s);
+        for (int pol = 0; pol < 4 && rc == MR_OK; ++pol) {
             m->merbit.policy = pol;
-            MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
-            MR_HIP_TRY(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r) MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
-            MR_HIP_TRY(hipEventRecord(e1, s));
-            MR_HIP_TRY(hipEventSynchronize(e1));
-            float ms = 0;
-            MR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            const float us = ms * 1000.0f / (float)reps;
-            if (us < best) { best = us; bs = sg; bp = pol; }
+            float us = 0;
+            rc = time_it(&us);
+            if (rc == MR_OK && us < best) { best = us; bs = sg; bp = pol; }
+        }
+    }
+    if (rc == MR_OK && bs != m->merbit.sigma) {
+        m->merbit.sigma = bs;
+        rc = merbit_build(m, s);
+    }
+    // wave tiles with the LDS x-cache: (items per lane, waves) x streaming hint
+    if (m->dtype != MR_F64) {
+        const int shapes[5][2] = {{8, 16}, {16, 16}, {8, 8}, {16, 8}, {4, 16}};
+        for (int q = 0; q < 5 && rc == MR_OK; ++q) {
+            if (shapes[q][0] == 4 && m->dtype != MR_F32) continue;
+            m->merbit.xs = shapes[q][0];
+            m->merbit.xw = shapes[q][1];
+            m->merbit.xc_built = false;
+            for (int nt = 0; nt < 2 && rc == MR_OK; ++nt) {
+                m->merbit.policy = 4 | nt;
+                float us = 0;
+                rc = time_it(&us);
+                if (rc == MR_OK && us < best) { best = us; bp = 4 | nt; bxs = shapes[q][0]; bxw = shapes[q][1]; }
+            }
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    if (rc != MR_OK) return rc;
     m->merbit.policy = bp;
-    if (m->merbit.sigma != bs) {
-        m->merbit.sigma = bs;
-        MR_TRY(merbit_build(m, s));
+    if (m->merbit.xs != bxs || m->merbit.xw != bxw) {
+        m->merbit.xs = bxs;
+        m->merbit.xw = bxw;
+        m->merbit.xc_built = false;
     }
+    if (bp & 4) MR_TRY(merbit_xc_build(m, s));
     if (best_sigma) *best_sigma = bs;
     if (best_policy) *best_policy = bp;
     if (best_us) *best_us = best;
+    return MR_OK;
+}
+
+int mr_merbit_xc_configure(mr_mat m, int items_per_lane, int waves, int cached_cols)
+{
+    MR_TRY(check_mat(m));
+    if (m->dtype == MR_F64) return set_error(MR_ERR_UNSUPPORTED, "the x-cache MERBIT variant needs F32/F16/BF16 values");
+    const int S = items_per_lane, W = waves;
+    if (!((S == 8 && W == 16) || (S == 16 && W == 16) || (S == 8 && W == 8) || (S == 16 && W == 8) ||
+          (S == 4 && W == 16)))
+        return set_error(MR_ERR_INVALID_ARG,
+                         "x-cache tiles: (items_per_lane, waves) must be (8,16), (16,16), (8,8), (16,8) or (4,16)");
+    if (S == 4 && m->dtype != MR_F32)
+        return set_error(MR_ERR_UNSUPPORTED, "x-cache tiles of 4 items per lane need F32 values");
+    if (cached_cols < -1) return set_error(MR_ERR_INVALID_ARG, "cached_cols must be >= -1");
+    m->merbit.xs = S;
+    m->merbit.xw = W;
+    m->merbit.xc_req = cached_cols;
+    m->merbit.xc_built = false;
+    return merbit_xc_build(m, m->ctx->stream);
+}
+
+int mr_merbit_xc_info(mr_mat m, int* items_per_lane, int* waves, int* cached_cols, int64_t* tiles)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    const MerbitPlan& p = m->merbit;
+    if (items_per_lane) *items_per_lane = p.xs;
+    if (waves) *waves = p.xw;
+    if (cached_cols) *cached_cols = p.xc_built ? p.xc : -1;
+    if (tiles) *tiles = p.xc_built ? p.xtiles : 0;
     return MR_OK;
 }
 

@@ -113,6 +113,16 @@ struct MerbitPlan {
     // pick their own sigma: index 0 / 1 / 2 = sigma 8 / 16 / 32, built on use
     mr::DevBuf cx[3], cy[3];
     int64_t cnb[3] = {-1, -1, -1};
+    // lane-padded wave-tile variant with an LDS cache of hot columns (policy
+    // bit 2): tiles of 64 x xs merge items, xw waves per persistent workgroup,
+    // xc cached columns (xc_req -1 = as many as the LDS holds); pc / pv = the
+    // merge path as padded column / value streams, xty = rows ended before
+    // every tile
+    int xs = 8, xw = 16, xc_req = -1;
+    int xc = 0;
+    int64_t xtiles = 0;
+    mr::DevBuf pc, pv, hot, xty, xcarry, xcarry_row;
+    bool xc_built = false;
     bool built = false;
     double build_ms = 0.0;
 };
@@ -139,4 +149,6 @@ int merbit_build(mr_mat m, hipStream_t s);
 int merbit_coords_sigma(mr_mat m, int sigma, hipStream_t s, const int32_t** bx, const int32_t** by, int64_t* nb);
 // the 4096-item SpMM grid of m, built on first use
 int merbit_mm_grid(mr_mat m, hipStream_t s);
+// metadata of the wave-tile / x-cache variant, built on first use
+int merbit_xc_build(mr_mat m, hipStream_t s);
 }  // namespace mr

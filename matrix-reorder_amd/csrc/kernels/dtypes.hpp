@@ -17,7 +17,11 @@ template <> struct Acc<double> { using type = double; };
 __device__ __forceinline__ float to_f(float v) { return v; }
 __device__ __forceinline__ double to_f(double v) { return v; }
 __device__ __forceinline__ float to_f(__half v) { return __half2float(v); }
-__device__ __forceinline__ float to_f(__hip_bfloat16 v) { return __bfloat162float(v); }
+// BF16 -> F32 is an exact widening: the 16 stored bits become the high half
+__device__ __forceinline__ float to_f(__hip_bfloat16 v)
+{
+    return __uint_as_float((uint32_t)__bfloat16_as_ushort(v) << 16);
+}
 
 // dtype codes match mr_dtype
 enum { DT_F64 = 0, DT_F32 = 1, DT_F16 = 2, DT_BF16 = 3 };

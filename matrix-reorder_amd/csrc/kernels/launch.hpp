@@ -74,5 +74,17 @@ hipError_t merbit_spmv(int dt, int sigma, int policy, int64_t n_rows, int64_t n_
                        const int32_t* blk_y, const void* x, void* y, void* carry,
                        int32_t* carry_row, hipStream_t s);
 
+// ---- MERBIT on lane-padded wave tiles with an LDS cache of the xc hottest
+// columns (merbit_lp.hip); F32 / F16 / BF16 storage only.  The merge path is
+// stored as pc / pv streams of `padded` items (column or -1 at a row end, -2
+// after the last item; a cached column is slot | 1 << 31).
+// (S, W) in {(8,16), (16,16), (8,8), (16,8), (4,16)} (S = 4 for F32 only)
+size_t merbit_lp_lds_bytes(int xc);
+hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int32_t* ci, const void* val,
+                           const int32_t* map, int64_t items, int64_t padded, int32_t* pc, void* pv, hipStream_t s);
+hipError_t merbit_lp_spmv(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pc,
+                          const void* pv, const int32_t* ty, const void* x, const int32_t* hot, int xc, void* y,
+                          void* carry, int32_t* carry_row, int grid, hipStream_t s);
+
 }  // namespace k
 }  // namespace mr

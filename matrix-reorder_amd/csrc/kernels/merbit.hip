@@ -38,6 +38,7 @@
 //    its private L2 sees the x window of that row range.
 #include "dtypes.hpp"
 #include "launch.hpp"
+#include "ldst.hpp"
 
 namespace mr {
 namespace k {
@@ -92,34 +93,6 @@ __device__ __forceinline__ int64_t remap_block(int64_t i, int64_t n)
     const int64_t full = (n / 8) * 8;
     if (i >= full) return i;
     return (i % 8) * (n / 8) + (i / 8);
-}
-
-template <typename T> __device__ __forceinline__ T wave_sum(T v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// streaming loads/stores: with NT the col/val streams and the y stores carry
-// the non-temporal hint so that they do not displace x from the caches
-template <bool NT, typename T> __device__ __forceinline__ T ld_stream(const T* p)
-{
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <bool NT> __device__ __forceinline__ __half ld_stream(const __half* p)
-{
-    return __ushort_as_half(ld_stream<NT>(reinterpret_cast<const unsigned short*>(p)));
-}
-template <bool NT> __device__ __forceinline__ __hip_bfloat16 ld_stream(const __hip_bfloat16* p)
-{
-    return __ushort_as_bfloat16(ld_stream<NT>(reinterpret_cast<const unsigned short*>(p)));
-}
-template <bool NT, typename T> __device__ __forceinline__ void st_stream(T* p, T v)
-{
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
 }
 
 // one MERBIT block b of one matrix (shared by the single-matrix kernel and the

@@ -147,6 +147,8 @@ SIGNATURES = {
     "mr_merbit_autotune": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _FP],
     "mr_merbit_info": [_P, C.POINTER(C.c_int), _I64P, _I64P, _DP],
     "mr_merbit_export": [_P, _U32P, _I32P, _I32P],
+    "mr_merbit_xc_configure": [_P, C.c_int, C.c_int, C.c_int],
+    "mr_merbit_xc_info": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I64P],
     "mr_batch_create": [_P, C.c_int, C.POINTER(_P), C.c_int, C.POINTER(_P)],
     "mr_batch_destroy": [_P],
     "mr_batch_set_io": [_P, C.POINTER(_P), _I64P, C.POINTER(_P), _I64P, _P],
@@ -574,6 +576,14 @@ class Mat:
         sg, pol, us = C.c_int(), C.c_int(), C.c_float()
         check(lib().mr_merbit_autotune(self.h, reps, C.byref(sg), C.byref(pol), C.byref(us)))
         return {"sigma": sg.value, "policy": pol.value, "us": us.value}
+
+    def merbit_xc_configure(self, items_per_lane: int = 16, waves: int = 16, cached_cols: int = -1):
+        check(lib().mr_merbit_xc_configure(self.h, items_per_lane, waves, cached_cols))
+
+    def merbit_xc_info(self):
+        s, w, c, t = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        check(lib().mr_merbit_xc_info(self.h, C.byref(s), C.byref(w), C.byref(c), C.byref(t)))
+        return {"items_per_lane": s.value, "waves": w.value, "cached_cols": c.value, "tiles": t.value}
 
     def merbit_info(self):
         s, it, nb, ms = C.c_int(), C.c_int64(), C.c_int64(), C.c_double()

@@ -87,6 +87,7 @@ def test_batch_spmv_equals_single(ctx, mr, oracle, sigma, nt, grid):
     ctx.sync()
     for j, m in enumerate(mats):
         m.merbit_configure(sigma)
+        m.merbit_options(1)   # the workgroup-tile kernel the batch runs
         rp, ci, val, nc = host[j]
         got = ys[j].to_host()
         want = single(ctx, mr, m, xs[j], len(rp) - 1, 1, np.float32)
@@ -215,6 +216,7 @@ def test_batch_autotune_keeps_results(ctx, mr):
     ctx.sync()
     for j, m in enumerate(mats):
         m.merbit_configure(cfg["sigma"])
+        m.merbit_options(1)
         want = single(ctx, mr, m, xs[j], len(host[j][0]) - 1, 1, np.float32)
         assert np.array_equal(ys[j].to_host().view(np.uint32), want.view(np.uint32)), j
     info = b.info()

@@ -40,6 +40,7 @@ def stored_f64(mr, v, store):
 def alone_library(ctx, mr, mat, n, k, seed):
     """The same request through mr_spmv / mr_spmm (MERBIT) on its own."""
     x = mr.serve_rhs(seed, n, k)
+    mat.merbit_options(1)   # the workgroup-tile kernel the server's batches run
     xd = ctx.upload(x)
     yd = ctx.empty((n, k), np.float32)
     if k == 1:

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import spmv_cases
+from conftest import rmat_csr, spmv_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -192,3 +192,78 @@ def test_spmm_merbit(ctx, mr, oracle, case, k):
 def sp_abs(rp, ci, val, nc):
     import scipy.sparse as sp
     return sp.csr_matrix((np.abs(val), ci, rp), shape=(len(rp) - 1, nc))
+
+
+XC_SHAPES = [(8, 16), (16, 16), (8, 8), (16, 8), (4, 16)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("shape", XC_SHAPES, ids=lambda s: f"S{s[0]}W{s[1]}")
+@pytest.mark.parametrize("cached", [-1, 0, 7])
+def test_merbit_xcache_f32_parity(ctx, mr, oracle, case, shape, cached):
+    """Wave-tile MERBIT with the LDS cache of hot columns: every tile shape,
+    cache sizes from none to the LDS capacity, FP32 within 1e-5 of the FP64
+    oracle; the fix-up joins rows spanning many tiles."""
+    name, rp, ci, nc = case
+    rng = np.random.default_rng(17)
+    val = rng.uniform(-1, 1, len(ci))
+    x = rng.uniform(-1, 1, nc).astype(np.float32)
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=mr.F32)
+    mat.merbit_xc_configure(shape[0], shape[1], cached)
+    mat.merbit_options(5)
+    info = mat.merbit_xc_info()
+    assert info["items_per_lane"] == shape[0] and info["waves"] == shape[1]
+    n = len(rp) - 1
+    assert info["tiles"] == -(-(n + len(ci)) // (64 * shape[0]))
+    if cached >= 0:
+        assert info["cached_cols"] <= cached
+    y = spmv(ctx, mr, mat, x, mr.MERBIT)
+    yhat, absrow = oracle.spmv_f64(rp, ci, val.astype(np.float32).astype(np.float64), x.astype(np.float64))
+    worst, l2, _ = oracle.spmv_parity(y, yhat, absrow)
+    assert worst <= 1e-5 and l2 <= 1e-5, (worst, l2)
+    mat.close()
+
+
+@pytest.mark.parametrize("store", ["bf16", "f16"])
+def test_merbit_xcache_low_precision_and_repro(ctx, mr, oracle, store):
+    """BF16 / FP16 values with FP32 accumulation through the x-cache kernel:
+    within 1e-5 of the FP64 oracle on the stored values, identical bits on
+    every run, identical through the caller-workspace entry point."""
+    rp, ci = rmat_csr(13, 16, seed=5)
+    n = len(rp) - 1
+    rng = np.random.default_rng(3)
+    val = rng.uniform(-1, 1, len(ci))
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    st = mr.BF16 if store == "bf16" else mr.F16
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, store=st)
+    mat.merbit_options(4)
+    _, _, vs = mat.export()
+    if st == mr.BF16:
+        v64 = (vs.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    else:
+        v64 = vs.view(np.float16).astype(np.float64)
+    y1 = spmv(ctx, mr, mat, x, mr.MERBIT)
+    y2 = spmv(ctx, mr, mat, x, mr.MERBIT)
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+    yhat, absrow = oracle.spmv_f64(rp, ci, v64, x.astype(np.float64))
+    worst, l2, _ = oracle.spmv_parity(y1, yhat, absrow)
+    assert worst <= 1e-5 and l2 <= 1e-5, (worst, l2)
+    ws = ctx.empty(mat.spmv_ws_bytes() // 4 + 1, np.float32)
+    xd = ctx.upload(x)
+    yd = ctx.empty(n, np.float32)
+    mat.spmv_ws(xd, yd, ws)
+    ctx.sync()
+    assert np.array_equal(yd.to_host().view(np.uint32), y1.view(np.uint32))
+    mat.close()
+
+
+def test_merbit_xcache_rejected_for_f64(ctx, mr):
+    rp, ci = rmat_csr(8, 4, seed=1)
+    mat = mr.Mat.from_csr(ctx, rp, ci, None, store=mr.F64)
+    with pytest.raises(mr.MRError) as e:
+        mat.merbit_options(4)
+    assert e.value.code == 5
+    with pytest.raises(mr.MRError) as e:
+        mat.merbit_xc_configure(16, 16, -1)
+    assert e.value.code == 5
+    mat.close()
