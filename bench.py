@@ -1,15 +1,29 @@
-"""Benchmark: SpMV throughput after pGOrder on one MI355X.
+"""Benchmark of the SpMV locality path on one MI355X.
 
-Workload (the single-large-SpMV configuration): R-MAT scale 22 (4,194,304
-rows), edge factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 2, duplicates
-merged, self loops kept, int32 CSR; FP32 values and x uniform in [-1,1].
-Pipeline: pGOrder (GPU placement loop + per-batch CPU Gorder, window 5) on the
-device-resident matrix, then MERBIT SpMV on the relabelled matrix.  One step =
-one y = A x over the whole matrix with A, x, y resident in HBM.
+Headline workload (the single-large-SpMV configuration): R-MAT scale 22
+(4,194,304 rows), edge factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 2,
+duplicates merged, self loops kept, int32 CSR; FP32 values and x uniform in
+[-1,1].  Pipeline: pGOrder (GPU placement loop + per-batch CPU Gorder, window
+5) on the device-resident matrix, the MERBIT autotuner, then MERBIT SpMV on the
+relabelled matrix.  One step = one y = A x over the whole matrix with A, x, y
+resident in HBM; `value` is GFLOP/s = 2 nnz / step time (hipEvents on the
+launch stream around the timed steps).
 
-Prints ONE JSON line on stdout; progress goes to stderr.  Extra figures
-(unreordered MERBIT, measured vs RD-predicted speedup, CSR-vector and
-BF16-value runs, pGOrder phase times) are under "extra".
+Further sections (reported under "extra", each on its own synthetic inputs):
+  precision  3-D 27-point stencil 160^3 and R-MAT s21 ef32 with log-uniform
+             values, stored BF16 / FP16 / FP32 (FP32 accumulation): max and
+             relative-L2 error against the FP64 GPU product, time per dtype;
+  batched    64 matrices (rows log-uniform in [2^10, 2^18], nnz/row
+             log-uniform in [4, 64], half R-MAT, half banded, seed 3), each
+             pGOrder-reordered, k = 1, 8, 32 right-hand sides in one
+             persistent fused launch (autotuned, captured in a hipGraph);
+  serving    32 resident reordered BF16 matrices, Poisson arrivals, k in
+             {1, 4, 16}, every 1000th request uploads an R-MAT s18 that is
+             reordered with pGOrder in the background; p50 / p99 latency;
+  cpu        the CPU baseline: the oracle's threaded CSR SpMV (FP32) on the
+             same matrix, plus host reorderers on a smaller R-MAT.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -18,6 +32,7 @@ import json
 import math
 import os
 import pathlib
+import re
 import sys
 import time
 
@@ -27,14 +42,24 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "matrix-reorder_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "merbit_lp_s22_fp32_pmc.md"
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def time_spmv(ctx, mr, mat, xd, yd, kernel, steps, warmup):
-    s = ctx.stream()
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def time_spmv(ctx, mr, mat, xd, yd, kernel, steps, warmup, stream=None):
+    """Average ms per SpMV over `steps` launches on one stream (hipEvents), and
+    the host wall time per step."""
+    s = stream or ctx.stream()
     for _ in range(warmup):
         mat.spmv(xd, yd, kernel, s)
     ctx.sync(s)
@@ -46,33 +71,329 @@ def time_spmv(ctx, mr, mat, xd, yd, kernel, steps, warmup):
     ctx.record(e1, s)
     ctx.sync(s)
     wall = (time.perf_counter() - t0) * 1e3
-    ev = ctx.elapsed(e0, e1)
-    return ev / steps, wall / steps
+    return ctx.elapsed(e0, e1) / steps, wall / steps
 
 
-def algorithmic_bytes(n, nnz, val_bytes, vec_bytes=4):
-    # nnz*(val + int32 col) + (n+1)*4 row pointer + n*|x| + n*|y|
-    return nnz * (val_bytes + 4) + (n + 1) * 4 + n * vec_bytes + n * vec_bytes
+def csr_bytes(n_rows, n_cols, nnz, val_bytes, vec_bytes=4, k=1):
+    """Algorithmic bytes of y = A X in canonical int32 CSR: every array once."""
+    return nnz * (val_bytes + 4) + (n_rows + 1) * 4 + k * (n_cols + n_rows) * vec_bytes
 
 
-def cpu_baseline(rp, ci, val32, x32, target_s=10.0):
+def traffic_from_profile():
+    """HBM bytes per launch of the MERBIT kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE scaled by the copy kernel)."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_PROFILE.read_text())
+    return float(m.group(1)) if m else None
+
+
+def err_stats(y, yref):
+    y = np.asarray(y, np.float64)
+    yref = np.asarray(yref, np.float64)
+    d = y - yref
+    nrm = float(np.linalg.norm(yref))
+    return {"max_abs": float(np.max(np.abs(d))) if d.size else 0.0,
+            "rel_l2": float(np.linalg.norm(d) / nrm) if nrm > 0 else 0.0}
+
+
+# ---------------------------------------------------------------------------
+def section_main(mr, ctx, a, extra):
+    log("generating R-MAT scale 22, edge factor 16, seed 2")
+    t = time.perf_counter()
+    rp, ci = mr.gen_rmat(22, 16, (0.57, 0.19, 0.19), seed=2)
+    n, nnz = len(rp) - 1, len(ci)
+    val = mr.gen_values(102, nnz).astype(np.float32)
+    x = mr.gen_values(103, n).astype(np.float32)
+    log(f"  n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
+    xd = ctx.upload(x)
+    yd = ctx.empty(n, np.float32)
+    if not a.only_main:
+        log("unreordered matrix: MERBIT (default variant) and the RD headroom predictor")
+        extra["merbit_unreordered_ms"], _ = time_spmv(ctx, mr, mat, xd, yd, mr.MERBIT, 30, 5)
+        t = time.perf_counter()
+        pred, feat, _ = mat.locality_predict(mr.MODEL_MERBIT)
+        extra["rd_predicted_speedup"] = pred
+        extra["rd_features"] = [round(float(v), 6) for v in feat]
+        extra["rd_ms"] = (time.perf_counter() - t) * 1e3
+        log(f"  {extra['merbit_unreordered_ms'] * 1e3:.1f} us/SpMV, predicted pGOrder speedup {pred:.3f}")
+    perm_path = pathlib.Path(a.perm_cache) if a.perm_cache else None
+    if perm_path is not None and perm_path.exists():
+        log(f"relabelling with the cached pGOrder permutation {perm_path}")
+        perm = np.load(perm_path)
+        mat.permute(perm)
+        extra["pgorder"] = "cached permutation"
+    else:
+        log("pGOrder (width auto, window 5)")
+        t = time.perf_counter()
+        perm, st = mat.reorder_pgorder(0, 5)
+        extra["pgorder"] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
+        extra["pgorder_wall_ms"] = (time.perf_counter() - t) * 1e3
+        log(f"  {json.dumps(extra['pgorder'])}")
+        if perm_path is not None:
+            np.save(perm_path, perm)
+    xp = ctx.empty(n, np.float32)
+    mat.permute_vec(0, xd, xp)           # x in the new order, outside the timed region
+    if not a.no_tune:
+        log("MERBIT autotuner")
+        best = mat.merbit_autotune(10)
+        extra["autotune"] = best
+        log(f"  {best}")
+    cfg = {"policy": mat.merbit_policy(), **mat.merbit_info(), **mat.merbit_xc_info()}
+    log(f"MERBIT FP32 after pGOrder: {a.warmup} warmup + {a.steps} timed steps")
+    ms, wall = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, a.steps, a.warmup)
+    gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
+    bytes_alg = csr_bytes(n, n, nnz, 4)
+    achieved = bytes_alg / (ms * 1e-3) / 1e9
+    log(f"  {ms * 1e3:.1f} us/SpMV (wall {wall * 1e3:.1f} us), {gflops:.1f} GFLOP/s, {achieved:.0f} GB/s")
+    res = {"ms": ms, "gflops": gflops, "achieved": achieved, "bytes": bytes_alg, "n": n, "nnz": nnz, "cfg": cfg}
+    # copy-kernel reference of the same session (measured bandwidth; also the
+    # calibration kernel of the rocprofv3 byte counters)
+    nb = 1 << 30
+    src, dst = ctx.empty(nb // 4, np.float32), ctx.empty(nb // 4, np.float32)
+    s = ctx.stream()
+    ctx.copy(dst, src, nb, s)
+    e0, e1 = ctx.event(), ctx.event()
+    ctx.record(e0, s)
+    for _ in range(10):
+        ctx.copy(dst, src, nb, s)
+    ctx.record(e1, s)
+    ctx.sync(s)
+    cms = ctx.elapsed(e0, e1) / 10
+    extra["copy_1GiB_GBs"] = 2.0 * nb / (cms * 1e-3) / 1e9
+    src.free()
+    dst.free()
+    log(f"  copy kernel {extra['copy_1GiB_GBs']:.0f} GB/s (read + write)")
+    if not a.only_main:
+        if "merbit_unreordered_ms" in extra:
+            extra["measured_speedup"] = extra["merbit_unreordered_ms"] / ms
+        # the workgroup-tile kernel and the CSR kernels on the same matrix
+        pol = mat.merbit_policy()
+        mat.merbit_options(1)
+        extra["merbit_workgroup_tiles_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, 30, 5)
+        mat.merbit_options(pol)
+        extra["csr_vector_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_VECTOR, 20, 3)
+        extra["csr_scalar_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_SCALAR, 3, 1)
+        # lower-precision values of the same reordered matrix vs the FP64 product
+        m64 = mat.convert(mr.F64)
+        x64 = ctx.upload(np.ascontiguousarray(xp.to_host().astype(np.float64)))
+        y64 = ctx.empty(n, np.float64)
+        m64.spmv(x64, y64, mr.MERBIT)
+        ctx.sync()
+        yref = y64.to_host()
+        m64.close()
+        prec = {}
+        for name, dt in (("f32", mr.F32), ("bf16", mr.BF16), ("f16", mr.F16)):
+            m2 = mat if dt == mr.F32 else mat.convert(dt)
+            if dt != mr.F32:
+                m2.merbit_autotune(5)
+            t2, _ = time_spmv(ctx, mr, m2, xp, yd, mr.MERBIT, 50, 5)
+            e = err_stats(yd.to_host(), yref)
+            prec[name] = {"ms": t2, "gflops": 2.0 * nnz / (t2 * 1e-3) / 1e9, **e}
+            if dt != mr.F32:
+                m2.close()
+        extra["s22_values_vs_fp64"] = prec
+        extra["bf16_speedup_vs_fp32"] = prec["f32"]["ms"] / prec["bf16"]["ms"]
+        log(f"  BF16 {prec['bf16']['ms'] * 1e3:.1f} us ({extra['bf16_speedup_vs_fp32']:.2f}x FP32), "
+            f"rel-L2 {prec['bf16']['rel_l2']:.2e}")
+    mat.close()
+    return res, (rp, ci, val, x)
+
+
+def section_precision(mr, ctx, a, extra):
+    """Stencil 160^3 and R-MAT s21 ef32, values log-uniform in [1e-3, 1] with
+    random sign, stored BF16 / FP16 / FP32 (FP32 accumulation) vs FP64."""
+    out = {}
+    cases = [("stencil27_160", lambda: mr.gen_stencil27(160, 160, 160)),
+             ("rmat_s21_ef32", lambda: mr.gen_rmat(21, 32, (0.57, 0.19, 0.19), seed=21))]
+    for name, make in cases:
+        t = time.perf_counter()
+        rp, ci = make()
+        n, nnz = len(rp) - 1, len(ci)
+        val = mr.gen_values(303, nnz, kind=1, lo=1e-3, hi=1.0)
+        x = mr.gen_values(304, n)
+        log(f"precision sweep {name}: n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+        m64 = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F64)
+        x64 = ctx.upload(x)
+        y64 = ctx.empty(n, np.float64)
+        m64.spmv(x64, y64, mr.MERBIT)
+        ctx.sync()
+        yref = y64.to_host()
+        t64, _ = time_spmv(ctx, mr, m64, x64, y64, mr.MERBIT, 10, 2)
+        res = {"n": n, "nnz": nnz, "f64_ms": t64}
+        del x64, y64
+        xd = ctx.upload(x.astype(np.float32))
+        yd = ctx.empty(n, np.float32)
+        for dname, dt in (("f32", mr.F32), ("f16", mr.F16), ("bf16", mr.BF16)):
+            m = m64.convert(dt)
+            m.merbit_autotune(5)
+            ms, _ = time_spmv(ctx, mr, m, xd, yd, mr.MERBIT, 30, 3)
+            e = err_stats(yd.to_host(), yref)
+            res[dname] = {"ms": ms, "gflops": 2.0 * nnz / (ms * 1e-3) / 1e9,
+                          "gbs_canonical": csr_bytes(n, n, nnz, 2 if dt != mr.F32 else 4) / (ms * 1e-3) / 1e9,
+                          **e}
+            log(f"  {dname}: {ms * 1e3:.1f} us, max {e['max_abs']:.2e}, rel-L2 {e['rel_l2']:.2e}")
+            m.close()
+        m64.close()
+        out[name] = res
+    extra["precision_sweep"] = out
+
+
+def batched_set(mr, seed=3, count=64):
+    """64 matrices: rows log-uniform in [2^10, 2^18], nnz/row log-uniform in
+    [4, 64]; even entries R-MAT (scale = round(log2 rows), edge factor = the
+    drawn nnz/row), odd entries random banded (bandwidth uniform in [8, 512])."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        rows = int(round(2.0 ** rng.uniform(10, 18)))
+        d = float(2.0 ** rng.uniform(2, 6))
+        if i % 2 == 0:
+            scale = int(round(math.log2(rows)))
+            rp, ci = mr.gen_rmat(scale, max(1, int(round(d))), (0.57, 0.19, 0.19), seed=1000 + i)
+        else:
+            bw = int(rng.integers(8, 513))
+            rp, ci = mr.gen_banded(rows, bw, max(1, min(int(round(d)), 2 * bw + 1)), seed=2000 + i)
+        out.append((rp, ci))
+    return out
+
+
+def section_batched(mr, ctx, a, extra):
+    t = time.perf_counter()
+    shapes = batched_set(mr)
+    mats = []
+    t_re = 0.0
+    for i, (rp, ci) in enumerate(shapes):
+        val = mr.gen_values(3000 + i, len(ci)).astype(np.float32)
+        m = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
+        t0 = time.perf_counter()
+        m.reorder_pgorder(0, 5)
+        t_re += time.perf_counter() - t0
+        mats.append(m)
+    tot_nnz = sum(len(c) for _, c in shapes)
+    tot_rows = sum(len(r) - 1 for r, _ in shapes)
+    log(f"batched: 64 matrices, {tot_rows} rows, {tot_nnz} nnz; pGOrder of all {t_re:.1f} s "
+        f"(setup {time.perf_counter() - t:.1f} s)")
+    res = {"matrices": 64, "rows": tot_rows, "nnz": tot_nnz, "pgorder_s": t_re}
+    st = ctx.new_stream()
+    for k in (1, 8, 32):
+        xs, ys = [], []
+        for m in mats:
+            n = m.info()[0]
+            xs.append(ctx.upload(mr.serve_rhs(k, n, k)))
+            ys.append(ctx.empty((n, k), np.float32))
+        b = mr.Batch(ctx, mats, k)
+        b.set_io(xs, ys, stream=st)
+        ctx.sync(st)
+        tune = b.autotune(5)
+        b.set_io(xs, ys, stream=st)
+        b.capture(st)
+        e0, e1 = ctx.event(), ctx.event()
+        reps = 50
+        b.launch(st)
+        ctx.sync(st)
+        ctx.record(e0, st)
+        for _ in range(reps):
+            b.launch(st)
+        ctx.record(e1, st)
+        ctx.sync(st)
+        g_ms = ctx.elapsed(e0, e1) / reps
+        ctx.record(e0, st)
+        for _ in range(reps):
+            b.run(st)
+        ctx.record(e1, st)
+        ctx.sync(st)
+        e_ms = ctx.elapsed(e0, e1) / reps
+        # the same work as one launch per matrix (reference style)
+        ctx.record(e0, st)
+        for _ in range(5):
+            for j, m in enumerate(mats):
+                if k == 1:
+                    m.spmv(xs[j], ys[j], mr.MERBIT, st)
+                else:
+                    m.spmm(k, xs[j], ys[j], mr.MERBIT, stream=st)
+        ctx.record(e1, st)
+        ctx.sync(st)
+        s_ms = ctx.elapsed(e0, e1) / 5
+        info = b.info()
+        res[f"k{k}"] = {"graph_ms": g_ms, "eager_ms": e_ms, "per_matrix_launches_ms": s_ms,
+                        "gflops": info["flops"] / (g_ms * 1e-3) / 1e9,
+                        "gbs_canonical": info["bytes"] / (g_ms * 1e-3) / 1e9,
+                        "tiles": info["tiles"], "autotune": tune}
+        log(f"  k={k}: graph {g_ms * 1e3:.1f} us, eager {e_ms * 1e3:.1f} us, per-matrix {s_ms * 1e3:.1f} us, "
+            f"{res[f'k{k}']['gflops']:.0f} GFLOP/s, tune {tune}")
+        b.close()
+        del xs, ys
+    extra["batched"] = res
+    return mats
+
+
+def section_serving(mr, ctx, a, extra, batch_mats):
+    """32 resident BF16 matrices: R-MAT s22 and s21 ef32 reordered, plus 30 of
+    the batched set; Poisson loads with uploads of R-MAT s18 every 1000th."""
+    t = time.perf_counter()
+    resident = []
+    for scale, ef, seed in ((22, 16, 2), (21, 32, 21)):
+        rp, ci = mr.gen_rmat(scale, ef, (0.57, 0.19, 0.19), seed=seed)
+        val = mr.gen_values(500 + scale, len(ci))
+        m = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.BF16)
+        m.reorder_pgorder(0, 5)
+        resident.append(m)
+    for m in batch_mats[:30]:
+        resident.append(m.convert(mr.BF16))
+    srv = mr.Server(ctx, slots=4, max_jobs=256, coalesce=1)
+    for m in resident:
+        srv.add(m)
+    log(f"serving: 32 resident BF16 matrices ({time.perf_counter() - t:.1f} s setup)")
+    res = {}
+    for rate in a.serve_rates:
+        nreq = int(min(max(rate * a.serve_seconds, 2000), 60000))
+        r = srv.run_load(rate=rate, n_requests=nreq, seed=int(rate), ks=(1, 4, 16), upload_every=1000,
+                         upload_scale=18, upload_edge_factor=16)
+        res[f"{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+        log(f"  {rate:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, {r['gflops']:.0f} GFLOP/s, "
+            f"{r['completed']} done, {r['uploads_admitted']} uploads ({r['upload_ms_mean']:.0f} ms each), "
+            f"{r['mean_requests_per_batch']:.1f} req/batch")
+    srv.close()
+    for m in resident:
+        m.close()
+    extra["serving"] = res
+
+
+def section_cpu(mr, ctx, a, data):
+    """The oracle's threaded CSR SpMV (FP32) on the headline matrix, ~10 s."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    _, ms = oracle.spmv_f32_threads(rp, ci, val32, x32, cores, 2)
+    rp, ci, val, x = data
+    cores = max(1, min(host_cores(), 16))
+    _, ms = oracle.spmv_f32_threads(rp, ci, val, x, cores, 2)
     per = ms / 2
-    iters = max(3, min(400, int(target_s * 1e3 / max(per, 1e-3))))
-    _, ms = oracle.spmv_f32_threads(rp, ci, val32, x32, cores, iters)
+    iters = max(3, min(400, int(10.0e3 / max(per, 1e-3))))
+    _, ms = oracle.spmv_f32_threads(rp, ci, val, x, cores, iters)
     per = ms / iters
     gflops = 2.0 * len(ci) / (per * 1e-3) / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"oracle threaded CSR SpMV (FP32, nnz-balanced rows) on the same R-MAT s22 matrix, "
-                      f"{iters} iterations, {per:.2f} ms/iter"}
+            "sample": f"oracle threaded CSR SpMV (FP32, nnz-balanced row ranges) on the unreordered R-MAT s22 "
+                      f"matrix, {iters} iterations, {per:.2f} ms/iter"}
+
+
+def section_cpu_reorders(mr, ctx, extra):
+    """Host reorderers (sequential Gorder, RCM) and the pGOrder host phase on
+    R-MAT s16, beside the GPU pGOrder of the same matrix."""
+    rp, ci = mr.gen_rmat(16, 16, (0.57, 0.19, 0.19), seed=16)
+    out = {"workload": "R-MAT s16 ef16 seed 16", "n": len(rp) - 1, "nnz": len(ci)}
+    for name, fn in (("gorder_cpu_w5", lambda m: m.reorder_gorder_cpu(5)), ("rcm", lambda m: m.reorder_rcm()),
+                     ("pgorder", lambda m: m.reorder_pgorder(0, 5))):
+        m = mr.Mat.from_csr(ctx, rp, ci, None, store=mr.F32)
+        t = time.perf_counter()
+        _, st = fn(m)
+        out[name] = {"wall_ms": (time.perf_counter() - t) * 1e3, **{k: v for k, v in st.items() if k.endswith("_ms")}}
+        m.close()
+    log(f"  host reorderers on s16: " + ", ".join(f"{k} {v['wall_ms']:.0f} ms" for k, v in out.items()
+                                                   if isinstance(v, dict)))
+    extra["cpu_reorders"] = out
 
 
 def main():
@@ -80,94 +401,49 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--only-main", action="store_true", help="headline SpMV only (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--perm-cache", default="", help="load (or save) the s22 pGOrder permutation here")
+    ap.add_argument("--sections", default="precision,batched,serving,cpu_reorders")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[5000.0, 20000.0, 50000.0])
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (this build has no multi-GPU path)",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (this build has no multi-GPU path; "
+              f"the sharding design is documented instead)", file=sys.stderr)
         sys.exit(2)
 
     import mrpy as mr
 
-    scale = a.scale
-    log(f"generating R-MAT scale {scale}, edge factor 16, seed 2")
-    t = time.perf_counter()
-    rp, ci = mr.gen_rmat(scale, 16, (0.57, 0.19, 0.19), seed=2)
-    n, nnz = len(rp) - 1, len(ci)
-    val = mr.gen_values(102, nnz).astype(np.float32)
-    x = mr.gen_values(103, n).astype(np.float32)
-    log(f"n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
-
     ctx = mr.Ctx(0)
     arch, cus = ctx.arch()
     log(f"device {arch}, {cus} CUs")
-    mat = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
-    xd = ctx.upload(x)
-    yd = ctx.empty(n, np.float32)
     extra = {}
-
-    if not a.no_extra:
-        log("unreordered MERBIT")
-        ms_orig, _ = time_spmv(ctx, mr, mat, xd, yd, mr.MERBIT, max(20, a.steps // 4), 5)
-        extra["merbit_unreordered_ms"] = ms_orig
-        log(f"  {ms_orig * 1e3:.1f} us/SpMV")
-        log("RD headroom predictor on the unreordered matrix")
-        t = time.perf_counter()
-        pred, feat, rdst = mat.locality_predict(mr.MODEL_MERBIT)
-        extra["rd_predicted_speedup"] = pred
-        extra["rd_features"] = [round(float(v), 6) for v in feat]
-        extra["rd_ms"] = (time.perf_counter() - t) * 1e3
-        log(f"  predicted speedup {pred:.3f} ({extra['rd_ms']:.0f} ms)")
-
-    log("pGOrder (width auto, window 5)")
-    t = time.perf_counter()
-    perm, st = mat.reorder_pgorder(0, 5)
-    extra["pgorder"] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}
-    extra["pgorder_wall_ms"] = (time.perf_counter() - t) * 1e3
-    log(f"  {json.dumps(extra['pgorder'])}")
-    # relabel x into the new order once (outside the timed region)
-    xp = ctx.empty(n, np.float32)
-    mat.permute_vec(0, xd, xp)
-    mi = mat.merbit_info()
-    extra["merbit_sigma"] = mi["sigma"]
-    extra["merbit_build_ms"] = mi["build_ms"]
-
-    log(f"MERBIT FP32 after pGOrder: {a.warmup} warmup + {a.steps} timed steps")
-    ms, wall = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, a.steps, a.warmup)
-    gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
-    bytes_alg = algorithmic_bytes(n, nnz, 4)
-    achieved = bytes_alg / (ms * 1e-3) / 1e9
-    log(f"  {ms * 1e3:.1f} us/SpMV (wall {wall * 1e3:.1f} us), {gflops:.1f} GFLOP/s, {achieved:.0f} GB/s")
-
-    if not a.no_extra:
-        if "merbit_unreordered_ms" in extra:
-            extra["measured_speedup"] = extra["merbit_unreordered_ms"] / ms
-        for sg in (8, 32):
-            mat.merbit_configure(sg)
-            t_s, _ = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, max(20, a.steps // 4), 5)
-            extra[f"merbit_sigma{sg}_ms"] = t_s
-        mat.merbit_configure(mi["sigma"])
-        t_csr, _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_VECTOR, max(20, a.steps // 4), 5)
-        extra["csr_vector_ms"] = t_csr
-        log(f"  CSR vector {t_csr * 1e3:.1f} us")
-        mb = mat.convert(mr.BF16)
-        t_bf, _ = time_spmv(ctx, mr, mb, xp, yd, mr.MERBIT, max(20, a.steps // 4), 5)
-        extra["merbit_bf16_ms"] = t_bf
-        extra["bf16_speedup_vs_fp32"] = ms / t_bf
-        log(f"  MERBIT BF16 values {t_bf * 1e3:.1f} us ({ms / t_bf:.2f}x FP32)")
-        mb.close()
-
+    main_res, data = section_main(mr, ctx, a, extra)
+    sections = set() if a.only_main else set(s for s in a.sections.split(",") if s)
+    if "precision" in sections:
+        section_precision(mr, ctx, a, extra)
+    batch_mats = []
+    if "batched" in sections or "serving" in sections:
+        batch_mats = section_batched(mr, ctx, a, extra)
+    if "serving" in sections:
+        section_serving(mr, ctx, a, extra, batch_mats)
+    for m in batch_mats:
+        m.close()
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and not a.only_main:
         log("CPU baseline (oracle threaded CSR SpMV, FP32)")
-        cpu = cpu_baseline(rp, ci, val, x)
+        cpu = section_cpu(mr, ctx, a, data)
         log(f"  {cpu['value']} GFLOP/s on {cpu['cores']} cores")
+        if "cpu_reorders" in sections:
+            section_cpu_reorders(mr, ctx, extra)
 
+    ms, n, nnz = main_res["ms"], main_res["n"], main_res["nnz"]
+    cfg = main_res["cfg"]
     out = {
         "metric": "SpMV GFLOP/s + % HBM roofline after pGOrder; batched SpMV p50/p99 latency",
-        "value": round(gflops, 3),
+        "value": round(main_res["gflops"], 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -178,15 +454,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"R-MAT scale {scale} ef16 (0.57,0.19,0.19) seed 2, FP32, pGOrder w=5 + MERBIT SpMV",
-                   "n_rows": n, "nnz": nnz, "kernel": "merbit", "sigma": mi["sigma"]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "config": {"workload": "R-MAT scale 22 ef16 (0.57,0.19,0.19) seed 2, FP32 values, pGOrder w=5 + "
+                               "autotuned MERBIT SpMV",
+                   "n_rows": n, "nnz": nnz, "kernel": "merbit", **cfg},
+        "roofline": {"bound": "hbm", "achieved": round(main_res["achieved"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(main_res["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile()},
         "cpu_baseline": cpu,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
-    mat.close()
     ctx.close()
 
 

@@ -238,18 +238,22 @@ int mr_merbit_configure(mr_mat m, int sigma);
 /* load policy of the MERBIT kernel: bit 0 = non-temporal col/val loads and
  * y stores, bit 1 = XCD-contiguous block order, bit 2 = the wave-tile variant
  * with an LDS cache of the hottest columns of x (F32/F16/BF16 only; its tiles
- * are set by mr_merbit_xc_configure).  Default 1.  Bits 0 and 1 do not change
+ * are set by mr_merbit_xc_configure).  Default 4 for F32/F16/BF16 matrices
+ * with at least 4,194,304 nonzeros, 1 otherwise.  Bits 0 and 1 do not change
  * results; bit 2 changes the tile partition and hence the rounding order
  * (results stay deterministic) */
 int mr_merbit_options(mr_mat m, int policy);
+int mr_merbit_get_options(mr_mat m, int* policy);
 /* autotuner: times every (sigma, policy) on this matrix (reps launches each,
  * hipEvents) and keeps the fastest; no reference counterpart (the reference
  * fixes BLOCK=256, SIGMA=14 at class.cu:1409-1410) */
 int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, float* best_us);
-/* wave-tile / x-cache variant (no reference counterpart): tiles of 64 lanes x
+/* lane-padded wave-tile / x-cache variant (no reference counterpart): the
+ * merge path stored as padded (column, value) streams, tiles of 64 lanes x
  * items_per_lane items, `waves` waves per persistent workgroup (one per CU),
  * cached_cols hottest columns of x held in LDS (-1 = as many as fit); pairs
- * (8,16), (16,16), (16,8), (32,8).  Rebuilds the variant's metadata. */
+ * (8,16), (16,16), (8,8), (16,8) and, for F32, (4,16).  Rebuilds the
+ * variant's metadata (about (n_rows + nnz) * (4 + |val|) bytes). */
 int mr_merbit_xc_configure(mr_mat m, int items_per_lane, int waves, int cached_cols);
 int mr_merbit_xc_info(mr_mat m, int* items_per_lane, int* waves, int* cached_cols, int64_t* tiles);
 int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms);

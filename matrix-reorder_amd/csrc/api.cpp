@@ -438,6 +438,9 @@ int mr_csr_create(mr_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz, const
     for (int64_t i = 0; i < n_rows; ++i) m->perm[(size_t)i] = (int32_t)i;
     MR_HIP_TRY(hipStreamSynchronize(s));
     MR_TRY(merbit_build(m.get(), s));
+    // large FP32-accumulate matrices default to the lane-padded x-cache
+    // variant; small ones (cache fill would dominate) to workgroup tiles
+    m->merbit.policy = (store_dtype != MR_F64 && nnz >= (int64_t(1) << 22)) ? 4 : 1;
     *out = m.release();
     return MR_OK;
 }
@@ -493,8 +496,11 @@ int mr_mat_convert(mr_mat m, int dtype, mr_mat* out)
     c->dtype = dtype;
     c->avg_row = m->avg_row;
     c->perm = m->perm;
-    c->merbit.sigma = m->merbit.sigma;
-    c->merbit.policy = m->merbit.policy;
+    c->merbit.sigma = dtype == MR_F64 ? std::min(m->merbit.sigma, 16) : m->merbit.sigma;
+    c->merbit.policy = dtype == MR_F64 ? (m->merbit.policy & 3) : m->merbit.policy;
+    c->merbit.xs = (dtype != MR_F32 && m->merbit.xs == 4) ? 8 : m->merbit.xs;
+    c->merbit.xw = m->merbit.xw;
+    c->merbit.xc_req = m->merbit.xc_req;
     MR_TRY(c->rp.alloc(m->rp.bytes));
     MR_TRY(c->ci.alloc(m->ci.bytes));
     MR_TRY(c->val.alloc(dtype_size(dtype) * (size_t)(m->nnz + kPad)));
@@ -684,6 +690,13 @@ int mr_merbit_options(mr_mat m, int policy)
     if ((policy & 4) && m->dtype == MR_F64)
         return set_error(MR_ERR_UNSUPPORTED, "the x-cache MERBIT variant needs F32/F16/BF16 values");
     m->merbit.policy = policy;
+    return MR_OK;
+}
+
+int mr_merbit_get_options(mr_mat m, int* policy)
+{
+    if (!m || !policy) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    *policy = m->merbit.policy;
     return MR_OK;
 }
 

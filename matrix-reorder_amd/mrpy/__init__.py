@@ -144,6 +144,7 @@ SIGNATURES = {
     "mr_spmm": [_P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int64, _P],
     "mr_merbit_configure": [_P, C.c_int],
     "mr_merbit_options": [_P, C.c_int],
+    "mr_merbit_get_options": [_P, C.POINTER(C.c_int)],
     "mr_merbit_autotune": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _FP],
     "mr_merbit_info": [_P, C.POINTER(C.c_int), _I64P, _I64P, _DP],
     "mr_merbit_export": [_P, _U32P, _I32P, _I32P],
@@ -571,6 +572,11 @@ class Mat:
 
     def merbit_options(self, policy: int):
         check(lib().mr_merbit_options(self.h, policy))
+
+    def merbit_policy(self) -> int:
+        p = C.c_int(0)
+        check(lib().mr_merbit_get_options(self.h, C.byref(p)))
+        return p.value
 
     def merbit_autotune(self, reps: int = 5):
         sg, pol, us = C.c_int(), C.c_int(), C.c_float()
