@@ -140,6 +140,8 @@ def section_main(mr, ctx, a, extra):
         best = mat.merbit_autotune(10)
         extra["autotune"] = best
         log(f"  {best}")
+    if a.policy >= 0:
+        mat.merbit_options(a.policy)
     cfg = {"policy": mat.merbit_policy(), **mat.merbit_info(), **mat.merbit_xc_info()}
     log(f"MERBIT FP32 after pGOrder: {a.warmup} warmup + {a.steps} timed steps")
     ms, wall = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, a.steps, a.warmup)
@@ -405,6 +407,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--perm-cache", default="", help="load (or save) the s22 pGOrder permutation here")
+    ap.add_argument("--policy", type=int, default=-1, help="force a MERBIT load policy (after the autotuner)")
     ap.add_argument("--sections", default="precision,batched,serving,cpu_reorders")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5000.0, 20000.0, 50000.0])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
