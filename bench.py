@@ -21,7 +21,9 @@ Further sections (reported under "extra", each on its own synthetic inputs):
              {1, 4, 16}, every 1000th request uploads an R-MAT s18 that is
              reordered with pGOrder in the background; p50 / p99 latency;
   cpu        the CPU baseline: the oracle's threaded CSR SpMV (FP32) on the
-             same matrix, plus host reorderers on a smaller R-MAT.
+             same matrix, plus the host reorderers (sequential Gorder, RCM,
+             Rabbit Order) beside pGOrder on R-MAT s18, each with the SpMV
+             time it yields.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -382,19 +384,31 @@ def section_cpu(mr, ctx, a, data):
 
 
 def section_cpu_reorders(mr, ctx, extra):
-    """Host reorderers (sequential Gorder, RCM) and the pGOrder host phase on
-    R-MAT s16, beside the GPU pGOrder of the same matrix."""
-    rp, ci = mr.gen_rmat(16, 16, (0.57, 0.19, 0.19), seed=16)
-    out = {"workload": "R-MAT s16 ef16 seed 16", "n": len(rp) - 1, "nnz": len(ci)}
-    for name, fn in (("gorder_cpu_w5", lambda m: m.reorder_gorder_cpu(5)), ("rcm", lambda m: m.reorder_rcm()),
+    """Host reorderers (sequential Gorder, RCM, Rabbit Order) beside the GPU
+    pGOrder on R-MAT s18, each followed by the MERBIT SpMV (FP32) it yields."""
+    rp, ci = mr.gen_rmat(18, 16, (0.57, 0.19, 0.19), seed=18)
+    n, nnz = len(rp) - 1, len(ci)
+    val = mr.gen_values(18, nnz).astype(np.float32)
+    x = mr.gen_values(19, n).astype(np.float32)
+    out = {"workload": "R-MAT s18 ef16 seed 18, FP32", "n": n, "nnz": nnz, "cores": host_cores()}
+    xd = ctx.upload(x)
+    yd = ctx.empty(n, np.float32)
+    for name, fn in (("none", None), ("gorder_cpu_w5", lambda m: m.reorder_gorder_cpu(5)),
+                     ("rcm", lambda m: m.reorder_rcm()), ("rabbit", lambda m: m.reorder_rabbit(False)),
                      ("pgorder", lambda m: m.reorder_pgorder(0, 5))):
-        m = mr.Mat.from_csr(ctx, rp, ci, None, store=mr.F32)
-        t = time.perf_counter()
-        _, st = fn(m)
-        out[name] = {"wall_ms": (time.perf_counter() - t) * 1e3, **{k: v for k, v in st.items() if k.endswith("_ms")}}
+        m = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
+        res = {}
+        if fn is not None:
+            t = time.perf_counter()
+            _, st = fn(m)
+            res = {"wall_ms": (time.perf_counter() - t) * 1e3,
+                   **{k: v for k, v in st.items() if k.endswith("_ms")}}
+        m.merbit_autotune(5)
+        res["spmv_us"] = time_spmv(ctx, mr, m, xd, yd, mr.MERBIT, 50, 5)[0] * 1e3
+        out[name] = res
         m.close()
-    log(f"  host reorderers on s16: " + ", ".join(f"{k} {v['wall_ms']:.0f} ms" for k, v in out.items()
-                                                   if isinstance(v, dict)))
+    log("  reorderers on s18: " + ", ".join(
+        f"{k} {v.get('wall_ms', 0):.0f} ms -> {v['spmv_us']:.1f} us" for k, v in out.items() if isinstance(v, dict)))
     extra["cpu_reorders"] = out
 
 

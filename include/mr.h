@@ -174,6 +174,12 @@ int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_st
 /* Reverse Cuthill-McKee = GRAPH::relabel_rcm (class.cu:1554-1602) over
  * R_Cuthill_Mckee (rcm/src/rcm.cpp:138-154); relabels */
 int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats);
+/* Rabbit Order = GRAPH::rabbitorder (class.cu:1607-1655) over rabbit_order
+ * (rabbitorder/src/rabbit_order.cpp:1-451), run in one deterministic order
+ * (the reference's OpenMP / CAS aggregation is timing dependent); relabels.
+ * reference_scatter != 0 applies the ids the way the reference's call site
+ * does (scattered as a position -> vertex sequence, i.e. the inverse map). */
+int mr_reorder_rabbit(mr_mat m, int reference_scatter, int32_t* perm_out, mr_reorder_stats* stats);
 
 /* ---- reuse distance and the locality predictor ---------------------------- */
 #define MR_RD_EXACT   0

@@ -509,4 +509,38 @@ int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats)
     return MR_OK;
 }
 
+int mr_reorder_rabbit(mr_mat m, int reference_scatter, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "Rabbit Order needs a square matrix");
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (n64 == 0) return MR_OK;
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    const auto t0 = clk::now();
+    const int32_t n = (int32_t)n64;
+    std::vector<int32_t> rp((size_t)n + 1), ci((size_t)std::max<int64_t>(nnz, 1));
+    MR_TRY(mr_mat_export(m, rp.data(), ci.data(), nullptr));
+    std::vector<uint32_t> id((size_t)n);
+    rabbit_order(n, rp.data(), ci.data(), id.data());
+    std::vector<int32_t> perm((size_t)n);
+    if (reference_scatter) {
+        // class.cu:1622-1627 scatters the returned ids as a position -> vertex
+        // sequence, i.e. applies the inverse of Rabbit's vertex -> id map
+        for (int32_t i = 0; i < n; ++i) perm[(size_t)id[(size_t)i]] = i;
+    } else {
+        for (int32_t i = 0; i < n; ++i) perm[(size_t)i] = (int32_t)id[(size_t)i];
+    }
+    const double order_ms = ms_since(t0);
+    double rl = 0;
+    MR_TRY(relabel(m, perm, &rl));
+    if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->cpu_ms = order_ms;
+        stats->relabel_ms = rl;
+        stats->total_ms = ms_since(t0);
+    }
+    return MR_OK;
+}
+
 }  // extern "C"

@@ -19,5 +19,7 @@ void gorder_cpu(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* 
                 int32_t window, uint32_t* order);
 // Reverse Cuthill-McKee over a COO edge list (both directions are neighbours)
 void rcm_order(int32_t n, int64_t nz, const int32_t* rows, const int32_t* cols, uint32_t* order);
+// Rabbit Order on the symmetrised pattern of M (rp, ci): new_id[v] = position of v
+void rabbit_order(int32_t n, const int32_t* rp, const int32_t* ci, uint32_t* new_id);
 
 }  // namespace mr

@@ -132,6 +132,7 @@ SIGNATURES = {
     "mr_pgorder_order": [_P, C.c_int, C.c_int, _U32P, _U32P, C.POINTER(ReorderStats)],
     "mr_reorder_gorder_cpu": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rcm": [_P, _I32P, C.POINTER(ReorderStats)],
+    "mr_reorder_rabbit": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_rd_params_default": [C.POINTER(RDParams)],
     "mr_rd_exact_stream": [_I32P, C.c_int64, C.c_int64, C.POINTER(RDParams), _I64P, _DP, C.POINTER(RDStats)],
     "mr_rd_features": [_P, C.c_int, C.POINTER(RDParams), _DP, C.POINTER(RDStats)],
@@ -525,6 +526,13 @@ class Mat:
         perm = np.empty(max(n, 1), np.int32)
         st = ReorderStats()
         check(lib().mr_reorder_rcm(self.h, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
+    def reorder_rabbit(self, reference_scatter: bool = False):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(lib().mr_reorder_rabbit(self.h, int(reference_scatter), ptr(perm, C.c_int32), C.byref(st)))
         return perm[:n], st.as_dict()
 
     # reuse distance

@@ -137,6 +137,10 @@ void oracle_merge_search(const int32_t* rp, int32_t n, int64_t nnz, int64_t diag
 void oracle_merbit_spmv(int32_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
                         const double* val, const double* x, int32_t sigma, double* y);
 
+/* ---- Rabbit Order, one deterministic order (rabbitorder/src/rabbit_order.cpp) -- */
+/* new_id[v] = position of v; returns the number of top-level communities */
+int oracle_rabbit(int32_t n, const int32_t* rp, const int32_t* ci, uint32_t* new_id);
+
 #ifdef __cplusplus
 }
 #endif

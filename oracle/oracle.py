@@ -38,6 +38,7 @@ _SIG = {
     "oracle_gorder1_score": (C.c_int64, [C.c_int32, _I32P, _I32P, _I32P, _I32P, _U32P, C.c_int32, C.c_int32,
                                          C.c_int32]),
     "oracle_rcm": (C.c_int, [C.c_int32, C.c_int64, _I32P, _I32P, _U32P]),
+    "oracle_rabbit": (C.c_int, [C.c_int32, _I32P, _I32P, _U32P]),
     "oracle_rd_exact": (C.c_int, [_I32P, C.c_int32, C.c_int64, C.c_int32, _I32P, C.c_int32, _I64P, _I64P, _DP]),
     "oracle_rd_bruteforce": (C.c_int, [_I32P, C.c_int64, C.c_int32, _I64P]),
     "oracle_hll_hash": (C.c_uint32, [C.c_uint32]),
@@ -182,6 +183,14 @@ def rcm(n, rows, cols):
     out = np.empty(n, np.uint32)
     lib().oracle_rcm(n, len(rows), _p(rows, C.c_int32), _p(cols, C.c_int32), _p(out, C.c_uint32))
     return out
+
+
+def rabbit(n, rp, ci):
+    """Rabbit Order in one deterministic order: (new_id per vertex, communities)."""
+    rp, ci = _i32(rp), _i32(ci)
+    out = np.empty(max(n, 1), np.uint32)
+    ncom = lib().oracle_rabbit(n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(out, C.c_uint32))
+    return out[:n], ncom
 
 
 def rd_exact(stream, n, sector=32, edges=DEFAULT_EDGES):

@@ -140,3 +140,37 @@ def test_graph_workflow_unpermutes(ctx, mr, oracle):
         y = g.spmv(x, method)
         worst, l2, _ = oracle.spmv_parity(y, yhat, absrow)
         assert worst <= 1e-12 and l2 <= 1e-12, (method, worst)
+
+
+def rabbit_cases():
+    rng = np.random.default_rng(71)
+    out = [("rmat_s12", *rmat_csr(12, 8, seed=12))]
+    rp, ci = random_csr(rng, 900, empty_frac=0.2, max_row=10)
+    out.append(("random_900", rp, ci))
+    # the reference's own demo graph (rabbitorder/src/main.cpp:9-11): COO
+    # (row_ind, col_ind) with 12 vertices; as M's CSR, row = col_ind
+    rows = [[] for _ in range(12)]
+    for r, c in zip([1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11], [2, 2, 4, 5, 5, 2, 8, 9, 10, 8, 1]):
+        rows[c].append(r)
+    out.append(("demo_12", *csr_rows([sorted(r) for r in rows], 12)))
+    return out
+
+
+@pytest.mark.parametrize("case", rabbit_cases(), ids=lambda c: c[0])
+def test_rabbit_matches_oracle(ctx, mr, oracle, case):
+    name, rp, ci = case
+    n = len(rp) - 1
+    ids, _ = oracle.rabbit(n, rp, ci)
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    perm, st = mat.reorder_rabbit(False)
+    assert np.array_equal(perm, ids.astype(np.int32))
+    grp, gci, _ = mat.export()
+    orp, oci, _ = oracle.permute(n, rp, ci, None, perm)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci)
+    mat.close()
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    perm2, _ = mat.reorder_rabbit(True)            # the reference call site's scatter
+    inv = np.empty(n, np.int32)
+    inv[ids] = np.arange(n, dtype=np.int32)
+    assert np.array_equal(perm2, inv)
+    mat.close()
