@@ -353,7 +353,7 @@ def section_serving(mr, ctx, a, extra, batch_mats):
     res = {}
     for rate in a.serve_rates:
         nreq = int(min(max(rate * a.serve_seconds, 2000), 60000))
-        r = srv.run_load(rate=rate, n_requests=nreq, seed=int(rate), ks=(1, 4, 16), upload_every=1000,
+        r = srv.run_load(rate=rate, n_requests=nreq, seed=int(rate), ks=(1, 4, 16), upload_every=a.serve_upload_every,
                          upload_scale=18, upload_edge_factor=16)
         res[f"{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
         log(f"  {rate:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, {r['gflops']:.0f} GFLOP/s, "
@@ -425,6 +425,7 @@ def main():
     ap.add_argument("--sections", default="precision,batched,serving,cpu_reorders")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5000.0, 20000.0, 50000.0])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-upload-every", type=int, default=1000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (this build has no multi-GPU path; "

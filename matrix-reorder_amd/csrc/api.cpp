@@ -178,8 +178,33 @@ int merbit_xc_build(mr_mat m, hipStream_t s)
                                 p.xty.as<int32_t>(), s));
     MR_HIP_TRY(hipStreamSynchronize(s));   // dmap and tx are freed on return
     p.xc = (int)xc;
+    p.max_lds = max_lds;
     p.xc_built = true;
+    p.pcm_built = false;
     return MR_OK;
+}
+
+int merbit_lp_mm_build(mr_mat m, hipStream_t s)
+{
+    MerbitPlan& p = m->merbit;
+    MR_TRY(merbit_xc_build(m, s));
+    if (p.pcm_built) return MR_OK;
+    const int64_t T = 64LL * p.xs;
+    const int64_t padded = std::max<int64_t>(p.xtiles * T, T);
+    MR_TRY(p.pcm.alloc(sizeof(int32_t) * (size_t)padded));
+    MR_TRY(p.xcarry_mm.alloc(sizeof(float) * 16 * (size_t)std::max<int64_t>(p.xtiles, 1)));
+    MR_HIP_TRY(k::merbit_lp_build(m->dtype, m->n_rows, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p, nullptr,
+                                  p.items, padded, p.pcm.as<int32_t>(), nullptr, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    p.pcm_built = true;
+    return MR_OK;
+}
+
+int merbit_lp_mm_rows(mr_mat m, int kk)
+{
+    const int K = k::merbit_lp_mm_chunk(kk);
+    const int64_t rows = (int64_t)m->merbit.max_lds / 4 / K;
+    return (int)std::max<int64_t>(0, std::min<int64_t>(rows, m->n_cols));
 }
 
 static int check_mat(mr_mat m)
@@ -658,6 +683,20 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
     }
     if (m->n_rows == 0) return MR_OK;
     MerbitPlan& p = m->merbit;
+    if ((p.policy & 4) && m->dtype != MR_F64 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
+        // lane-padded tiles: 16 columns per pass, X rows cached in LDS
+        MR_TRY(merbit_lp_mm_build(m, st));
+        const size_t vb = vec_size(m->dtype);
+        for (int j0 = 0; j0 < k; j0 += 16) {
+            const int kk = std::min(16, k - j0);
+            MR_HIP_TRY(k::merbit_lp_spmm(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles, p.pcm.as<int32_t>(),
+                                         p.pv.p, p.xty.as<int32_t>(), static_cast<const char*>(X) + vb * (size_t)j0,
+                                         ldx, kk, merbit_lp_mm_rows(m, kk), static_cast<char*>(Y) + vb * (size_t)j0,
+                                         ldy, p.xcarry_mm.p, p.xcarry_row.as<int32_t>(),
+                                         std::max(m->ctx->cu_count, 1), st));
+        }
+        return MR_OK;
+    }
     MR_TRY(merbit_mm_grid(m, st));
     // column chunks of at most 64 right-hand sides
     const size_t vb = vec_size(m->dtype);

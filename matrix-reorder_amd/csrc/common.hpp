@@ -123,6 +123,10 @@ struct MerbitPlan {
     int64_t xtiles = 0;
     mr::DevBuf pc, pv, hot, xty, xcarry, xcarry_row;
     bool xc_built = false;
+    // SpMM on the same tiles: plain-column stream and 16-wide carries
+    mr::DevBuf pcm, xcarry_mm;
+    bool pcm_built = false;
+    int max_lds = 0;
     bool built = false;
     double build_ms = 0.0;
 };
@@ -151,4 +155,8 @@ int merbit_coords_sigma(mr_mat m, int sigma, hipStream_t s, const int32_t** bx, 
 int merbit_mm_grid(mr_mat m, hipStream_t s);
 // metadata of the wave-tile / x-cache variant, built on first use
 int merbit_xc_build(mr_mat m, hipStream_t s);
+// plain-column stream for SpMM on the lane-padded tiles (builds the tiles too)
+int merbit_lp_mm_build(mr_mat m, hipStream_t s);
+// rows of X a LP SpMM launch of kk columns caches in LDS
+int merbit_lp_mm_rows(mr_mat m, int kk);
 }  // namespace mr

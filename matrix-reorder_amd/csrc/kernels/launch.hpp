@@ -80,8 +80,18 @@ hipError_t merbit_spmv(int dt, int sigma, int policy, int64_t n_rows, int64_t n_
 // after the last item; a cached column is slot | 1 << 31).
 // (S, W) in {(8,16), (16,16), (8,8), (16,8), (4,16)} (S = 4 for F32 only)
 size_t merbit_lp_lds_bytes(int xc);
+// map == NULL: plain columns; pv == NULL: the value stream is left alone
 hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int32_t* ci, const void* val,
                            const int32_t* map, int64_t items, int64_t padded, int32_t* pc, void* pv, hipStream_t s);
+// SpMM on the lane-padded tiles for kk <= 16 columns: pcm = plain-column
+// stream (merbit_lp_build with map == NULL and pv == NULL), X rows of ldx
+// floats (ldx % 4 == 0, ldx >= merbit_lp_mm_chunk(kk) = kk rounded up to 4, X
+// 16-byte aligned), the first xr rows of X cached in LDS, carries of
+// merbit_lp_mm_chunk(kk) floats per tile
+int merbit_lp_mm_chunk(int kk);
+hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm,
+                          const void* pv, const int32_t* ty, const void* X, int64_t ldx, int kk, int xr, void* Y,
+                          int64_t ldy, void* carry, int32_t* carry_row, int grid, hipStream_t s);
 hipError_t merbit_lp_spmv(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pc,
                           const void* pv, const int32_t* ty, const void* x, const int32_t* hot, int xc, void* y,
                           void* carry, int32_t* carry_row, int grid, hipStream_t s);

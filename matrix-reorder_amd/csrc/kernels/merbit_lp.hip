@@ -215,6 +215,219 @@ __global__ __launch_bounds__(256) void tile_fixup_kernel(int64_t n_tiles, int64_
     if (lane == 63 && in && fl && r >= 0 && r < n_rows && r == rnext) y[r] = y[r] + s + extra;
 }
 
+// ---- SpMM on the same lane-padded tiles ------------------------------------
+// Y = A X for kk <= 4*KC right-hand sides.  The column stream here holds plain
+// column ids (-1 row end, -2 padding).  Lane l walks its S items with 4*KC
+// accumulators; a nonzero gathers the 16-byte chunks of its X row (X rows are
+// ldx floats, ldx % 4 == 0, ldx >= 4*KC, so the vector loads stay inside X).
+// The first xr rows of X (the hub columns after pGOrder) are cached in LDS.
+template <typename VT, int S, int W, int KC, bool NT>
+__global__ __launch_bounds__(64 * W) void merbit_lp_mm_kernel(int64_t n_tiles, const int32_t* __restrict__ pcm,
+                                                              const VT* __restrict__ pv, const int32_t* __restrict__ ty,
+                                                              const float* __restrict__ X, int64_t ldx, int kk, int xr,
+                                                              float* __restrict__ Y, int64_t ldy,
+                                                              float* __restrict__ carry, int32_t* __restrict__ carry_row)
+{
+    constexpr int T = 64 * S;
+    constexpr int K = 4 * KC;
+    constexpr int VW = S * (int)sizeof(VT) / 4;
+    static_assert(S % 4 == 0 && VW % 4 == 0, "lanes load whole 16-byte vectors");
+    extern __shared__ float xrows[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+
+    for (int64_t i = threadIdx.x; i < (int64_t)xr * K; i += 64 * W) {
+        const int64_t r = i / K;
+        const int c = (int)(i - r * K);
+        xrows[i] = X[r * ldx + c];
+    }
+    __syncthreads();
+
+    for (int64_t t = (int64_t)blockIdx.x * W + wid; t < n_tiles; t += (int64_t)gridDim.x * W) {
+        const int32_t y0 = ty[t], y1 = ty[t + 1];
+        const int64_t base = t * (int64_t)T + (int64_t)lane * S;
+        uint32_t cw[S], vw[VW];
+        const u32x4* cp = reinterpret_cast<const u32x4*>(pcm + base);
+        const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pv) + base * (int64_t)sizeof(VT));
+#pragma unroll
+        for (int q = 0; q < S / 4; ++q) {
+            const u32x4 a = ld_vec<NT>(cp + q);
+            cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
+        }
+#pragma unroll
+        for (int q = 0; q < VW / 4; ++q) {
+            const u32x4 b = ld_vec<NT>(vp + q);
+            vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
+        }
+
+        float acc[K], head[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) { acc[c] = 0.0f; head[c] = 0.0f; }
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) cnt += ((int32_t)cw[j] == kRowEnd) ? 1 : 0;
+        int inc = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        int row = inc - cnt;
+        bool has_end = false;
+        int head_row = 0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int32_t col = (int32_t)cw[j];
+            if (col == kRowEnd) {
+                if (!has_end) {
+#pragma unroll
+                    for (int c = 0; c < K; ++c) head[c] = acc[c];
+                    head_row = row;
+                    has_end = true;
+                } else {
+                    float* yr = Y + (int64_t)(y0 + row) * ldy;
+#pragma unroll
+                    for (int c = 0; c < K; ++c)
+                        if (c < kk) yr[c] = acc[c];
+                }
+#pragma unroll
+                for (int c = 0; c < K; ++c) acc[c] = 0.0f;
+                ++row;
+            } else if (col >= 0) {
+                const float v = value_at<VT>(vw, j);
+                if (col < xr) {
+                    const float4* src = reinterpret_cast<const float4*>(xrows + (int64_t)col * K);
+#pragma unroll
+                    for (int q = 0; q < KC; ++q) {
+                        const float4 xv = src[q];
+                        acc[4 * q] += v * xv.x; acc[4 * q + 1] += v * xv.y;
+                        acc[4 * q + 2] += v * xv.z; acc[4 * q + 3] += v * xv.w;
+                    }
+                } else {
+                    const float4* src = reinterpret_cast<const float4*>(X + (int64_t)col * ldx);
+#pragma unroll
+                    for (int q = 0; q < KC; ++q) {
+                        const float4 xv = src[q];
+                        acc[4 * q] += v * xv.x; acc[4 * q + 1] += v * xv.y;
+                        acc[4 * q + 2] += v * xv.z; acc[4 * q + 3] += v * xv.w;
+                    }
+                }
+            }
+        }
+
+        if (y0 == y1) {
+            // every item consumes a nonzero of row y0 (fast track)
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                const float tot = wave_sum(acc[c]);
+                if (lane == 0) carry[t * K + c] = tot;
+            }
+            if (lane == 0) carry_row[t] = y0;
+            continue;
+        }
+        // segmented scan of (tails, has_end) over the wave, column by column
+        int fl = has_end ? 1 : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int fu = __shfl_up(fl, d, 64);
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                const float vu = __shfl_up(acc[c], d, 64);
+                if (lane >= d && !fl) acc[c] += vu;
+            }
+            if (lane >= d) fl |= fu;
+        }
+        // every lane takes part in the shuffles; lanes with a row end use them
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            const float up = __shfl_up(acc[c], 1, 64);
+            head[c] += lane == 0 ? 0.0f : up;
+        }
+        if (has_end) {
+            float* yr = Y + (int64_t)(y0 + head_row) * ldy;
+#pragma unroll
+            for (int c = 0; c < K; ++c)
+                if (c < kk) yr[c] = head[c];
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) carry[t * K + c] = acc[c];
+            carry_row[t] = y1;
+        }
+    }
+}
+
+// k-wide version of tile_fixup_kernel: carry[t * K + c], columns c < kk
+template <int K>
+__global__ __launch_bounds__(256) void tile_fixup_mm_kernel(int64_t n_tiles, int64_t n_rows, int kk,
+                                                            const float* __restrict__ carry,
+                                                            const int32_t* __restrict__ carry_row,
+                                                            float* __restrict__ Y, int64_t ldy)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t base = w * 64;
+    if (base >= n_tiles) return;
+    const int64_t t = base + lane;
+    const bool in = t < n_tiles;
+    const int32_t r = in ? carry_row[t] : -1;
+    int32_t rprev = __shfl_up(r, 1, 64);
+    if (lane == 0) rprev = base > 0 ? carry_row[base - 1] : -2;
+    int32_t rnext = __shfl_down(r, 1, 64);
+    if (lane == 63) rnext = t + 1 < n_tiles ? carry_row[t + 1] : -3;
+    if (!in) rnext = -3;
+    const int32_t r63 = __shfl(r, 63, 64);
+    const int32_t rn63 = __shfl(rnext, 63, 64);
+    for (int c = 0; c < K && c < kk; ++c) {
+        float s = in ? carry[t * K + c] : 0.0f;
+        int fl = (in && r != rprev) ? 1 : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const float vu = __shfl_up(s, d, 64);
+            const int fu = __shfl_up(fl, d, 64);
+            if (lane >= d) {
+                if (!fl) s += vu;
+                fl |= fu;
+            }
+        }
+        const int fl63 = __shfl(fl, 63, 64);
+        float extra = 0.0f;
+        if (fl63 && r63 >= 0 && r63 < n_rows && rn63 == r63) {
+            for (int64_t b2 = base + 64; b2 < n_tiles; b2 += 64) {
+                const int64_t t2 = b2 + lane;
+                const bool same = t2 < n_tiles && carry_row[t2] == r63;
+                extra += wave_sum(same ? carry[t2 * K + c] : 0.0f);
+                if (__ballot(same) != ~0ull) break;
+            }
+        }
+        if (in && fl && r >= 0 && r < n_rows && r != rnext) Y[(int64_t)r * ldy + c] = Y[(int64_t)r * ldy + c] + s;
+        if (lane == 63 && in && fl && r >= 0 && r < n_rows && r == rnext)
+            Y[(int64_t)r * ldy + c] = Y[(int64_t)r * ldy + c] + s + extra;
+    }
+}
+
+template <typename VT, int S, int W, int KC>
+hipError_t launch_lp_mm(int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm, const void* pv, const int32_t* ty,
+                        const void* X, int64_t ldx, int kk, int xr, void* Y, int64_t ldy, void* carry,
+                        int32_t* carry_row, int grid, hipStream_t s)
+{
+    constexpr int K = 4 * KC;
+    const size_t lds = sizeof(float) * (size_t)xr * K;
+    auto kern = nt ? merbit_lp_mm_kernel<VT, S, W, KC, true> : merbit_lp_mm_kernel<VT, S, W, KC, false>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    kern<<<(unsigned)grid, 64 * W, lds, s>>>(n_tiles, pcm, static_cast<const VT*>(pv), ty,
+                                             static_cast<const float*>(X), ldx, kk, xr, static_cast<float*>(Y), ldy,
+                                             static_cast<float*>(carry), carry_row);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t waves = (n_tiles + 63) / 64;
+    tile_fixup_mm_kernel<K><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(n_tiles, n_rows, kk,
+                                                                        static_cast<const float*>(carry), carry_row,
+                                                                        static_cast<float*>(Y), ldy);
+    return hipGetLastError();
+}
+
 // ---- plan build: the merge path as padded (column, value) streams ----------
 __global__ void lp_fill_kernel(int64_t from, int64_t to, int32_t* __restrict__ pc, uint16_t* __restrict__ pv16,
                                uint32_t* __restrict__ pv32)
@@ -239,12 +452,12 @@ __global__ __launch_bounds__(256) void lp_scatter_kernel(int64_t n_rows, const i
     for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rows; r += waves) {
         const int32_t b = rp[r], e = rp[r + 1];
         for (int32_t k = b + lane; k < e; k += 64) {
-            pc[(int64_t)k + r] = map[ci[k]];
-            pv[(int64_t)k + r] = val[k];
+            pc[(int64_t)k + r] = map ? map[ci[k]] : ci[k];
+            if (pv) pv[(int64_t)k + r] = val[k];
         }
         if (lane == 0) {
             pc[(int64_t)e + r] = kRowEnd;
-            pv[(int64_t)e + r] = VT(0);
+            if (pv) pv[(int64_t)e + r] = VT(0);
         }
     }
 }
@@ -272,6 +485,51 @@ hipError_t launch_lp(int64_t n_rows, int64_t n_tiles, const int32_t* pc, const v
 
 }  // namespace
 
+int merbit_lp_mm_chunk(int kk) { return (kk + 3) / 4 * 4; }
+
+hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm,
+                          const void* pv, const int32_t* ty, const void* X, int64_t ldx, int kk, int xr, void* Y,
+                          int64_t ldy, void* carry, int32_t* carry_row, int grid, hipStream_t s)
+{
+    if (n_tiles <= 0 || n_rows <= 0 || kk <= 0) return hipSuccess;
+    const int K = merbit_lp_mm_chunk(kk);
+    if (dt == DT_F64 || grid <= 0 || kk > 16 || ldx < K || (ldx & 3) || ldy < kk ||
+        (reinterpret_cast<uintptr_t>(X) & 15))
+        return hipErrorInvalidValue;
+    return dispatch_dt(dt, [&](auto v) -> hipError_t {
+        using VT = decltype(v);
+        if constexpr (sizeof(typename Acc<VT>::type) == 8) {
+            return hipErrorInvalidValue;
+        } else {
+            auto go = [&](auto stag, auto wtag) -> hipError_t {
+                constexpr int SS = decltype(stag)::value;
+                constexpr int WW = decltype(wtag)::value;
+                if constexpr (SS * sizeof(VT) % 16 != 0 || SS % 4 != 0) {
+                    return hipErrorInvalidValue;
+                } else {
+                    switch (K) {
+                    case 4: return launch_lp_mm<VT, SS, WW, 1>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y, ldy,
+                                                               carry, carry_row, grid, s);
+                    case 8: return launch_lp_mm<VT, SS, WW, 2>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y, ldy,
+                                                               carry, carry_row, grid, s);
+                    case 12: return launch_lp_mm<VT, SS, WW, 3>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y,
+                                                                ldy, carry, carry_row, grid, s);
+                    default: return launch_lp_mm<VT, SS, WW, 4>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y,
+                                                                 ldy, carry, carry_row, grid, s);
+                    }
+                }
+            };
+            using I8 = std::integral_constant<int, 8>;
+            using I16 = std::integral_constant<int, 16>;
+            if (S == 8 && W == 16) return go(I8{}, I16{});
+            if (S == 16 && W == 16) return go(I16{}, I16{});
+            if (S == 8 && W == 8) return go(I8{}, I8{});
+            if (S == 16 && W == 8) return go(I16{}, I8{});
+            return hipErrorInvalidValue;
+        }
+    });
+}
+
 size_t merbit_lp_lds_bytes(int xc) { return sizeof(float) * (size_t)((xc + 63) / 64 * 64 + 64); }
 
 hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int32_t* ci, const void* val,
@@ -281,8 +539,9 @@ hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int3
     if (padded > items) {
         int64_t g = (padded - items + 255) / 256;
         if (g > 4096) g = 4096;
-        lp_fill_kernel<<<(unsigned)g, 256, 0, s>>>(items, padded, pc, dt == DT_F32 ? nullptr : static_cast<uint16_t*>(pv),
-                                                   dt == DT_F32 ? static_cast<uint32_t*>(pv) : nullptr);
+        lp_fill_kernel<<<(unsigned)g, 256, 0, s>>>(items, padded, pc,
+                                                   (dt == DT_F32 || !pv) ? nullptr : static_cast<uint16_t*>(pv),
+                                                   (dt == DT_F32 && pv) ? static_cast<uint32_t*>(pv) : nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

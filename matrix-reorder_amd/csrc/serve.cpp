@@ -60,6 +60,8 @@ struct Job {
     int mat = 0;
     mr_mat m = nullptr;
     int K = 0;
+    bool lp = false;                  // runs through the lane-padded kernels
+    int64_t ldx = 0;
     std::vector<int64_t> reqs;
     std::vector<int> off;             // first column of each request
     std::vector<int> ks;              // its k and seed (copied under the lock)
@@ -108,6 +110,15 @@ int class_of(int K)
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
+// jobs on matrices with a lane-padded plan run through the lane-padded kernels
+// (one launch each) instead of the fused batch
+bool use_lp(const mr_mat m)
+{
+    return (m->merbit.policy & 4) && m->dtype != MR_F64 && m->merbit.xc_built && m->merbit.pcm_built;
+}
+
+int64_t lp_ldx(int K) { return K == 1 ? 1 : (K + 3) / 4 * 4; }
+
 }  // namespace
 
 struct mr_server_s {
@@ -143,6 +154,14 @@ namespace {
 // device bytes of a job with K columns on matrix m
 size_t job_bytes(const mr_mat m, int K, int sigma, int64_t* tiles)
 {
+    if (use_lp(m)) {
+        const int64_t t = m->merbit.xtiles;
+        if (tiles) *tiles = t;
+        return align256(sizeof(float) * (size_t)std::max<int64_t>(m->n_cols, 1) * (size_t)lp_ldx(K)) +
+               align256(sizeof(float) * (size_t)std::max<int64_t>(m->n_rows, 1) * (size_t)K) +
+               align256(sizeof(float) * (size_t)(K == 1 ? 1 : 16) * (size_t)std::max<int64_t>(t, 1)) +
+               align256(sizeof(int32_t) * (size_t)std::max<int64_t>(t, 1));
+    }
     const int64_t items = m->n_rows + m->nnz;
     const int64_t t = (K == 1) ? (items + 256LL * sigma - 1) / (256LL * sigma) : (items + 4095) / 4096;
     if (tiles) *tiles = t;
@@ -246,10 +265,12 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
     size_t cur = s->table_bytes;
     for (Job& J : sl.jobs) {
         const mr_mat m = J.m;
-        const int G = carry_group(J.K);
+        J.lp = use_lp(m);
+        J.ldx = J.lp ? lp_ldx(J.K) : J.K;
+        const int G = J.lp ? (J.K == 1 ? 1 : 16) : carry_group(J.K);
         job_bytes(m, J.K, sigma, &J.tiles);
         J.x_off = cur;
-        cur += align256(vb * (size_t)std::max<int64_t>(m->n_cols, 1) * (size_t)J.K);
+        cur += align256(vb * (size_t)std::max<int64_t>(m->n_cols, 1) * (size_t)J.ldx);
         J.y_off = cur;
         cur += align256(vb * (size_t)std::max<int64_t>(m->n_rows, 1) * (size_t)J.K);
         J.c_off = cur;
@@ -264,7 +285,8 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
     struct ClassTab { size_t jobs_off = 0, begin_off = 0; int n = 0; int64_t total = 0; };
     ClassTab ct[kClasses];
     std::vector<std::vector<size_t>> members(kClasses);
-    for (size_t j = 0; j < sl.jobs.size(); ++j) members[(size_t)class_of(sl.jobs[j].K)].push_back(j);
+    for (size_t j = 0; j < sl.jobs.size(); ++j)
+        if (!sl.jobs[j].lp) members[(size_t)class_of(sl.jobs[j].K)].push_back(j);
     for (int c = 0; c < kClasses; ++c) {
         if (members[(size_t)c].empty()) continue;
         ClassTab& T = ct[c];
@@ -331,7 +353,7 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
             k::ServeReq& E = rq[i];
             E.x = dev + J.x_off + vb * (size_t)J.off[q];
             E.y = dev + J.y_off + vb * (size_t)J.off[q];
-            E.ldx = J.K;
+            E.ldx = J.ldx;
             E.ldy = J.K;
             E.n_cols = J.m->n_cols;
             E.n_rows = J.m->n_rows;
@@ -356,6 +378,28 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
             MR_HIP_TRY(k::merbit_batch_spmv(s->dtype, sigma, 1, tab, beg, T.n, T.total, (int)grid, st));
         else
             MR_HIP_TRY(k::merbit_batch_spmm(s->dtype, 1 << c, tab, beg, T.n, T.total, (int)grid, st));
+    }
+    // large matrices: one lane-padded launch per job (SpMM in 16-column passes)
+    for (const Job& J : sl.jobs) {
+        if (!J.lp) continue;
+        const mr_mat m = J.m;
+        const MerbitPlan& p = m->merbit;
+        const int grid = std::max(s->ctx->cu_count, 1);
+        if (J.K == 1) {
+            MR_HIP_TRY(k::merbit_lp_spmv(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles, p.pc.as<int32_t>(),
+                                         p.pv.p, p.xty.as<int32_t>(), dev + J.x_off, p.hot.as<int32_t>(), p.xc,
+                                         dev + J.y_off, dev + J.c_off, reinterpret_cast<int32_t*>(dev + J.r_off), grid,
+                                         st));
+        } else {
+            for (int j0 = 0; j0 < J.K; j0 += 16) {
+                const int kk = std::min(16, J.K - j0);
+                MR_HIP_TRY(k::merbit_lp_spmm(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles,
+                                             p.pcm.as<int32_t>(), p.pv.p, p.xty.as<int32_t>(),
+                                             dev + J.x_off + vb * (size_t)j0, J.ldx, kk, merbit_lp_mm_rows(m, kk),
+                                             dev + J.y_off + vb * (size_t)j0, J.K, dev + J.c_off,
+                                             reinterpret_cast<int32_t*>(dev + J.r_off), grid, st));
+            }
+        }
     }
     if (s->cfg.verify) {
         auto* hd = reinterpret_cast<uint64_t*>(dev + s->hash_off);
@@ -554,6 +598,7 @@ void upload_loop(mr_server s)
             int64_t nb;
             rc = merbit_coords_sigma(mat, s->cfg.sigma, s->ctx->stream, &bx, &by, &nb);
             if (rc == MR_OK) rc = merbit_mm_grid(mat, s->ctx->stream);
+            if (rc == MR_OK && (mat->merbit.policy & 4)) rc = merbit_lp_mm_build(mat, s->ctx->stream);
             if (rc == MR_OK && hipStreamSynchronize(s->ctx->stream) != hipSuccess) rc = MR_ERR_HIP;
         }
         if (rc != MR_OK && mat) {
@@ -737,6 +782,7 @@ int mr_serve_add(mr_server s, mr_mat m, int* id)
     if (m->n_rows > 0) {
         MR_TRY(merbit_coords_sigma(m, s->cfg.sigma, s->ctx->stream, &bx, &by, &nb));
         MR_TRY(merbit_mm_grid(m, s->ctx->stream));
+        if (m->merbit.policy & 4) MR_TRY(merbit_lp_mm_build(m, s->ctx->stream));
         MR_HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     }
     std::lock_guard<std::mutex> g(s->mu);

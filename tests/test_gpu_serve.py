@@ -209,3 +209,40 @@ def test_server_argument_errors(ctx, mr):
     assert e.value.code == 5
     srv.close()
     f32.close()
+
+
+def test_lane_padded_route_matches_library(ctx, mr):
+    """Matrices with the lane-padded plan (policy bit 2) are served through the
+    lane-padded SpMV / SpMM kernels; without coalescing every result equals
+    the library's mr_spmv / mr_spmm on the same matrix bit for bit."""
+    rp, ci = rmat_csr(12, 16, seed=31)
+    n = len(rp) - 1
+    rng = np.random.default_rng(9)
+    big = mr.Mat.from_csr(ctx, rp, ci, rng.uniform(-1, 1, len(ci)), n_cols=n, store=mr.BF16)
+    big.merbit_options(4)
+    mats, host = resident(ctx, mr)
+    srv = mr.Server(ctx, verify=1, coalesce=0, slots=2, max_jobs=8)
+    bid = srv.add(big)
+    ids = [srv.add(m) for m in mats]
+    mix = [(bid, k, 100 + i) for i, k in enumerate([1, 4, 16, 3, 8, 1, 16])]
+    mix += [(ids[i % len(ids)], 4, 500 + i) for i in range(6)]
+    reqs = [srv.submit(m, k, sd) for m, k, sd in mix]
+    got = [srv.wait(r) for r in reqs]
+    assert all(st == OK for st, _, _ in got)
+    for (mid, k, sd), (_, _, h) in zip(mix, got):
+        if mid != bid:
+            continue
+        x = mr.serve_rhs(sd, n, k)
+        ldx = 1 if k == 1 else -(-k // 4) * 4       # the server's X layout
+        xp = np.zeros((n, ldx), np.float32)
+        xp[:, :k] = x
+        xd = ctx.upload(xp)
+        yd = ctx.empty((n, k), np.float32)
+        if k == 1:
+            big.spmv(xd, yd, mr.MERBIT)
+        else:
+            big.spmm(k, xd, yd, mr.MERBIT, ldx=ldx, ldy=k)
+        ctx.sync()
+        assert mr.serve_hash(yd.to_host()) == h, (k, sd)
+    srv.close()
+    big.close()

@@ -267,3 +267,39 @@ def test_merbit_xcache_rejected_for_f64(ctx, mr):
         mat.merbit_xc_configure(16, 16, -1)
     assert e.value.code == 5
     mat.close()
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("k,ldx", [(2, 4), (3, 4), (4, 4), (8, 12), (13, 16), (16, 16), (33, 36), (5, 5)])
+@pytest.mark.parametrize("store", ["f32", "bf16"])
+def test_spmm_lane_padded(ctx, mr, oracle, case, k, ldx, store):
+    """SpMM on the lane-padded tiles (policy bit 2) with the LDS cache of the
+    first X rows: 16 columns per pass, ldx a multiple of 4 (ldx = 5 takes the
+    workgroup-tile SpMM instead); 1e-5 of the FP64 oracle on the stored
+    values, padding columns of Y untouched, bit-reproducible."""
+    name, rp, ci, nc = case
+    rng = np.random.default_rng(200 + k)
+    val = rng.uniform(-1, 1, len(ci))
+    X = rng.uniform(-1, 1, (nc, ldx)).astype(np.float32)
+    st = mr.F32 if store == "f32" else mr.BF16
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=st)
+    mat.merbit_options(4)
+    _, _, vs = mat.export()
+    v64 = vs.astype(np.float64) if st == mr.F32 else (vs.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    n = len(rp) - 1
+    ldy = k + 2
+    Xd = ctx.upload(X)
+    Yd = ctx.upload(np.full((n, ldy), 7.0, np.float32))
+    mat.spmm(k, Xd, Yd, mr.MERBIT, ldx=ldx, ldy=ldy)
+    ctx.sync()
+    Y = Yd.to_host()
+    Yh = oracle.spmm_f64(rp, ci, v64, X[:, :k].astype(np.float64))
+    absm = np.abs(sp_abs(rp, ci, v64, nc)) @ np.abs(X[:, :k].astype(np.float64)) if n else np.zeros((0, k))
+    err = np.abs(Y[:, :k] - Yh)
+    assert np.all(err <= 1e-5 * np.maximum(absm, 1e-30) + 1e-30), float(np.max(err / np.maximum(absm, 1e-30)))
+    assert np.all(Y[:, k:] == 7.0)
+    Yd2 = ctx.upload(np.zeros((n, ldy), np.float32))
+    mat.spmm(k, Xd, Yd2, mr.MERBIT, ldx=ldx, ldy=ldy)
+    ctx.sync()
+    assert np.array_equal(Yd2.to_host()[:, :k].view(np.uint32), Y[:, :k].view(np.uint32))
+    mat.close()
