@@ -683,7 +683,8 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
     }
     if (m->n_rows == 0) return MR_OK;
     MerbitPlan& p = m->merbit;
-    if ((p.policy & 4) && m->dtype != MR_F64 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0) {
+    if ((p.policy & 4) && m->dtype != MR_F64 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+        k::merbit_lp_mm_fits(p.xs, p.xw, std::min(k, 16))) {
         // lane-padded tiles: 16 columns per pass, X rows cached in LDS
         MR_TRY(merbit_lp_mm_build(m, st));
         const size_t vb = vec_size(m->dtype);

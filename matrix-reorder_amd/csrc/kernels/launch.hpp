@@ -89,6 +89,7 @@ hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int3
 // 16-byte aligned), the first xr rows of X cached in LDS, carries of
 // merbit_lp_mm_chunk(kk) floats per tile
 int merbit_lp_mm_chunk(int kk);
+bool merbit_lp_mm_fits(int S, int W, int kk);
 hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm,
                           const void* pv, const int32_t* ty, const void* X, int64_t ldx, int kk, int xr, void* Y,
                           int64_t ldy, void* carry, int32_t* carry_row, int grid, hipStream_t s);

@@ -216,12 +216,14 @@ __global__ __launch_bounds__(256) void tile_fixup_kernel(int64_t n_tiles, int64_
 }
 
 // ---- SpMM on the same lane-padded tiles ------------------------------------
-// Y = A X for kk <= 4*KC right-hand sides.  The column stream here holds plain
-// column ids (-1 row end, -2 padding).  Lane l walks its S items with 4*KC
-// accumulators; a nonzero gathers the 16-byte chunks of its X row (X rows are
-// ldx floats, ldx % 4 == 0, ldx >= 4*KC, so the vector loads stay inside X).
-// The first xr rows of X (the hub columns after pGOrder) are cached in LDS.
-template <typename VT, int S, int W, int KC, bool NT>
+// Y = A X for kk <= 4*G right-hand sides.  The column stream here holds plain
+// column ids (-1 row end, -2 padding).  The tiles are the SpMV tiles (64 * S
+// items); a group of G lanes walks G*S consecutive items together and lane q
+// of the group owns columns [4q, 4q+4): a nonzero's X row is read by the G
+// lanes of its group with one 16-byte load each, which the L1 coalesces into
+// one line request.  X rows are ldx floats (ldx % 4 == 0, ldx >= 4*ceil(kk/4));
+// the first xr rows of X (the hub columns after pGOrder) are cached in LDS.
+template <typename VT, int S, int W, int G, bool NT>
 __global__ __launch_bounds__(64 * W) void merbit_lp_mm_kernel(int64_t n_tiles, const int32_t* __restrict__ pcm,
                                                               const VT* __restrict__ pv, const int32_t* __restrict__ ty,
                                                               const float* __restrict__ X, int64_t ldx, int kk, int xr,
@@ -229,45 +231,46 @@ __global__ __launch_bounds__(64 * W) void merbit_lp_mm_kernel(int64_t n_tiles, c
                                                               float* __restrict__ carry, int32_t* __restrict__ carry_row)
 {
     constexpr int T = 64 * S;
-    constexpr int K = 4 * KC;
-    constexpr int VW = S * (int)sizeof(VT) / 4;
-    static_assert(S % 4 == 0 && VW % 4 == 0, "lanes load whole 16-byte vectors");
+    constexpr int K = 4 * G;            // cached / carried columns
+    constexpr int SG = S * G;           // items per group
+    constexpr int VW = SG * (int)sizeof(VT) / 4;
+    static_assert(SG % 4 == 0 && VW % 4 == 0, "groups load whole 16-byte vectors");
     extern __shared__ float xrows[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int grp = lane / G, q = lane % G;
+    const bool own = 4 * q < kk;        // this lane has columns to compute
 
     for (int64_t i = threadIdx.x; i < (int64_t)xr * K; i += 64 * W) {
         const int64_t r = i / K;
         const int c = (int)(i - r * K);
-        xrows[i] = X[r * ldx + c];
+        xrows[i] = c < 4 * ((kk + 3) / 4) ? X[r * ldx + c] : 0.0f;
     }
     __syncthreads();
 
     for (int64_t t = (int64_t)blockIdx.x * W + wid; t < n_tiles; t += (int64_t)gridDim.x * W) {
         const int32_t y0 = ty[t], y1 = ty[t + 1];
-        const int64_t base = t * (int64_t)T + (int64_t)lane * S;
-        uint32_t cw[S], vw[VW];
+        const int64_t base = t * (int64_t)T + (int64_t)grp * SG;
+        uint32_t cw[SG], vw[VW];
         const u32x4* cp = reinterpret_cast<const u32x4*>(pcm + base);
         const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pv) + base * (int64_t)sizeof(VT));
 #pragma unroll
-        for (int q = 0; q < S / 4; ++q) {
-            const u32x4 a = ld_vec<NT>(cp + q);
-            cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
+        for (int i = 0; i < SG / 4; ++i) {
+            const u32x4 a = ld_vec<NT>(cp + i);
+            cw[4 * i] = a.x; cw[4 * i + 1] = a.y; cw[4 * i + 2] = a.z; cw[4 * i + 3] = a.w;
         }
 #pragma unroll
-        for (int q = 0; q < VW / 4; ++q) {
-            const u32x4 b = ld_vec<NT>(vp + q);
-            vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
+        for (int i = 0; i < VW / 4; ++i) {
+            const u32x4 b = ld_vec<NT>(vp + i);
+            vw[4 * i] = b.x; vw[4 * i + 1] = b.y; vw[4 * i + 2] = b.z; vw[4 * i + 3] = b.w;
         }
 
-        float acc[K], head[K];
-#pragma unroll
-        for (int c = 0; c < K; ++c) { acc[c] = 0.0f; head[c] = 0.0f; }
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, head[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         int cnt = 0;
 #pragma unroll
-        for (int j = 0; j < S; ++j) cnt += ((int32_t)cw[j] == kRowEnd) ? 1 : 0;
+        for (int j = 0; j < SG; ++j) cnt += ((int32_t)cw[j] == kRowEnd) ? 1 : 0;
         int inc = cnt;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
+        for (int d = G; d < 64; d <<= 1) {
             const int o = __shfl_up(inc, d, 64);
             if (lane >= d) inc += o;
         }
@@ -275,83 +278,71 @@ __global__ __launch_bounds__(64 * W) void merbit_lp_mm_kernel(int64_t n_tiles, c
         bool has_end = false;
         int head_row = 0;
 #pragma unroll
-        for (int j = 0; j < S; ++j) {
+        for (int j = 0; j < SG; ++j) {
             const int32_t col = (int32_t)cw[j];
             if (col == kRowEnd) {
                 if (!has_end) {
 #pragma unroll
-                    for (int c = 0; c < K; ++c) head[c] = acc[c];
+                    for (int e = 0; e < 4; ++e) head[e] = acc[e];
                     head_row = row;
                     has_end = true;
-                } else {
-                    float* yr = Y + (int64_t)(y0 + row) * ldy;
+                } else if (own) {
+                    float* yr = Y + (int64_t)(y0 + row) * ldy + 4 * q;
 #pragma unroll
-                    for (int c = 0; c < K; ++c)
-                        if (c < kk) yr[c] = acc[c];
+                    for (int e = 0; e < 4; ++e)
+                        if (4 * q + e < kk) yr[e] = acc[e];
                 }
 #pragma unroll
-                for (int c = 0; c < K; ++c) acc[c] = 0.0f;
+                for (int e = 0; e < 4; ++e) acc[e] = 0.0f;
                 ++row;
-            } else if (col >= 0) {
+            } else if (col >= 0 && own) {
                 const float v = value_at<VT>(vw, j);
-                if (col < xr) {
-                    const float4* src = reinterpret_cast<const float4*>(xrows + (int64_t)col * K);
-#pragma unroll
-                    for (int q = 0; q < KC; ++q) {
-                        const float4 xv = src[q];
-                        acc[4 * q] += v * xv.x; acc[4 * q + 1] += v * xv.y;
-                        acc[4 * q + 2] += v * xv.z; acc[4 * q + 3] += v * xv.w;
-                    }
-                } else {
-                    const float4* src = reinterpret_cast<const float4*>(X + (int64_t)col * ldx);
-#pragma unroll
-                    for (int q = 0; q < KC; ++q) {
-                        const float4 xv = src[q];
-                        acc[4 * q] += v * xv.x; acc[4 * q + 1] += v * xv.y;
-                        acc[4 * q + 2] += v * xv.z; acc[4 * q + 3] += v * xv.w;
-                    }
-                }
+                const float4 xv = col < xr ? *reinterpret_cast<const float4*>(xrows + (int64_t)col * K + 4 * q)
+                                           : *reinterpret_cast<const float4*>(X + (int64_t)col * ldx + 4 * q);
+                acc[0] += v * xv.x; acc[1] += v * xv.y; acc[2] += v * xv.z; acc[3] += v * xv.w;
             }
         }
 
         if (y0 == y1) {
-            // every item consumes a nonzero of row y0 (fast track)
+            // every item consumes a nonzero of row y0 (fast track): sum the
+            // groups, lanes with the same q hold the same columns
 #pragma unroll
-            for (int c = 0; c < K; ++c) {
-                const float tot = wave_sum(acc[c]);
-                if (lane == 0) carry[t * K + c] = tot;
+            for (int e = 0; e < 4; ++e) {
+                float v = acc[e];
+#pragma unroll
+                for (int off = 32; off >= G; off >>= 1) v += __shfl_xor(v, off, 64);
+                if (lane < G && 4 * q + e < kk) carry[t * K + 4 * q + e] = v;
             }
             if (lane == 0) carry_row[t] = y0;
             continue;
         }
-        // segmented scan of (tails, has_end) over the wave, column by column
+        // segmented scan of (tails, has_end) over the groups of the wave
         int fl = has_end ? 1 : 0;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
+        for (int d = G; d < 64; d <<= 1) {
             const int fu = __shfl_up(fl, d, 64);
 #pragma unroll
-            for (int c = 0; c < K; ++c) {
-                const float vu = __shfl_up(acc[c], d, 64);
-                if (lane >= d && !fl) acc[c] += vu;
+            for (int e = 0; e < 4; ++e) {
+                const float vu = __shfl_up(acc[e], d, 64);
+                if (lane >= d && !fl) acc[e] += vu;
             }
             if (lane >= d) fl |= fu;
         }
-        // every lane takes part in the shuffles; lanes with a row end use them
 #pragma unroll
-        for (int c = 0; c < K; ++c) {
-            const float up = __shfl_up(acc[c], 1, 64);
-            head[c] += lane == 0 ? 0.0f : up;
+        for (int e = 0; e < 4; ++e) {
+            const float up = __shfl_up(acc[e], G, 64);
+            head[e] += lane < G ? 0.0f : up;
         }
-        if (has_end) {
-            float* yr = Y + (int64_t)(y0 + head_row) * ldy;
+        if (has_end && own) {
+            float* yr = Y + (int64_t)(y0 + head_row) * ldy + 4 * q;
 #pragma unroll
-            for (int c = 0; c < K; ++c)
-                if (c < kk) yr[c] = head[c];
+            for (int e = 0; e < 4; ++e)
+                if (4 * q + e < kk) yr[e] = head[e];
         }
-        if (lane == 63) {
+        if (lane >= 64 - G) {
 #pragma unroll
-            for (int c = 0; c < K; ++c) carry[t * K + c] = acc[c];
-            carry_row[t] = y1;
+            for (int e = 0; e < 4; ++e) carry[t * K + 4 * q + e] = acc[e];
+            if (lane == 63) carry_row[t] = y1;
         }
     }
 }
@@ -405,14 +396,14 @@ __global__ __launch_bounds__(256) void tile_fixup_mm_kernel(int64_t n_tiles, int
     }
 }
 
-template <typename VT, int S, int W, int KC>
+template <typename VT, int S, int W, int G>
 hipError_t launch_lp_mm(int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm, const void* pv, const int32_t* ty,
                         const void* X, int64_t ldx, int kk, int xr, void* Y, int64_t ldy, void* carry,
                         int32_t* carry_row, int grid, hipStream_t s)
 {
-    constexpr int K = 4 * KC;
+    constexpr int K = 4 * G;
     const size_t lds = sizeof(float) * (size_t)xr * K;
-    auto kern = nt ? merbit_lp_mm_kernel<VT, S, W, KC, true> : merbit_lp_mm_kernel<VT, S, W, KC, false>;
+    auto kern = nt ? merbit_lp_mm_kernel<VT, S, W, G, true> : merbit_lp_mm_kernel<VT, S, W, G, false>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -485,7 +476,11 @@ hipError_t launch_lp(int64_t n_rows, int64_t n_tiles, const int32_t* pc, const v
 
 }  // namespace
 
-int merbit_lp_mm_chunk(int kk) { return (kk + 3) / 4 * 4; }
+int merbit_lp_mm_chunk(int kk) { return kk <= 4 ? 4 : kk <= 8 ? 8 : 16; }
+
+// shapes whose group registers fit without spilling (16 items per lane with
+// 16 waves leaves 128 VGPRs: only one lane per item fits)
+bool merbit_lp_mm_fits(int S, int W, int kk) { return !(S == 16 && W == 16 && kk > 4); }
 
 hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm,
                           const void* pv, const int32_t* ty, const void* X, int64_t ldx, int kk, int xr, void* Y,
@@ -493,7 +488,8 @@ hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t 
 {
     if (n_tiles <= 0 || n_rows <= 0 || kk <= 0) return hipSuccess;
     const int K = merbit_lp_mm_chunk(kk);
-    if (dt == DT_F64 || grid <= 0 || kk > 16 || ldx < K || (ldx & 3) || ldy < kk ||
+    if (!merbit_lp_mm_fits(S, W, kk)) return hipErrorInvalidValue;
+    if (dt == DT_F64 || grid <= 0 || kk > 16 || ldx < (kk + 3) / 4 * 4 || (ldx & 3) || ldy < kk ||
         (reinterpret_cast<uintptr_t>(X) & 15))
         return hipErrorInvalidValue;
     return dispatch_dt(dt, [&](auto v) -> hipError_t {
@@ -512,8 +508,6 @@ hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t 
                                                                carry, carry_row, grid, s);
                     case 8: return launch_lp_mm<VT, SS, WW, 2>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y, ldy,
                                                                carry, carry_row, grid, s);
-                    case 12: return launch_lp_mm<VT, SS, WW, 3>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y,
-                                                                ldy, carry, carry_row, grid, s);
                     default: return launch_lp_mm<VT, SS, WW, 4>(nt, n_rows, n_tiles, pcm, pv, ty, X, ldx, kk, xr, Y,
                                                                  ldy, carry, carry_row, grid, s);
                     }

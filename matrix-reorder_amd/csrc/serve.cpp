@@ -112,9 +112,10 @@ size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
 // jobs on matrices with a lane-padded plan run through the lane-padded kernels
 // (one launch each) instead of the fused batch
-bool use_lp(const mr_mat m)
+bool use_lp(const mr_mat m, int K)
 {
-    return (m->merbit.policy & 4) && m->dtype != MR_F64 && m->merbit.xc_built && m->merbit.pcm_built;
+    return (m->merbit.policy & 4) && m->dtype != MR_F64 && m->merbit.xc_built && m->merbit.pcm_built &&
+           (K == 1 || k::merbit_lp_mm_fits(m->merbit.xs, m->merbit.xw, std::min(K, 16)));
 }
 
 int64_t lp_ldx(int K) { return K == 1 ? 1 : (K + 3) / 4 * 4; }
@@ -154,7 +155,7 @@ namespace {
 // device bytes of a job with K columns on matrix m
 size_t job_bytes(const mr_mat m, int K, int sigma, int64_t* tiles)
 {
-    if (use_lp(m)) {
+    if (use_lp(m, K)) {
         const int64_t t = m->merbit.xtiles;
         if (tiles) *tiles = t;
         return align256(sizeof(float) * (size_t)std::max<int64_t>(m->n_cols, 1) * (size_t)lp_ldx(K)) +
@@ -265,7 +266,7 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
     size_t cur = s->table_bytes;
     for (Job& J : sl.jobs) {
         const mr_mat m = J.m;
-        J.lp = use_lp(m);
+        J.lp = use_lp(m, J.K);
         J.ldx = J.lp ? lp_ldx(J.K) : J.K;
         const int G = J.lp ? (J.K == 1 ? 1 : 16) : carry_group(J.K);
         job_bytes(m, J.K, sigma, &J.tiles);
