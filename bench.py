@@ -20,6 +20,9 @@ Further sections (reported under "extra", each on its own synthetic inputs):
   serving    32 resident reordered BF16 matrices, Poisson arrivals, k in
              {1, 4, 16}, every 1000th request uploads an R-MAT s18 that is
              reordered with pGOrder in the background; p50 / p99 latency;
+  reference_scale  the shape of the reference's published MERBIT runs (symmetric
+             R-MAT s24 ef4, all-ones values and x), unreordered and after
+             pGOrder, beside the published sm_89 figures;
   cpu        the CPU baseline: the oracle's threaded CSR SpMV (FP32) on the
              same matrix, plus the host reorderers (sequential Gorder, RCM,
              Rabbit Order) beside pGOrder on R-MAT s18, each with the SpMV
@@ -365,6 +368,39 @@ def section_serving(mr, ctx, a, extra, batch_mats):
     extra["serving"] = res
 
 
+def section_reference_scale(mr, ctx, a, extra):
+    """The shape of the reference's published MERBIT runs: a symmetric R-MAT
+    s24, edge factor 4, (0.57, 0.19, 0.19), values and x all ones, FP32;
+    MERBIT unreordered and after pGOrder (width 1024: get_width asks for 16384
+    here, which this pGOrder loop does not support)."""
+    t = time.perf_counter()
+    rp, ci = mr.gen_rmat(24, 4, (0.57, 0.19, 0.19), seed=24, symmetric=True)
+    n, nnz = len(rp) - 1, len(ci)
+    log(f"reference-scale R-MAT s24 ef4 symmetric: n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+    m = mr.Mat.from_csr(ctx, rp, ci, None, store=mr.F32)
+    del rp, ci
+    xd = ctx.upload(np.ones(n, np.float32))
+    yd = ctx.empty(n, np.float32)
+    out = {"workload": "R-MAT s24 ef4 (0.57,0.19,0.19) symmetric, values 1, x 1, FP32", "n": n, "nnz": nnz}
+    m.merbit_autotune(5)
+    out["unreordered_ms"], _ = time_spmv(ctx, mr, m, xd, yd, mr.MERBIT, 50, 5)
+    t = time.perf_counter()
+    _, st = m.reorder_pgorder(1024, 5)
+    out["pgorder"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in st.items()}
+    out["pgorder_wall_ms"] = (time.perf_counter() - t) * 1e3
+    m.merbit_autotune(5)
+    out["pgorder_ms"], _ = time_spmv(ctx, mr, m, xd, yd, mr.MERBIT, 50, 5)
+    for key in ("unreordered", "pgorder"):
+        ms = out[f"{key}_ms"]
+        out[f"{key}_gflops"] = 2.0 * nnz / (ms * 1e-3) / 1e9
+        out[f"{key}_gbs"] = csr_bytes(n, n, nnz, 4) / (ms * 1e-3) / 1e9
+    out["reference_published_sm89"] = {"unreordered_gflops": 95.1, "gorder_gflops": 153.8}
+    log(f"  unreordered {out['unreordered_ms'] * 1e3:.0f} us ({out['unreordered_gflops']:.0f} GFLOP/s), pGOrder "
+        f"{out['pgorder_wall_ms'] / 1e3:.1f} s, after {out['pgorder_ms'] * 1e3:.0f} us ({out['pgorder_gflops']:.0f} GFLOP/s)")
+    m.close()
+    extra["reference_scale"] = out
+
+
 def section_cpu(mr, ctx, a, data):
     """The oracle's threaded CSR SpMV (FP32) on the headline matrix, ~10 s."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -422,7 +458,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--perm-cache", default="", help="load (or save) the s22 pGOrder permutation here")
     ap.add_argument("--policy", type=int, default=-1, help="force a MERBIT load policy (after the autotuner)")
-    ap.add_argument("--sections", default="precision,batched,serving,cpu_reorders")
+    ap.add_argument("--sections", default="precision,batched,serving,reference_scale,cpu_reorders")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5000.0, 20000.0, 50000.0])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-upload-every", type=int, default=1000)
@@ -449,6 +485,8 @@ def main():
         section_serving(mr, ctx, a, extra, batch_mats)
     for m in batch_mats:
         m.close()
+    if "reference_scale" in sections:
+        section_reference_scale(mr, ctx, a, extra)
     cpu = None
     if not a.no_cpu and not a.only_main:
         log("CPU baseline (oracle threaded CSR SpMV, FP32)")

@@ -216,13 +216,18 @@ def device_count() -> int:
 
 # ---------------------------------------------------------------- host generators
 def gen_rmat(scale: int, edge_factor: int = 16, abc=(0.57, 0.19, 0.19), seed: int = 1, scramble: bool = True,
-             dedup: bool = True):
-    """R-MAT CSR (row_ptr, col_idx) with 2^scale rows; self loops kept."""
+             dedup: bool = True, symmetric: bool = False):
+    """R-MAT CSR (row_ptr, col_idx) with 2^scale rows; self loops kept.
+    symmetric: every edge in both directions (as symmetric MatrixMarket files
+    of Graph500 R-MATs are read)."""
     m = edge_factor << scale
     src = np.empty(m, np.int32)
     dst = np.empty(m, np.int32)
     check(lib().mr_gen_rmat_edges(scale, edge_factor, abc[0], abc[1], abc[2], seed, int(scramble),
                                   ptr(src, C.c_int32), ptr(dst, C.c_int32)))
+    if symmetric:
+        src, dst = np.concatenate([src, dst]), np.concatenate([dst, src])
+        m = len(src)
     n = 1 << scale
     rp = np.empty(n + 1, np.int32)
     ci = np.empty(m, np.int32)
