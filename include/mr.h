@@ -186,6 +186,11 @@ int mr_reorder_rabbit(mr_mat m, int reference_scatter, int32_t* perm_out, mr_reo
 #define MR_RD_SAMPLED 1
 #define MR_MODEL_MERBIT 0
 #define MR_MODEL_CSR    1
+/* Models fitted on MI355X timings: the same method as the reference's
+ * (ridge/train.py:596-665), with bin edges at MI355X cache sizes.  They have
+ * no counterpart in the reference. */
+#define MR_MODEL_MERBIT_MI355X 2
+#define MR_MODEL_CSR_MI355X    3
 typedef struct {
     int32_t sector_size;     /* 32                                               */
     int32_t samples;         /* 30960                                            */
@@ -204,6 +209,10 @@ typedef struct {
 } mr_rd_stats;
 /* the fixed parameters of GRAPH::locality_predict (class.cu:1265-1277) */
 int mr_rd_params_default(mr_rd_params* p);
+/* the feature parameters a model was fitted with: the defaults above for the
+ * reference's models, MI355X bin edges (and rd_cap = last edge) for the
+ * MI355X models; no reference counterpart */
+int mr_rd_params_model(int model, mr_rd_params* p);
 /* exact sector RD histogram of a host access stream, compute_sector_rd_histogram
  * (class.cu:895-936); sectors sized by n as the reference does (:899).
  * counts / feat hold n_edges + 1 entries (either may be NULL). */
@@ -216,10 +225,13 @@ int mr_rd_features(mr_mat m, int mode, const mr_rd_params* p, double* feat, mr_r
 /* per-sample outputs of the sampled kernel (estimate, status, position) */
 int mr_rd_sampled_raw(mr_mat m, const mr_rd_params* p, float* est, int32_t* status, int32_t* pos);
 /* rd_merbit_model::PredictSpeedup (include/model.cuh:91-94) and the CSR model
- * of ridge/out/csr/sampled; MR_ERR_NUMERIC for non-finite input or result */
+ * of ridge/out/csr/sampled, or an MI355X model; feat holds the 10 features
+ * computed with mr_rd_params_model(model); MR_ERR_NUMERIC for non-finite
+ * input or result */
 int mr_predict(const double* feat, int model, double* speedup);
 /* GRAPH::locality_predict (class.cu:1263-1326): sampled RD features with the
- * default parameters, then the model; feat_out (10 doubles) may be NULL */
+ * model's parameters (mr_rd_params_model), then the model; feat_out (10
+ * doubles) may be NULL */
 int mr_locality_predict(mr_mat m, int model, double* speedup, double* feat_out, mr_rd_stats* stats);
 
 /* ---- SpMV / SpMM ---------------------------------------------------------- */

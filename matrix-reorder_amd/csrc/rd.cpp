@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "kernels/launch_rd.hpp"
+#include "model_mi355x.hpp"
 
 namespace mr {
 namespace {
@@ -217,14 +218,36 @@ int mr_rd_features(mr_mat m, int mode, const mr_rd_params* p, double* feat, mr_r
     return MR_OK;
 }
 
+int mr_rd_params_model(int model, mr_rd_params* p)
+{
+    MR_TRY(mr_rd_params_default(p));
+    switch (model) {
+    case MR_MODEL_MERBIT:
+    case MR_MODEL_CSR: return MR_OK;
+    case MR_MODEL_MERBIT_MI355X:
+    case MR_MODEL_CSR_MI355X:
+        p->n_edges = 9;
+        std::memcpy(p->bin_edges, mi355x::kEdges, sizeof(mi355x::kEdges));
+        p->rd_cap = mi355x::kRdCap;
+        return MR_OK;
+    default: return set_error(MR_ERR_INVALID_ARG, "unknown model %d", model);
+    }
+}
+
 int mr_predict(const double* feat, int model, double* speedup)
 {
     if (!feat || !speedup) return set_error(MR_ERR_INVALID_ARG, "null argument");
-    if (model != MR_MODEL_MERBIT && model != MR_MODEL_CSR) return set_error(MR_ERR_INVALID_ARG, "unknown model");
+    const double* w;
+    double v;
+    switch (model) {
+    case MR_MODEL_MERBIT: w = kMerbitW; v = kMerbitB; break;
+    case MR_MODEL_CSR: w = kCsrW; v = kCsrB; break;
+    case MR_MODEL_MERBIT_MI355X: w = mi355x::kMerbitW; v = mi355x::kMerbitB; break;
+    case MR_MODEL_CSR_MI355X: w = mi355x::kCsrW; v = mi355x::kCsrB; break;
+    default: return set_error(MR_ERR_INVALID_ARG, "unknown model");
+    }
     for (int i = 0; i < 10; ++i)
         if (!std::isfinite(feat[i])) return set_error(MR_ERR_NUMERIC, "feature %d is not finite", i);
-    const double* w = model == MR_MODEL_CSR ? kCsrW : kMerbitW;
-    double v = model == MR_MODEL_CSR ? kCsrB : kMerbitB;
     for (int i = 0; i < 10; ++i) v += w[i] * feat[i];
     const double sp = std::exp(v);
     if (!std::isfinite(sp) || sp <= 0.0) return set_error(MR_ERR_NUMERIC, "Predictor returned an invalid speedup");
@@ -236,7 +259,7 @@ int mr_locality_predict(mr_mat m, int model, double* speedup, double* feat_out, 
 {
     if (!m || !speedup) return set_error(MR_ERR_INVALID_ARG, "null argument");
     mr_rd_params p;
-    MR_TRY(mr_rd_params_default(&p));
+    MR_TRY(mr_rd_params_model(model, &p));
     double feat[10];
     MR_TRY(mr_rd_features(m, MR_RD_SAMPLED, &p, feat, stats));
     if (feat_out) std::memcpy(feat_out, feat, sizeof(feat));

@@ -23,7 +23,7 @@ F64, F32, F16, BF16 = 0, 1, 2, 3
 MERBIT, CSR_SCALAR, CSR_VECTOR = 0, 1, 2
 H2D, D2H, D2D = 0, 1, 2
 RD_EXACT, RD_SAMPLED = 0, 1
-MODEL_MERBIT, MODEL_CSR = 0, 1
+MODEL_MERBIT, MODEL_CSR, MODEL_MERBIT_MI355X, MODEL_CSR_MI355X = 0, 1, 2, 3
 
 _NP_OF = {F64: np.float64, F32: np.float32, F16: np.float16, BF16: np.uint16}
 STATUS = {
@@ -134,6 +134,7 @@ SIGNATURES = {
     "mr_reorder_rcm": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rabbit": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_rd_params_default": [C.POINTER(RDParams)],
+    "mr_rd_params_model": [C.c_int, C.POINTER(RDParams)],
     "mr_rd_exact_stream": [_I32P, C.c_int64, C.c_int64, C.POINTER(RDParams), _I64P, _DP, C.POINTER(RDStats)],
     "mr_rd_features": [_P, C.c_int, C.POINTER(RDParams), _DP, C.POINTER(RDStats)],
     "mr_rd_sampled_raw": [_P, C.POINTER(RDParams), _FP, _I32P, _I32P],
@@ -295,9 +296,14 @@ def predict(feat, model: int = MODEL_MERBIT) -> float:
     return out.value
 
 
-def rd_params(**kw) -> RDParams:
+def rd_params(model: int | None = None, **kw) -> RDParams:
+    """Feature parameters: the reference's defaults, or those a model was
+    fitted with; keyword arguments override fields."""
     p = RDParams()
-    check(lib().mr_rd_params_default(C.byref(p)))
+    if model is None:
+        check(lib().mr_rd_params_default(C.byref(p)))
+    else:
+        check(lib().mr_rd_params_model(model, C.byref(p)))
     for k, v in kw.items():
         if k == "bin_edges":
             p.n_edges = len(v)

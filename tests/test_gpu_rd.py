@@ -64,3 +64,18 @@ def test_exact_rd_and_predict(ctx, mr, oracle):
     assert np.array_equal(f10, ref["prob"])
     assert sp == oracle.predict(ref["prob"], 0)
     mat.close()
+
+
+@pytest.mark.parametrize("model", [2, 3])
+def test_locality_predict_mi355x_models(ctx, mr, oracle, model):
+    """The MI355X models compute their features with their own bin edges and
+    cap; the features equal the oracle's sampled RD under those parameters."""
+    rp, ci = rmat_csr(14, 16, seed=1)
+    mat = mr.Mat.from_csr(ctx, rp, ci)
+    p = mr.rd_params(model)
+    sp, f10, _ = mat.locality_predict(model)
+    edges = np.array([p.bin_edges[i] for i in range(p.n_edges)], np.int32)
+    ref = oracle.rd_sampled(ci, rd_cap=p.rd_cap, seed=oracle.hll_seed(0, 0), edges=edges)
+    assert np.array_equal(f10, ref["prob"])
+    assert sp == mr.predict(f10, model)
+    mat.close()
