@@ -67,6 +67,8 @@ const char* mr_version(void);
 int mr_device_count(int* count);
 /* replaces ENV (include/class.cuh:13-36); the device id is honoured */
 int mr_ctx_create(int device, mr_ctx* out);
+/* destroy every server, batch and matrix of the context (and free its
+ * buffers) first: they refer to it until they are destroyed */
 int mr_ctx_destroy(mr_ctx ctx);
 int mr_ctx_stream(mr_ctx ctx, mr_stream* out);
 int mr_ctx_mem_info(mr_ctx ctx, size_t* free_bytes, size_t* total_bytes);
@@ -180,6 +182,28 @@ int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats);
  * reference_scatter != 0 applies the ids the way the reference's call site
  * does (scattered as a position -> vertex sequence, i.e. the inverse map). */
 int mr_reorder_rabbit(mr_mat m, int reference_scatter, int32_t* perm_out, mr_reorder_stats* stats);
+
+/* ---- plan cache ------------------------------------------------------------
+ * Persisted pGOrder permutations and tuned MERBIT options, keyed by a 64-bit
+ * fingerprint of the matrix pattern (dimensions, row_ptr, col_idx), the width
+ * and the window.  No reference counterpart: the reference reorders from
+ * scratch every run (GRAPH::gorder6, class.cu:2955-3140). */
+/* fingerprint of a host CSR pattern; equals mr_mat_hash of the same matrix */
+int mr_csr_hash(int64_t n_rows, int64_t n_cols, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                uint64_t* hash);
+/* fingerprint of the matrix's current pattern, computed on the device */
+int mr_mat_hash(mr_mat m, uint64_t* hash);
+/* mr_reorder_pgorder through the cache in cache_dir (an existing directory):
+ * an entry for this pattern, width (0 = get_width) and window is applied with
+ * its saved MERBIT options; otherwise pGOrder runs and the entry is written.
+ * hit: 1 entry applied; 0 computed and stored; 2 computed, a stale or corrupt
+ * entry replaced; -1 computed, the entry could not be written */
+int mr_reorder_pgorder_cached(mr_mat m, int width, int window, const char* cache_dir, int32_t* perm_out,
+                              mr_reorder_stats* stats, int* hit);
+/* store the matrix's current MERBIT options (after mr_merbit_autotune) with
+ * the entry of its last cached reorder; MR_ERR_INVALID_ARG if it was not
+ * reordered through the cache or was relabelled since */
+int mr_plan_save(mr_mat m);
 
 /* ---- reuse distance and the locality predictor ---------------------------- */
 #define MR_RD_EXACT   0
@@ -377,6 +401,10 @@ int mr_serve_upload_rmat(mr_server s, int scale, int edge_factor, uint64_t seed,
 int mr_serve_upload_wait(mr_server s, int ticket, int* status, int* mat_id, double* ms);
 int mr_serve_submit(mr_server s, int mat_id, int k, uint64_t seed, int64_t* req);
 int mr_serve_cancel(mr_server s, int64_t req);   /* only while queued */
+/* reorder later uploads through the plan cache in dir (NULL or "" = off), and
+ * count the uploads that found their entry; no reference counterpart */
+int mr_serve_set_plan_cache(mr_server s, const char* dir);
+int mr_serve_plan_cache_stats(mr_server s, int64_t* hits, int64_t* misses);
 int mr_serve_wait(mr_server s, int64_t req, int* status, double* latency_us, uint64_t* hash);
 /* y[n_rows][k] of a completed request (verify = 2) */
 int mr_serve_result(mr_server s, int64_t req, float* y);

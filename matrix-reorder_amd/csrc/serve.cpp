@@ -144,6 +144,8 @@ struct mr_server_s {
     uint64_t stamp = 0;
     int64_t batches = 0, batch_jobs = 0, batch_reqs = 0, evictions = 0;
     int64_t uploaded_bytes = 0;
+    std::string plan_dir;                 // plan cache of the uploads ("" = off)
+    int64_t plan_hits = 0, plan_misses = 0;
     size_t table_bytes = 0;            // table region of an arena / staging buffer
     size_t hash_off = 0;               // hash region inside the table region
 
@@ -534,6 +536,7 @@ void upload_loop(mr_server s)
         if (s->up_tasks.empty()) break;
         const UploadTask task = s->up_tasks.front();
         s->up_tasks.pop_front();
+        const std::string plan_dir = s->plan_dir;
         lk.unlock();
         const auto t0 = Clock::now();
         UploadRes res;
@@ -592,7 +595,11 @@ void upload_loop(mr_server s)
         mr_mat mat = nullptr;
         if (rc == MR_OK)
             rc = mr_csr_create(s->ctx, n, n, nnz, rp.data(), ci.data(), val.data(), MR_F64, s->dtype, &mat);
-        if (rc == MR_OK) rc = mr_reorder_pgorder(mat, 0, 5, nullptr, nullptr);
+        int plan_hit = 0;
+        if (rc == MR_OK) {
+            if (plan_dir.empty()) rc = mr_reorder_pgorder(mat, 0, 5, nullptr, nullptr);
+            else rc = mr_reorder_pgorder_cached(mat, 0, 5, plan_dir.c_str(), nullptr, nullptr, &plan_hit);
+        }
         if (rc == MR_OK) {
             const int32_t* bx;
             const int32_t* by;
@@ -618,6 +625,7 @@ void upload_loop(mr_server s)
             s->uploaded_bytes += bytes;
             res.mat = (int)s->mats.size() - 1;
         }
+        if (rc == MR_OK && !plan_dir.empty()) ++(plan_hit == 1 ? s->plan_hits : s->plan_misses);
         res.status = rc;
         s->up_res[(size_t)task.ticket] = res;
         s->done_cv.notify_all();
@@ -826,6 +834,23 @@ int mr_serve_upload_wait(mr_server s, int ticket, int* status, int* mat_id, doub
     if (status) *status = r.status < 0 ? MR_ERR_CANCELLED : r.status;
     if (mat_id) *mat_id = r.mat;
     if (ms) *ms = r.ms;
+    return MR_OK;
+}
+
+int mr_serve_set_plan_cache(mr_server s, const char* dir)
+{
+    if (!s) return set_error(MR_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    s->plan_dir = dir ? dir : "";
+    return MR_OK;
+}
+
+int mr_serve_plan_cache_stats(mr_server s, int64_t* hits, int64_t* misses)
+{
+    if (!s) return set_error(MR_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (hits) *hits = s->plan_hits;
+    if (misses) *misses = s->plan_misses;
     return MR_OK;
 }
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import pathlib
+import weakref
 
 import numpy as np
 
@@ -133,6 +134,11 @@ SIGNATURES = {
     "mr_reorder_gorder_cpu": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rcm": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rabbit": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
+    "mr_csr_hash": [C.c_int64, C.c_int64, C.c_int64, _I32P, _I32P, C.POINTER(C.c_uint64)],
+    "mr_mat_hash": [_P, C.POINTER(C.c_uint64)],
+    "mr_reorder_pgorder_cached": [_P, C.c_int, C.c_int, C.c_char_p, _I32P, C.POINTER(ReorderStats),
+                                  C.POINTER(C.c_int)],
+    "mr_plan_save": [_P],
     "mr_rd_params_default": [C.POINTER(RDParams)],
     "mr_rd_params_model": [C.c_int, C.POINTER(RDParams)],
     "mr_rd_exact_stream": [_I32P, C.c_int64, C.c_int64, C.POINTER(RDParams), _I64P, _DP, C.POINTER(RDStats)],
@@ -169,6 +175,8 @@ SIGNATURES = {
     "mr_serve_upload_wait": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP],
     "mr_serve_submit": [_P, C.c_int, C.c_int, C.c_uint64, _I64P],
     "mr_serve_cancel": [_P, C.c_int64],
+    "mr_serve_set_plan_cache": [_P, C.c_char_p],
+    "mr_serve_plan_cache_stats": [_P, _I64P, _I64P],
     "mr_serve_wait": [_P, C.c_int64, C.POINTER(C.c_int), _DP, C.POINTER(C.c_uint64)],
     "mr_serve_result": [_P, C.c_int64, _FP],
     "mr_serve_run_load": [_P, C.POINTER(ServeLoad), C.POINTER(ServeReport)],
@@ -274,6 +282,18 @@ def gen_banded(n: int, bandwidth: int, nnz_per_row: int, seed: int):
     return rp, ci[: nnz.value]
 
 
+def csr_hash(rp: np.ndarray, ci: np.ndarray, n_cols: int | None = None) -> int:
+    """Fingerprint of a host CSR pattern (the plan-cache key); equals
+    ``Mat.hash()`` of the same matrix."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    n = len(rp) - 1
+    h = C.c_uint64(0)
+    check(lib().mr_csr_hash(n, n if n_cols is None else n_cols, len(ci), ptr(rp, C.c_int32), ptr(ci, C.c_int32),
+                            C.byref(h)))
+    return h.value
+
+
 def get_width(n: int, nnz: int) -> int:
     w = C.c_int(0)
     check(lib().mr_get_width(n, nnz, C.byref(w)))
@@ -338,6 +358,7 @@ class DeviceArray:
         p = C.c_void_p()
         check(lib().mr_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
         self.ptr = p
+        ctx._children.add(self)
 
     @classmethod
     def from_host(cls, ctx, a: np.ndarray):
@@ -361,6 +382,8 @@ class DeviceArray:
             lib().mr_free(self.ctx.h, self.ptr)
             self.ptr = None
 
+    close = free
+
     def __del__(self):
         try:
             self.free()
@@ -376,9 +399,16 @@ class Ctx:
         check(lib().mr_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+        self._children = weakref.WeakSet()
 
     def close(self):
+        """Destroy the context after every server, batch, matrix and buffer
+        still open on it (the library frees them against the context)."""
         if self.h:
+            for kind in (Server, Batch, Mat, DeviceArray):
+                for c in list(self._children):
+                    if isinstance(c, kind):
+                        c.close()
             lib().mr_ctx_destroy(self.h)
             self.h = None
 
@@ -438,6 +468,7 @@ class Mat:
     def __init__(self, ctx: Ctx, handle):
         self.ctx = ctx
         self.h = handle
+        ctx._children.add(self)
 
     @classmethod
     def from_csr(cls, ctx: Ctx, rp, ci, val=None, n_cols=None, store=F32):
@@ -524,6 +555,26 @@ class Mat:
         st = ReorderStats()
         check(lib().mr_pgorder_order(self.h, width, window, ptr(g, C.c_uint32), ptr(f, C.c_uint32), C.byref(st)))
         return g[:n], f[:n], st.as_dict()
+
+    def reorder_pgorder_cached(self, cache_dir: str, width: int = 0, window: int = 5):
+        """pGOrder through the on-disk plan cache; returns (perm, stats, hit)
+        with hit 1 = entry applied, 0 = computed and stored, 2 = a stale entry
+        replaced, -1 = computed but not stored."""
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        hit = C.c_int(0)
+        check(lib().mr_reorder_pgorder_cached(self.h, width, window, str(cache_dir).encode(),
+                                              ptr(perm, C.c_int32), C.byref(st), C.byref(hit)))
+        return perm[:n], st.as_dict(), hit.value
+
+    def plan_save(self):
+        check(lib().mr_plan_save(self.h))
+
+    def hash(self) -> int:
+        h = C.c_uint64(0)
+        check(lib().mr_mat_hash(self.h, C.byref(h)))
+        return h.value
 
     def reorder_gorder_cpu(self, window: int = 5):
         n = self.info()[0]
@@ -636,6 +687,7 @@ class Batch:
         h = _P()
         check(lib().mr_batch_create(ctx.h, len(self.mats), arr, k, C.byref(h)))
         self.h = h
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -710,6 +762,7 @@ class Server:
         check(lib().mr_serve_create(ctx.h, C.byref(c), C.byref(h)))
         self.h = h
         self._mats = []
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -745,6 +798,14 @@ class Server:
 
     def cancel(self, req: int):
         check(lib().mr_serve_cancel(self.h, req))
+
+    def set_plan_cache(self, cache_dir: str | None):
+        check(lib().mr_serve_set_plan_cache(self.h, (str(cache_dir) if cache_dir else "").encode()))
+
+    def plan_cache_stats(self):
+        h, m = C.c_int64(0), C.c_int64(0)
+        check(lib().mr_serve_plan_cache_stats(self.h, C.byref(h), C.byref(m)))
+        return h.value, m.value
 
     def wait(self, req: int):
         st, lat, h = C.c_int(), C.c_double(), C.c_uint64()
