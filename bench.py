@@ -367,6 +367,19 @@ def section_serving(mr, ctx, a, extra, batch_mats):
         log(f"  {rate:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, {r['gflops']:.0f} GFLOP/s, "
             f"{r['completed']} done, {r['uploads_admitted']} uploads ({r['upload_ms_mean']:.0f} ms each), "
             f"{r['mean_requests_per_batch']:.1f} req/batch")
+    # an upload of a matrix seen before: pGOrder replaced by the plan cache
+    import tempfile
+    with tempfile.TemporaryDirectory(prefix="mr_plans_") as plans:
+        srv.set_plan_cache(plans)
+        up = {}
+        for key in ("first_upload_ms", "repeat_upload_ms"):
+            st, mid, ms = srv.upload_wait(srv.upload_rmat(18, 16, seed=1818))
+            up[key] = round(ms, 1) if st == 0 else f"status {st}"
+        up["hits_misses"] = list(srv.plan_cache_stats())
+        srv.set_plan_cache(None)
+    res["plan_cache_s18_upload"] = up
+    log(f"  R-MAT s18 upload with the plan cache: first {up['first_upload_ms']} ms, repeat "
+        f"{up['repeat_upload_ms']} ms")
     srv.close()
     for m in resident:
         m.close()
