@@ -171,6 +171,15 @@ int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_re
  * phase (class.cu:3093); either may be NULL */
 int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
                      mr_reorder_stats* stats);
+/* GRAPH::gorder5 (class.cu:2727-2873), the earlier pGOrder schedule: batches
+ * of vsize (0 = get_width; at most 1024), start scores = column counts, the
+ * GPU window = the last width1 batches (width1 * vsize <= 1024) with no hub
+ * caps, then order_on_cpu with window width2 on every batch.  The reference
+ * calls it with (get_width, 1, 5) (main.cu:112-118).  Relabels. */
+int mr_reorder_gorder5(mr_mat m, int vsize, int width1, int width2, int32_t* perm_out, mr_reorder_stats* stats);
+/* gorder5's placements without relabelling, as mr_pgorder_order */
+int mr_gorder5_order(mr_mat m, int vsize, int width1, int width2, uint32_t* gpu_order, uint32_t* final_order,
+                     mr_reorder_stats* stats);
 /* sequential CPU Gorder = GRAPH::gorder1 (class.cu:1825-2013); relabels */
 int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_stats* stats);
 /* Reverse Cuthill-McKee = GRAPH::relabel_rcm (class.cu:1554-1602) over
@@ -182,6 +191,12 @@ int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats);
  * reference_scatter != 0 applies the ids the way the reference's call site
  * does (scattered as a position -> vertex sequence, i.e. the inverse map). */
 int mr_reorder_rabbit(mr_mat m, int reference_scatter, int32_t* perm_out, mr_reorder_stats* stats);
+/* degree order = GRAPH::relabel_degree (class.cu:1496-1549): vertices by
+ * column count of M descending, ties by id; relabels */
+int mr_reorder_degree(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats);
+/* degree-based grouping = GRAPH::dbg (class.cu:1683-1756): the key is
+ * ceil(log2(column count + 1)), descending, ties by id; relabels */
+int mr_reorder_dbg(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats);
 
 /* ---- plan cache ------------------------------------------------------------
  * Persisted pGOrder permutations and tuned MERBIT options, keyed by a 64-bit

@@ -23,7 +23,7 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
                     int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
-                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, hipStream_t s);
+                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap, hipStream_t s);
 // live[y] = full length of rows(y); then, from time to time, rows(y) keeps
 // only unplaced vertices (in place, first live[y] entries of the segment)
 hipError_t pg_live_init(int64_t n, const int32_t* tp, int32_t* live, hipStream_t s);

@@ -9,8 +9,10 @@
 // the |P(b)| unplaced vertices that come first by (count descending, id
 // ascending) -- the order the stable radix sort gives over identity ids -- then
 // every score is reset to its status bit and the window O(b) adds one per
-// 2-hop path x -> y -> z (y in cols(x), z in rows(y), hub caps 10000) to every
-// unplaced z.
+// 2-hop path x -> y -> z (y in cols(x), z in rows(y)) to every unplaced z.
+// gorder6 skips rows x and columns y of 10000 or more entries (hub_cap =
+// 10000); gorder5 (:2727-2873, kernel3 :2392-2420) has no cap (hub_cap =
+// INT32_MAX) and starts from column counts instead of row lengths.
 //
 // The reference's per-batch full sort and full reset are O(n) per batch
 // (quadratic overall).  After a reset only vertices touched by the window have
@@ -24,8 +26,9 @@
 //   |T| < |P(b)|      : the rest comes from untouched unplaced vertices in
 //                       ascending id order (count 0 ties), found by a
 //                       workgroup scan of per-1024-id unplaced counters.
-// Only T is reset between batches.  Batch 0 starts from count = row length
-// with T = all vertices with a non-empty row.  The placement is identical to
+// Only T is reset between batches.  Batch 0 starts from count = the start
+// lengths (row lengths for gorder6, column counts for gorder5) with T = all
+// vertices whose start length is non-zero.  The placement is identical to
 // the reference's for every input (the oracle restates the full-sort loop and
 // the tests compare the two bit for bit).
 #include "dtypes.hpp"
@@ -44,7 +47,6 @@ constexpr int kChunk = 1024;      // ids per unplaced counter
 constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
 constexpr int kWindowGrid = 2048; // blocks of the window-score kernel (8192 waves)
 constexpr int kSureCap = 4096;    // capacity of the candidates-above-threshold list
-constexpr int32_t kHubCap = 10000; // gorder6_kernel3's row / column length caps (class.cu:2930,2938)
 
 __device__ __forceinline__ int count_bin(uint32_t c)
 {
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
                                                     int32_t* __restrict__ hist, int32_t* __restrict__ chunk_left,
                                                     uint32_t* __restrict__ placement, int32_t ps, int32_t pl,
                                                     const int32_t* __restrict__ rp, int32_t os, int32_t ol,
-                                                    int32_t* __restrict__ wprefix)
+                                                    int32_t* __restrict__ wprefix, int32_t hub_cap)
 {
     __shared__ uint64_t keys[kSmallT];
     __shared__ int32_t sel[1024];
@@ -410,7 +412,7 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
             int32_t len = 0;
             if (xu < (uint32_t)n) {
                 len = rp[xu + 1] - rp[xu];
-                if (len >= kHubCap) len = 0;
+                if (len >= hub_cap) len = 0;
             }
             s_len[w] = len;
         }
@@ -449,7 +451,8 @@ __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t
                                                        const int32_t* __restrict__ wprefix,
                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                        const int32_t* __restrict__ tp, int32_t* __restrict__ h,
-                                                       int32_t* __restrict__ ylist, PgState* __restrict__ st)
+                                                       int32_t* __restrict__ ylist, PgState* __restrict__ st,
+                                                       int32_t hub_cap)
 {
     __shared__ int32_t s_pre[1025];
     for (int i = threadIdx.x; i <= ol; i += blockDim.x) s_pre[i] = wprefix[i];
@@ -470,7 +473,7 @@ __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t
             const uint32_t xu = window[lo];
             if (xu < (uint32_t)n) {   // every position of the window is placed
                 y = ci[rp[xu] + (int32_t)(p - s_pre[lo])];
-                if (tp[y + 1] - tp[y] < kHubCap) first = atomicAdd(&h[y], 1) == 0;
+                if (tp[y + 1] - tp[y] < hub_cap) first = atomicAdd(&h[y], 1) == 0;
             }
         }
         wave_append(first, y, ylist, &st->ycount);
@@ -559,17 +562,18 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
                     int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
-                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, hipStream_t s)
+                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap, hipStream_t s)
 {
     const int nxt = cur ^ 1;
     hist_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, hist);
     thresh_kernel<<<1, 256, 0, s>>>(hist, st, cur, pl);
     collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl);
     final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, hist, chunk_left, placement, ps, pl, rp,
-                                   os, ol, wprefix);
+                                   os, ol, wprefix, hub_cap);
     reset_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score);
     if (ol > 0) {
-        window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st);
+        window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st,
+                                                    hub_cap);
         window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, ylist, score, Tl[nxt], st, nxt);
     }
     return hipGetLastError();

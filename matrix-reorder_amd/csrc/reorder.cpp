@@ -14,6 +14,7 @@
 #include "bucket_heap.hpp"
 #include "common.hpp"
 #include "kernels/launch.hpp"
+#include "kernels/launch_degree.hpp"
 #include "kernels/launch_perm.hpp"
 #include "kernels/launch_pg.hpp"
 #include "parallel.hpp"
@@ -287,29 +288,26 @@ int mr_permute(mr_mat m, const int32_t* perm)
     return relabel(m, p, nullptr);
 }
 
-int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
-                     mr_reorder_stats* stats)
+// The Gorder-family placement shared by gorder6 (pGOrder) and gorder5: the
+// GPU loop over the batch schedule (Ps, Pl) with windows (Os, Ol), then
+// order_on_cpu on the host for every slice (Cs, Cl).
+struct GorderPlan {
+    std::vector<int32_t> Ps, Pl, Os, Ol;   // GPU batches and their windows
+    std::vector<int32_t> Cs, Cl;           // slices of the CPU phase
+    bool start_columns = false;            // start scores: column counts (else row lengths)
+    int32_t hub_cap = 10000;               // window rows / columns at or above it are skipped
+    int32_t cpu_window = 5;
+};
+
+static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_order, uint32_t* final_order,
+                               mr_reorder_stats* stats)
 {
-    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
-    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "pGOrder needs a square matrix");
-    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
-    const int64_t n64 = m->n_rows, nnz = m->nnz;
-    if (n64 == 0) return MR_OK;
-    int W = width;
-    if (W <= 0) MR_TRY(mr_get_width(n64, nnz, &W));
-    if (W < 32 || W > 1024 || (W & (W - 1)))
-        return set_error(MR_ERR_UNSUPPORTED, "pGOrder GPU loop supports power-of-two widths 32..1024 (got %d)", W);
     MR_HIP_TRY(hipSetDevice(m->ctx->device));
     const auto t_all = clk::now();
-    const int32_t n = (int32_t)n64;
+    const int64_t nnz = m->nnz;
+    const int32_t n = (int32_t)m->n_rows;
+    const int32_t B = (int32_t)P.Ps.size();
     hipStream_t s = m->ctx->stream;
-    std::vector<int32_t> Ps, Pl, Os, Ol;
-    const int32_t B = pg_schedule(n, W, Ps, Pl, Os, Ol);
-    // the reference never clips the first window (class.cu:3003); for n < 32 it
-    // would read past the placement, here it is clipped to n
-    if (Ol[0] > n) Ol[0] = n;
-    for (int32_t b = 0; b < B; ++b)
-        if (Os[(size_t)b] + Ol[(size_t)b] > n) Ol[(size_t)b] = n - Os[(size_t)b];
 
     // ---- GPU loop ----
     const auto t_gpu = clk::now();
@@ -342,7 +340,8 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
         MR_HIP_TRY(hipMemcpyAsync(ti_live.p, ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToDevice, s));
     MR_HIP_TRY(k::pg_live_init(n, tp.as<int32_t>(), live.as<int32_t>(), s));
     const int32_t compact_every = std::max<int32_t>(32, B / 32);
-    MR_HIP_TRY(k::pg_init(n, m->rp.as<int32_t>(), score.as<uint32_t>(), T0.as<int32_t>(), st, chunk.as<int32_t>(), s));
+    MR_HIP_TRY(k::pg_init(n, P.start_columns ? tp.as<int32_t>() : m->rp.as<int32_t>(), score.as<uint32_t>(),
+                          T0.as<int32_t>(), st, chunk.as<int32_t>(), s));
     int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
     int cur = 0;
     for (int32_t b = 0; b < B; ++b) {
@@ -353,7 +352,7 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
                                ti_live.as<int32_t>(), score.as<uint32_t>(), Tl, st, sure.as<int32_t>(),
                                tie.as<int32_t>(), hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(),
                                wprefix.as<int32_t>(), hcol.as<int32_t>(), ylist.as<int32_t>(), live.as<int32_t>(),
-                               Ps[(size_t)b], Pl[(size_t)b], Os[(size_t)b], Ol[(size_t)b], s));
+                               P.Ps[(size_t)b], P.Pl[(size_t)b], P.Os[(size_t)b], P.Ol[(size_t)b], P.hub_cap, s));
         cur ^= 1;
     }
     std::vector<uint32_t> order((size_t)n);
@@ -364,10 +363,10 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     {
         std::vector<int32_t> chk(order.begin(), order.end());
         if (mr_check_perm(chk.data(), n) != MR_OK)
-            return set_error(MR_ERR_NUMERIC, "pGOrder GPU loop produced an invalid placement");
+            return set_error(MR_ERR_NUMERIC, "Gorder GPU loop produced an invalid placement");
     }
 
-    // ---- CPU phase: order_on_cpu per batch from log2(W) ----
+    // ---- CPU phase: order_on_cpu per slice ----
     const auto t_cpu = clk::now();
     std::vector<int32_t> hrp((size_t)n + 1), hci((size_t)std::max<int64_t>(nnz, 1)), htp((size_t)n + 1),
         hti((size_t)std::max<int64_t>(nnz, 1));
@@ -378,32 +377,129 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
         MR_HIP_TRY(hipMemcpyAsync(hti.data(), ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
     }
     MR_HIP_TRY(hipStreamSynchronize(s));
-    const int32_t first = (int32_t)std::log2((double)W);
+    const int32_t C = (int32_t)P.Cs.size();
     std::vector<int32_t> local_idx((size_t)n, -1);
-    for (int32_t b = first; b < B; ++b)
-        for (int32_t i = 0; i < Ol[(size_t)b]; ++i) local_idx[(size_t)order[(size_t)(Os[(size_t)b] + i)]] = i;
-    PgCpuShared g{hrp.data(), hci.data(), htp.data(), hti.data(), local_idx.data(), window,
+    for (int32_t c = 0; c < C; ++c)
+        for (int32_t i = 0; i < P.Cl[(size_t)c]; ++i) local_idx[(size_t)order[(size_t)(P.Cs[(size_t)c] + i)]] = i;
+    PgCpuShared g{hrp.data(), hci.data(), htp.data(), hti.data(), local_idx.data(), P.cpu_window,
                   (int32_t)(std::sqrt((double)n) / 4)};
-    std::atomic<int32_t> next{first};
-    const int nth = std::max(1, std::min<int>(host_threads(), B - first));
+    std::atomic<int32_t> next{0};
+    const int nth = std::max(1, std::min<int>(host_threads(), C));
     std::vector<std::thread> pool;
-    for (int t = 0; t < nth && first < B; ++t)
+    for (int t = 0; t < nth && C > 0; ++t)
         pool.emplace_back([&] {
             PgCpuWorker w;
             w.yslot.assign((size_t)n, -1);
-            for (int32_t b = next++; b < B; b = next++)
-                order_batch(g, w, order.data() + Os[(size_t)b], Ol[(size_t)b]);
+            for (int32_t c = next++; c < C; c = next++)
+                order_batch(g, w, order.data() + P.Cs[(size_t)c], P.Cl[(size_t)c]);
         });
     for (auto& th : pool) th.join();
     const double cpu_ms = ms_since(t_cpu);
     if (final_order) std::memcpy(final_order, order.data(), sizeof(uint32_t) * (size_t)n);
     if (stats) {
-        stats->width = W;
-        stats->window = window;
         stats->batches = B;
         stats->gpu_ms = gpu_ms;
         stats->cpu_ms = cpu_ms;
         stats->total_ms = ms_since(t_all);
+    }
+    return MR_OK;
+}
+
+int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
+                     mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "pGOrder needs a square matrix");
+    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (n64 == 0) return MR_OK;
+    int W = width;
+    if (W <= 0) MR_TRY(mr_get_width(n64, nnz, &W));
+    if (W < 32 || W > 1024 || (W & (W - 1)))
+        return set_error(MR_ERR_UNSUPPORTED, "pGOrder GPU loop supports power-of-two widths 32..1024 (got %d)", W);
+    const int32_t n = (int32_t)n64;
+    GorderPlan P;
+    const int32_t B = pg_schedule(n, W, P.Ps, P.Pl, P.Os, P.Ol);
+    // the reference never clips the first window (class.cu:3003); for n < 32 it
+    // would read past the placement, here it is clipped to n
+    if (P.Ol[0] > n) P.Ol[0] = n;
+    for (int32_t b = 0; b < B; ++b)
+        if (P.Os[(size_t)b] + P.Ol[(size_t)b] > n) P.Ol[(size_t)b] = n - P.Os[(size_t)b];
+    // CPU phase: the windows of every batch from log2(W) (class.cu:3077-3093)
+    for (int32_t b = (int32_t)std::log2((double)W); b < B; ++b) {
+        P.Cs.push_back(P.Os[(size_t)b]);
+        P.Cl.push_back(P.Ol[(size_t)b]);
+    }
+    P.start_columns = false;
+    P.hub_cap = 10000;
+    P.cpu_window = window;
+    MR_TRY(gorder_family_order(m, P, gpu_order, final_order, stats));
+    if (stats) {
+        stats->width = W;
+        stats->window = window;
+    }
+    return MR_OK;
+}
+
+int mr_gorder5_order(mr_mat m, int vsize, int width1, int width2, uint32_t* gpu_order, uint32_t* final_order,
+                     mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "gorder5 needs a square matrix");
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (n64 == 0) return MR_OK;
+    if (vsize <= 0) MR_TRY(mr_get_width(n64, nnz, &vsize));
+    if (vsize < 1 || vsize > 1024)
+        return set_error(MR_ERR_UNSUPPORTED, "gorder5: batch size must be in [1, 1024] (got %d)", vsize);
+    if (width1 < 1 || (int64_t)width1 * vsize > 1024)
+        return set_error(MR_ERR_UNSUPPORTED, "gorder5: the GPU window (width1 x batch size) must hold 1..1024 vertices");
+    if (width2 < 1) return set_error(MR_ERR_INVALID_ARG, "gorder5: width2 must be >= 1");
+    const int32_t n = (int32_t)n64;
+    GorderPlan P;
+    // fixed batches and a window of the last width1 batches (class.cu:2800-2813)
+    const int32_t B = (n + vsize - 1) / vsize;
+    for (int32_t b = 0; b < B; ++b) {
+        const int32_t start = b * vsize;
+        const int32_t len = (start + vsize < n) ? vsize : n - start;
+        const int32_t os = (b < width1) ? 0 : (b - width1 + 1) * vsize;
+        int32_t ol = (b < width1) ? (b + 1) * vsize : width1 * vsize;
+        if (os + ol > n) ol = n - os;
+        P.Ps.push_back(start);
+        P.Pl.push_back(len);
+        P.Os.push_back(os);
+        P.Ol.push_back(ol);
+        P.Cs.push_back(start);     // CPU phase on every batch (class.cu:2833-2850)
+        P.Cl.push_back(len);
+    }
+    P.start_columns = true;
+    P.hub_cap = INT32_MAX;
+    P.cpu_window = width2;
+    MR_TRY(gorder_family_order(m, P, gpu_order, final_order, stats));
+    if (stats) {
+        stats->width = vsize;
+        stats->window = width2;
+    }
+    return MR_OK;
+}
+
+int mr_reorder_gorder5(mr_mat m, int vsize, int width1, int width2, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    const int64_t n = m->n_rows;
+    if (n == 0) return MR_OK;
+    const auto t0 = clk::now();
+    std::vector<uint32_t> order((size_t)n);
+    mr_reorder_stats st{};
+    MR_TRY(mr_gorder5_order(m, vsize, width1, width2, nullptr, order.data(), &st));
+    std::vector<int32_t> perm((size_t)n);
+    for (int64_t i = 0; i < n; ++i) perm[(size_t)order[(size_t)i]] = (int32_t)i;
+    double rl = 0;
+    MR_TRY(relabel(m, perm, &rl));
+    if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+    if (stats) {
+        *stats = st;
+        stats->relabel_ms = rl;
+        stats->total_ms = ms_since(t0);
     }
     return MR_OK;
 }
@@ -507,6 +603,52 @@ int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats)
         stats->total_ms = ms_since(t0);
     }
     return MR_OK;
+}
+
+// GRAPH::relabel_degree (src/class.cu:1496-1549) and GRAPH::dbg
+// (src/class.cu:1683-1756): column counts of M on the device, the order
+// (key desc, id asc) from one radix sort of composite keys, then the relabel
+static int degree_relabel(mr_mat m, int grouped, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "a symmetric relabel needs a square matrix");
+    const int64_t n = m->n_rows;
+    if (n == 0) return MR_OK;
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    const auto t0 = clk::now();
+    hipStream_t s = m->ctx->stream;
+    DevBuf cnt, dperm, ws;
+    const size_t ws_bytes = k::degree_order_ws_bytes(n);
+    MR_TRY(cnt.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(dperm.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(ws.alloc(ws_bytes));
+    MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
+    MR_HIP_TRY(k::transpose_count(m->nnz, m->ci.as<int32_t>(), cnt.as<int32_t>(), s));
+    MR_HIP_TRY(k::degree_order(n, cnt.as<int32_t>(), grouped, dperm.as<int32_t>(), ws.p, ws_bytes, s));
+    std::vector<int32_t> perm((size_t)n);
+    MR_HIP_TRY(hipMemcpyAsync(perm.data(), dperm.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    const double order_ms = ms_since(t0);
+    double rl = 0;
+    MR_TRY(relabel(m, perm, &rl));
+    if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->gpu_ms = order_ms;
+        stats->relabel_ms = rl;
+        stats->total_ms = ms_since(t0);
+    }
+    return MR_OK;
+}
+
+int mr_reorder_degree(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    return degree_relabel(m, 0, perm_out, stats);
+}
+
+int mr_reorder_dbg(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    return degree_relabel(m, 1, perm_out, stats);
 }
 
 int mr_reorder_rabbit(mr_mat m, int reference_scatter, int32_t* perm_out, mr_reorder_stats* stats)

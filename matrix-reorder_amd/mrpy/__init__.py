@@ -134,6 +134,10 @@ SIGNATURES = {
     "mr_reorder_gorder_cpu": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rcm": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rabbit": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
+    "mr_reorder_gorder5": [_P, C.c_int, C.c_int, C.c_int, _I32P, C.POINTER(ReorderStats)],
+    "mr_gorder5_order": [_P, C.c_int, C.c_int, C.c_int, _U32P, _U32P, C.POINTER(ReorderStats)],
+    "mr_reorder_degree": [_P, _I32P, C.POINTER(ReorderStats)],
+    "mr_reorder_dbg": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_csr_hash": [C.c_int64, C.c_int64, C.c_int64, _I32P, _I32P, C.POINTER(C.c_uint64)],
     "mr_mat_hash": [_P, C.POINTER(C.c_uint64)],
     "mr_reorder_pgorder_cached": [_P, C.c_int, C.c_int, C.c_char_p, _I32P, C.POINTER(ReorderStats),
@@ -556,6 +560,35 @@ class Mat:
         check(lib().mr_pgorder_order(self.h, width, window, ptr(g, C.c_uint32), ptr(f, C.c_uint32), C.byref(st)))
         return g[:n], f[:n], st.as_dict()
 
+    def reorder_gorder5(self, vsize: int = 0, width1: int = 1, width2: int = 5):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(lib().mr_reorder_gorder5(self.h, vsize, width1, width2, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
+    def gorder5_order(self, vsize: int = 0, width1: int = 1, width2: int = 5):
+        n = self.info()[0]
+        g = np.empty(max(n, 1), np.uint32)
+        f = np.empty(max(n, 1), np.uint32)
+        st = ReorderStats()
+        check(lib().mr_gorder5_order(self.h, vsize, width1, width2, ptr(g, C.c_uint32), ptr(f, C.c_uint32),
+                                     C.byref(st)))
+        return g[:n], f[:n], st.as_dict()
+
+    def reorder_degree(self):
+        return self._reorder_plain(lib().mr_reorder_degree)
+
+    def reorder_dbg(self):
+        return self._reorder_plain(lib().mr_reorder_dbg)
+
+    def _reorder_plain(self, fn):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(fn(self.h, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
     def reorder_pgorder_cached(self, cache_dir: str, width: int = 0, window: int = 5):
         """pGOrder through the on-disk plan cache; returns (perm, stats, hit)
         with hit 1 = entry applied, 0 = computed and stored, 2 = a stale entry
@@ -850,6 +883,15 @@ class Graph:
 
     def relabel_rcm(self):
         return self.mat.reorder_rcm()
+
+    def relabel_degree(self):
+        return self.mat.reorder_degree()
+
+    def gorder5(self, vsize: int = 0, width1: int = 1, width2: int = 5):
+        return self.mat.reorder_gorder5(vsize, width1, width2)
+
+    def dbg(self):
+        return self.mat.reorder_dbg()
 
     def locality_predict(self) -> float:
         return self.mat.locality_predict(MODEL_MERBIT)[0]

@@ -12,6 +12,8 @@
  *   oracle_pgorder           src/class.cu:2955-3140 (CPU phase from batch log2(W),
  *                            :3077-3093; perm scatter :3101-3102)
  *   oracle_gorder1           src/class.cu:1825-2013
+ *   oracle_gorder5           src/class.cu:2727-2873 (fixed batches, column-count
+ *                            start scores, no hub caps, CPU phase on every batch)
  *
  * Divergence kept on purpose: for n < 32 the reference's first window reads
  * d_permutation past n (Oleng[0] = 32 is never clipped, class.cu:3003); here the
@@ -333,4 +335,61 @@ int64_t oracle_gorder1_score(int32_t n, const int32_t* rp, const int32_t* ci,
         }
     }
     return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GRAPH::gorder5 (src/class.cu:2727-2873): fixed batches of vsize; the first
+ * scores are the column counts of M (gorder_gpu_cpu_kernel1 over offset1,
+ * :2789); each batch places the first len vertices by (score desc, id asc)
+ * (gorder_gpu_cpu_kernel2, :2371-2389), then counts 2-hop paths from the
+ * window of the last width1 batches with no hub caps (gorder_gpu_cpu_kernel3,
+ * :2392-2420); the CPU phase runs order_on_cpu with width2 on every batch
+ * (:2833-2850). */
+int oracle_gorder5(int32_t n, const int32_t* rp, const int32_t* ci, int32_t vsize, int32_t width1,
+                   int32_t width2, uint32_t* placement_gpu, uint32_t* placement_final, int32_t* perm)
+{
+    if (n <= 0) return 0;
+    int32_t* tp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* ti = (int32_t*)malloc(sizeof(int32_t) * ((size_t)rp[n] + 1));
+    oracle_transpose(n, n, rp, ci, tp, ti);
+    const int32_t B = (n + vsize - 1) / vsize;
+    uint32_t* score = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    skey* keys = (skey*)malloc(sizeof(skey) * (size_t)n);
+    for (int32_t v = 0; v < n; ++v) score[v] = (uint32_t)(tp[v + 1] - tp[v]) | 0x80000000u;
+    for (int32_t b = 0; b < B; ++b) {
+        const int32_t start = b * vsize;
+        const int32_t len = (start + vsize < n) ? vsize : n - start;
+        const int32_t os = (b < width1) ? 0 : (b - width1 + 1) * vsize;
+        int32_t ol = (b < width1) ? (b + 1) * vsize : width1 * vsize;
+        if (os + ol > n) ol = n - os;
+        for (int32_t v = 0; v < n; ++v) { keys[v].score = score[v]; keys[v].id = (uint32_t)v; }
+        qsort(keys, (size_t)n, sizeof(skey), cmp_desc_stable);
+        for (int32_t v = 0; v < n; ++v) score[v] &= 0x80000000u;
+        for (int32_t i = 0; i < len; ++i) {
+            score[keys[i].id] = 0u;
+            placement_gpu[start + i] = keys[i].id;
+        }
+        for (int32_t w = 0; w < ol; ++w) {
+            const int32_t x = (int32_t)placement_gpu[os + w];
+            for (int32_t a = rp[x]; a < rp[x + 1]; ++a) {
+                const int32_t y = ci[a];
+                for (int32_t c = tp[y]; c < tp[y + 1]; ++c) {
+                    const int32_t z = ti[c];
+                    if (score[z] >> 31) score[z] += 1u;
+                }
+            }
+        }
+    }
+    memcpy(placement_final, placement_gpu, sizeof(uint32_t) * (size_t)n);
+    int32_t* batch_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    for (int32_t v = 0; v < n; ++v) batch_of[v] = -1;
+    const int32_t L = (int32_t)(sqrt((double)n) / 4);
+    for (int32_t b = 0; b < B; ++b) {
+        const int32_t start = b * vsize;
+        const int32_t len = (start + vsize < n) ? vsize : n - start;
+        oracle_order_on_cpu(n, rp, ci, tp, ti, batch_of, &placement_final[start], b, len, width2, L);
+    }
+    for (int32_t i = 0; i < n; ++i) perm[placement_final[i]] = i;
+    free(tp); free(ti); free(score); free(keys); free(batch_of);
+    return B;
 }

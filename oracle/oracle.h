@@ -54,6 +54,12 @@ int32_t oracle_get_width(int64_t n, int64_t nnz);                 /* class.cu:73
 void oracle_transpose(int32_t n_rows, int32_t n_cols, const int32_t* rp,
                       const int32_t* ci, int32_t* tp, int32_t* ti);
 int  oracle_check_perm(const int32_t* perm, int64_t n);            /* utils.cuh:5-55 */
+/* GRAPH::gorder5 (class.cu:2727-2873); returns the batch count */
+int oracle_gorder5(int32_t n, const int32_t* rp, const int32_t* ci, int32_t vsize, int32_t width1,
+                   int32_t width2, uint32_t* placement_gpu, uint32_t* placement_final, int32_t* perm);
+/* degree order and DBG (class.cu:1496-1549, 1683-1756): perm[old] = new */
+void oracle_order_degree(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* perm);
+void oracle_order_dbg(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* perm);
 /* relabel: perm[old]=new; A' = P A P^T, columns sorted (class.cu:3099-3116 + 423-457) */
 void oracle_permute(int32_t n, const int32_t* rp, const int32_t* ci, const double* val,
                     const int32_t* perm, int32_t* rp2, int32_t* ci2, double* val2);

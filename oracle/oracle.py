@@ -29,6 +29,9 @@ _SIG = {
     "oracle_transpose": (None, [C.c_int32, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "oracle_check_perm": (C.c_int, [_I32P, C.c_int64]),
     "oracle_permute": (None, [C.c_int32, _I32P, _I32P, _DP, _I32P, _I32P, _I32P, _DP]),
+    "oracle_order_degree": (None, [C.c_int32, _I32P, _I32P, _I32P]),
+    "oracle_gorder5": (C.c_int, [C.c_int32, _I32P, _I32P, C.c_int32, C.c_int32, C.c_int32, _U32P, _U32P, _I32P]),
+    "oracle_order_dbg": (None, [C.c_int32, _I32P, _I32P, _I32P]),
     "oracle_pgorder_schedule": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "oracle_pgorder_gpu_loop": (C.c_int, [C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32, _U32P]),
     "oracle_order_on_cpu": (None, [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, _U32P, C.c_int32, C.c_int32,
@@ -113,6 +116,26 @@ def get_width(n, nnz):
 def check_perm(perm):
     perm = _i32(perm)
     return lib().oracle_check_perm(_p(perm, C.c_int32), len(perm)) == 0
+
+
+def gorder5(n, rp, ci, vsize, width1=1, width2=5):
+    """(gpu placement, final placement, perm) of GRAPH::gorder5."""
+    rp, ci = _i32(rp), _i32(ci if len(ci) else np.zeros(1, np.int32))
+    g = np.zeros(max(n, 1), np.uint32)
+    f = np.zeros(max(n, 1), np.uint32)
+    perm = np.zeros(max(n, 1), np.int32)
+    lib().oracle_gorder5(n, _p(rp, C.c_int32), _p(ci, C.c_int32), vsize, width1, width2, _p(g, C.c_uint32),
+                         _p(f, C.c_uint32), _p(perm, C.c_int32))
+    return g[:n], f[:n], perm[:n]
+
+
+def order_degree(n, rp, ci, grouped=False):
+    """perm[old] = new of the degree order (grouped=False) or DBG (True)."""
+    rp, ci = _i32(rp), _i32(ci if len(ci) else np.zeros(1, np.int32))
+    perm = np.zeros(max(n, 1), np.int32)
+    fn = lib().oracle_order_dbg if grouped else lib().oracle_order_degree
+    fn(n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(perm, C.c_int32))
+    return perm[:n]
 
 
 def permute(n, rp, ci, val, perm):

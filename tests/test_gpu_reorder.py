@@ -174,3 +174,73 @@ def test_rabbit_matches_oracle(ctx, mr, oracle, case):
     inv[ids] = np.arange(n, dtype=np.int32)
     assert np.array_equal(perm2, inv)
     mat.close()
+
+
+def degree_cases():
+    rng = np.random.default_rng(81)
+    out = [("rmat_s14", *rmat_csr(14, 16, seed=1)), ("rmat_s17_hubs", *rmat_csr(17, 16, seed=17))]
+    rp, ci = random_csr(rng, 3000, empty_frac=0.3, max_row=40)
+    out.append(("random_3000", rp, ci))
+    out.append(("no_edges_700", np.zeros(701, np.int32), np.zeros(0, np.int32)))
+    rp = np.array([0, 2, 3, 6, 6, 8], np.int32)
+    out.append(("hand_5", rp, np.array([1, 2, 2, 2, 3, 4, 0, 2], np.int32)))
+    return out
+
+
+@pytest.mark.parametrize("grouped", [False, True], ids=["degree", "dbg"])
+@pytest.mark.parametrize("case", degree_cases(), ids=lambda c: c[0])
+def test_degree_relabels_bit_exact(ctx, mr, oracle, case, grouped):
+    name, rp, ci = case
+    n = len(rp) - 1
+    want = oracle.order_degree(n, rp, ci, grouped)
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    perm, st = mat.reorder_dbg() if grouped else mat.reorder_degree()
+    assert np.array_equal(perm, want)
+    grp, gci, _ = mat.export()
+    orp, oci, _ = oracle.permute(n, rp, ci, None, want)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci)
+    mat.close()
+
+
+def gorder5_cases():
+    rng = np.random.default_rng(91)
+    out = [("rmat_s14_w1024", *rmat_csr(14, 16, seed=1), 0, 1, 5)]
+    rp, ci = random_csr(rng, 2000, empty_frac=0.3, max_row=12)
+    out.append(("random_2000_v64_w2", rp, ci, 64, 2, 5))
+    rp, ci = random_csr(rng, 700, empty_frac=0.1, max_row=30)
+    out.append(("random_700_v10_w3", rp, ci, 10, 3, 3))
+    out.append(("rmat_s12_v128", *rmat_csr(12, 8, seed=4), 128, 1, 5))
+    out.append(("no_edges_3000", np.zeros(3001, np.int32), np.zeros(0, np.int32), 1024, 1, 5))
+    # hubs above gorder6's 10000 cap: gorder5 counts them
+    n = 30000
+    rows = [np.sort(rng.choice(n, 12000, replace=False)) if i in (5, 9000) else
+            np.sort(rng.choice(n, int(rng.integers(0, 6)), replace=False)) for i in range(n)]
+    out.append(("hubs_30000", *csr_rows(rows, n), 1024, 1, 5))
+    out.append(("rmat_s13_v32_compacted", *rmat_csr(13, 16, seed=13), 32, 1, 5))
+    return out
+
+
+@pytest.mark.parametrize("case", gorder5_cases(), ids=lambda c: c[0])
+def test_gorder5_bit_exact(ctx, mr, oracle, case):
+    name, rp, ci, vsize, w1, w2 = case
+    n = len(rp) - 1
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    g, f, st = mat.gorder5_order(vsize, w1, w2)
+    og, of, operm = oracle.gorder5(n, rp, ci, vsize or mr.get_width(n, len(ci)), w1, w2)
+    assert np.array_equal(g, og), "GPU placement loop differs"
+    assert np.array_equal(f, of), "CPU phase differs"
+    perm, _ = mat.reorder_gorder5(vsize, w1, w2)
+    assert np.array_equal(perm, operm)
+    grp, gci, _ = mat.export()
+    orp, oci, _ = oracle.permute(n, rp, ci, None, operm)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci)
+    mat.close()
+
+
+def test_gorder5_argument_limits(ctx, mr):
+    rp, ci = rmat_csr(10, 8, seed=1)
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    for args in ((2048, 1, 5), (512, 3, 5), (64, 0, 5), (64, 1, 0)):
+        with pytest.raises(mr.MRError):
+            mat.reorder_gorder5(*args)
+    mat.close()

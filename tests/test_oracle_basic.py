@@ -154,3 +154,44 @@ def test_rcm_matches_reference(oracle):
     for g in json.loads((GOLDEN / "rcm_ref.json").read_text()):
         out = oracle.rcm(g["n"], np.array(g["rows"], np.int32), np.array(g["cols"], np.int32))
         assert out.tolist() == g["perm"]
+
+
+def _csr_with_column_counts(counts):
+    """n x n pattern whose column j holds counts[j] entries (rows 0..counts[j]-1)."""
+    n = len(counts)
+    rows = [[j for j in range(n) if counts[j] > i] for i in range(n)]
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.array([c for r in rows for c in r], np.int32)
+    return rp, ci
+
+
+def test_degree_orders_hand_worked(oracle):
+    # rows 0..4: [1,2], [2], [2,3,4], [], [0,2]; column counts 1,1,4,1,1 differ
+    # from row counts 2,1,3,0,2 -- the reference counts row_ind, the columns of M
+    rp = np.array([0, 2, 3, 6, 6, 8], np.int32)
+    ci = np.array([1, 2, 2, 2, 3, 4, 0, 2], np.int32)
+    assert oracle.order_degree(5, rp, ci).tolist() == [1, 2, 0, 3, 4]
+    # column counts 2,3,0,4,1,6,7: degree order 6,5,3,1,0,4,2; DBG groups
+    # ceil(log2(c+1)) = 2,2,0,3,1,3,3 -> 3,5,6,0,1,4,2
+    rp, ci = _csr_with_column_counts([2, 3, 0, 4, 1, 6, 7])
+    assert oracle.order_degree(7, rp, ci).tolist() == [4, 3, 6, 2, 5, 1, 0]
+    assert oracle.order_degree(7, rp, ci, grouped=True).tolist() == [3, 4, 6, 0, 5, 1, 2]
+
+
+def test_degree_orders_properties(oracle):
+    """Along the new order the key never increases and equal keys keep id
+    order; the DBG key is the bit length of the count (= ceil(log2(c+1)))."""
+    rng = np.random.default_rng(12)
+    rp, ci = random_csr(rng, 3000, empty_frac=0.2, max_row=60)
+    n = len(rp) - 1
+    cnt = np.bincount(ci, minlength=n)
+    for grouped, key in ((False, cnt), (True, np.array([int(c).bit_length() for c in cnt]))):
+        perm = oracle.order_degree(n, rp, ci, grouped)
+        assert np.array_equal(np.sort(perm), np.arange(n))
+        order = np.empty(n, np.int64)
+        order[perm] = np.arange(n)
+        k = key[order]
+        assert np.all(np.diff(k) <= 0)
+        same = np.diff(k) == 0
+        assert np.all(np.diff(order)[same] > 0)

@@ -159,3 +159,53 @@ def test_pgorder_cpu_phase_leaves_prefix(oracle):
         s, l = os_[b], ol[b]
         assert sorted(f[s:s + l].tolist()) == sorted(g[s:s + l].tolist())
     assert oracle.check_perm(perm) and np.array_equal(perm[f], np.arange(n))
+
+
+def brute_gorder5_gpu(n, A, vsize, width1):
+    """gorder5's GPU loop in dense algebra: start scores are column counts,
+    batches of vsize, window = the last width1 batches, no hub caps."""
+    placed = np.zeros(n, bool)
+    placement = np.full(n, -1, np.int64)
+    score = A.sum(0).astype(np.int64)
+    B = (n + vsize - 1) // vsize
+    for b in range(B):
+        cand = np.flatnonzero(~placed)
+        order = cand[np.lexsort((cand, -score[cand]))]
+        start = b * vsize
+        take = order[: min(vsize, n - start)]
+        placement[start: start + len(take)] = take
+        placed[take] = True
+        os_ = 0 if b < width1 else (b - width1 + 1) * vsize
+        ol = min((b + 1) * vsize if b < width1 else width1 * vsize, n - os_)
+        w = np.zeros(n, np.int64)
+        w[placement[os_: os_ + ol]] = 1
+        score = (w @ A) @ A.T
+        score[placed] = 0
+    return placement
+
+
+@pytest.mark.parametrize("seed,n,vsize,width1", [(11, 40, 8, 1), (12, 150, 32, 1), (13, 300, 16, 3), (14, 97, 10, 2),
+                                                 (15, 5, 8, 1)])
+def test_gorder5_gpu_loop_equals_dense_brute_force(oracle, seed, n, vsize, width1):
+    rng = np.random.default_rng(seed)
+    rp, ci = random_csr(rng, n, max_row=6, empty_frac=0.3)
+    g, f, perm = oracle.gorder5(n, rp, ci, vsize, width1, 5)
+    assert np.array_equal(g.astype(np.int64), brute_gorder5_gpu(n, dense(n, rp, ci), vsize, width1))
+    # the CPU phase permutes inside every batch and nothing across batches
+    for b in range(0, n, vsize):
+        assert sorted(f[b:b + vsize].tolist()) == sorted(g[b:b + vsize].tolist())
+    assert oracle.check_perm(perm) and np.array_equal(perm[f.astype(np.int64)], np.arange(n))
+
+
+def test_gorder5_start_scores_are_column_counts(oracle):
+    # rows 0..3 of M: [3], [3], [3], [0, 1, 2]: row 3 is the longest row but
+    # column 3 has the most entries, so the first placed vertex is 3 only
+    # because of the column count; then 0, 1, 2 by id (one entry each)
+    rp = np.array([0, 1, 2, 3, 6], np.int32)
+    ci = np.array([3, 3, 3, 0, 1, 2], np.int32)
+    g, _, _ = oracle.gorder5(4, rp, ci, 1, 1, 5)
+    assert g[0] == 3
+    rp = np.array([0, 3, 4, 5, 6], np.int32)     # row 0 longest, column counts 1,2,2,1
+    ci = np.array([1, 2, 3, 1, 2, 0], np.int32)
+    g, _, _ = oracle.gorder5(4, rp, ci, 1, 1, 5)
+    assert g[0] == 1
