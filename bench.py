@@ -24,8 +24,9 @@ Further sections (reported under "extra", each on its own synthetic inputs):
              R-MAT s24 ef4, all-ones values and x), unreordered and after
              pGOrder, beside the published sm_89 figures;
   cpu        the CPU baseline: the oracle's threaded CSR SpMV (FP32) on the
-             same matrix, plus the host reorderers (sequential Gorder, RCM,
-             Rabbit Order) beside pGOrder on R-MAT s18, each with the SpMV
+             same matrix, plus every reorderer on R-MAT s18 -- host
+             (sequential Gorder, RCM, Rabbit Order, tile-aware order) and
+             device (degree, DBG, gorder5, pGOrder) -- each with the SpMV
              time it yields.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
@@ -438,8 +439,9 @@ def section_cpu(mr, ctx, a, data):
 
 
 def section_cpu_reorders(mr, ctx, extra):
-    """Host reorderers (sequential Gorder, RCM, Rabbit Order) beside the GPU
-    pGOrder on R-MAT s18, each followed by the MERBIT SpMV (FP32) it yields."""
+    """Every reorderer on R-MAT s18, each followed by the MERBIT SpMV (FP32) it
+    yields: the host ones (sequential Gorder, RCM, Rabbit Order, the
+    tile-aware order) and the device ones (degree, DBG, gorder5, pGOrder)."""
     rp, ci = mr.gen_rmat(18, 16, (0.57, 0.19, 0.19), seed=18)
     n, nnz = len(rp) - 1, len(ci)
     val = mr.gen_values(18, nnz).astype(np.float32)
@@ -449,6 +451,9 @@ def section_cpu_reorders(mr, ctx, extra):
     yd = ctx.empty(n, np.float32)
     for name, fn in (("none", None), ("gorder_cpu_w5", lambda m: m.reorder_gorder_cpu(5)),
                      ("rcm", lambda m: m.reorder_rcm()), ("rabbit", lambda m: m.reorder_rabbit(False)),
+                     ("tile_order_1024_512", lambda m: m.reorder_tile_order(1024, 512)),
+                     ("degree", lambda m: m.reorder_degree()), ("dbg", lambda m: m.reorder_dbg()),
+                     ("gorder5", lambda m: m.reorder_gorder5(0, 1, 5)),
                      ("pgorder", lambda m: m.reorder_pgorder(0, 5))):
         m = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
         res = {}

@@ -180,6 +180,11 @@ int mr_reorder_gorder5(mr_mat m, int vsize, int width1, int width2, int32_t* per
 /* gorder5's placements without relabelling, as mr_pgorder_order */
 int mr_gorder5_order(mr_mat m, int vsize, int width1, int width2, uint32_t* gpu_order, uint32_t* final_order,
                      mr_reorder_stats* stats);
+/* tile-aware greedy order = GRAPH::order (class.cu:3158-3365), sequential on
+ * the host: rows sharing x entries are kept inside tiles of width1 x entries
+ * (the reference names block x sigma), columns with width2 or more entries are
+ * ignored; main.cu:129-133 takes both from the command line.  Relabels. */
+int mr_reorder_tile_order(mr_mat m, int width1, int width2, int32_t* perm_out, mr_reorder_stats* stats);
 /* sequential CPU Gorder = GRAPH::gorder1 (class.cu:1825-2013); relabels */
 int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_stats* stats);
 /* Reverse Cuthill-McKee = GRAPH::relabel_rcm (class.cu:1554-1602) over

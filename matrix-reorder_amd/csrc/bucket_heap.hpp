@@ -46,6 +46,20 @@ public:
             if (key1_[(size_t)v] > pmax_) pmax_ = key1_[(size_t)v];
         }
     }
+    // Integer_Heap::refresh (integer_heap.cuh:112-120): a node with a positive
+    // key goes back to key 0 (plus its pending delta), entering that bucket
+    // after its head
+    void refresh(int32_t v)
+    {
+        if (key1_[(size_t)v] > 0) {
+            remove(v);
+            key1_[(size_t)v] = 0;
+            insert(v);
+        }
+    }
+    // the caller's direct write of PMAX (class.cu:3310)
+    void set_pmax(int32_t p) { pmax_ = p; }
+
     int32_t pop()
     {
         while (cnt_ > 0) {

@@ -1,6 +1,7 @@
 // cpu_order.cpp -- the host-side reorder baselines of the reference:
-// sequential Gorder (GRAPH::gorder1, src/class.cu:1825-2013) and Reverse
-// Cuthill-McKee (rcm/src/rcm.cpp:1-154, rcm/src/queue.cpp:39-55).
+// sequential Gorder (GRAPH::gorder1, src/class.cu:1825-2013), the tile-aware
+// order (GRAPH::order, src/class.cu:3158-3365) and Reverse Cuthill-McKee
+// (rcm/src/rcm.cpp:1-154, rcm/src/queue.cpp:39-55).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -43,6 +44,63 @@ void gorder_cpu(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* 
         order[id] = (uint32_t)node;
         apply(node, 1);
         if (id >= window) apply((int32_t)order[id - window], -1);
+    }
+}
+
+// GRAPH::order (src/class.cu:3158-3365): greedy order that keeps rows sharing
+// x entries inside one tile of width1 x entries.  curr counts the x entries
+// laid out so far and [start, start + width1) is the current tile.  A placed
+// row whose entries fit raises every unplaced row sharing one of its columns
+// (columns of width2 or more entries skipped); a row that crosses the tile
+// boundary first drops every row the old tile raised back to key 0
+// (refresh), resets the top bucket, moves the tile to the one holding its
+// last entry and raises rows only for its entries inside it.
+void tile_order(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
+                int32_t width1, int32_t width2, uint32_t* order)
+{
+    int32_t first = 0;
+    for (int32_t v = 1; v < n; ++v)
+        if (rp[v + 1] - rp[v] > rp[first + 1] - rp[first]) first = v;
+    BucketHeap heap;
+    heap.reset(n);
+    std::vector<uint8_t> sts((size_t)n, 1);
+    std::vector<int32_t> tile((size_t)width1);
+    for (int32_t v = 0; v < n; ++v) heap.push(v);
+    heap.modify(first, 1);
+    auto raise = [&](int32_t col) {
+        if (tp[col + 1] - tp[col] >= width2) return;
+        for (int32_t j = tp[col]; j < tp[col + 1]; ++j)
+            if (sts[(size_t)ti[j]]) heap.modify(ti[j], 1);
+    };
+    int64_t start = 0, curr = 0;
+    for (int32_t id = 0; id < n; ++id) {
+        const int32_t node = heap.pop();
+        sts[(size_t)node] = 0;
+        order[id] = (uint32_t)node;
+        const int32_t deg = rp[node + 1] - rp[node];
+        if (curr - start + deg < width1) {
+            for (int32_t i = 0; i < deg; ++i) {
+                const int32_t col = ci[rp[node] + i];
+                tile[(size_t)(curr - start)] = col;
+                raise(col);
+                ++curr;
+            }
+            continue;
+        }
+        for (int64_t i = start; i < curr; ++i) {
+            const int32_t col = tile[(size_t)(i - start)];
+            if (tp[col + 1] - tp[col] >= width2) continue;
+            for (int32_t j = tp[col]; j < tp[col + 1]; ++j)
+                if (sts[(size_t)ti[j]]) heap.refresh(ti[j]);
+        }
+        heap.set_pmax(0);
+        start = (curr + deg) / width1 * width1;
+        for (int32_t i = 0; i < deg; ++i, ++curr) {
+            if (curr < start) continue;
+            const int32_t col = ci[rp[node] + i];
+            tile[(size_t)(curr - start)] = col;
+            raise(col);
+        }
     }
 }
 

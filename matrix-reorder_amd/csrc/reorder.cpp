@@ -526,16 +526,18 @@ int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_re
     return MR_OK;
 }
 
-int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_stats* stats)
+}  // extern "C"
+
+// a sequential host order over the CSR and CSC of the matrix (the CSC is
+// built on the device), applied with the device relabel
+template <typename F>
+static int host_order_relabel(mr_mat m, F&& fn, int32_t width, int32_t window, int32_t* perm_out,
+                              mr_reorder_stats* stats)
 {
-    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
-    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "Gorder needs a square matrix");
-    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
-    const int64_t n64 = m->n_rows, nnz = m->nnz;
-    if (n64 == 0) return MR_OK;
+    const int64_t nnz = m->nnz;
     MR_HIP_TRY(hipSetDevice(m->ctx->device));
     const auto t0 = clk::now();
-    const int32_t n = (int32_t)n64;
+    const int32_t n = (int32_t)m->n_rows;
     hipStream_t s = m->ctx->stream;
     DevBuf tp, ti;
     MR_TRY(gpu_transpose(m, tp, ti, s));
@@ -549,7 +551,7 @@ int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_st
     }
     MR_HIP_TRY(hipStreamSynchronize(s));
     std::vector<uint32_t> order((size_t)n);
-    gorder_cpu(n, rp.data(), ci.data(), tpp.data(), tii.data(), window, order.data());
+    fn(n, rp.data(), ci.data(), tpp.data(), tii.data(), order.data());
     std::vector<int32_t> perm((size_t)n);
     for (int32_t i = 0; i < n; ++i) perm[(size_t)order[(size_t)i]] = i;
     const double order_ms = ms_since(t0);
@@ -557,7 +559,7 @@ int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_st
     MR_TRY(relabel(m, perm, &rl));
     if (perm_out) std::memcpy(perm_out, perm.data(), sizeof(int32_t) * (size_t)n);
     if (stats) {
-        stats->width = 0;
+        stats->width = width;
         stats->window = window;
         stats->batches = 1;
         stats->gpu_ms = 0;
@@ -566,6 +568,36 @@ int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_st
         stats->total_ms = ms_since(t0);
     }
     return MR_OK;
+}
+
+extern "C" {
+
+int mr_reorder_gorder_cpu(mr_mat m, int window, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "Gorder needs a square matrix");
+    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
+    if (m->n_rows == 0) return MR_OK;
+    return host_order_relabel(
+        m,
+        [&](int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti, uint32_t* order) {
+            gorder_cpu(n, rp, ci, tp, ti, window, order);
+        },
+        0, window, perm_out, stats);
+}
+
+int mr_reorder_tile_order(mr_mat m, int width1, int width2, int32_t* perm_out, mr_reorder_stats* stats)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "the tile order needs a square matrix");
+    if (width1 < 1 || width2 < 1) return set_error(MR_ERR_INVALID_ARG, "tile order: widths must be >= 1");
+    if (m->n_rows == 0) return MR_OK;
+    return host_order_relabel(
+        m,
+        [&](int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti, uint32_t* order) {
+            tile_order(n, rp, ci, tp, ti, width1, width2, order);
+        },
+        width1, width2, perm_out, stats);
 }
 
 int mr_reorder_rcm(mr_mat m, int32_t* perm_out, mr_reorder_stats* stats)

@@ -17,6 +17,9 @@ int32_t pg_schedule(int32_t n, int32_t W, std::vector<int32_t>& Ps, std::vector<
 // sequential CPU Gorder (gorder1) on host CSR + CSC; order[pos] = vertex
 void gorder_cpu(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                 int32_t window, uint32_t* order);
+// tile-aware greedy order (GRAPH::order) on host CSR + CSC; order[pos] = vertex
+void tile_order(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
+                int32_t width1, int32_t width2, uint32_t* order);
 // Reverse Cuthill-McKee over a COO edge list (both directions are neighbours)
 void rcm_order(int32_t n, int64_t nz, const int32_t* rows, const int32_t* cols, uint32_t* order);
 // Rabbit Order on the symmetrised pattern of M (rp, ci): new_id[v] = position of v

@@ -136,6 +136,7 @@ SIGNATURES = {
     "mr_reorder_rabbit": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_gorder5": [_P, C.c_int, C.c_int, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_gorder5_order": [_P, C.c_int, C.c_int, C.c_int, _U32P, _U32P, C.POINTER(ReorderStats)],
+    "mr_reorder_tile_order": [_P, C.c_int, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_degree": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_dbg": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_csr_hash": [C.c_int64, C.c_int64, C.c_int64, _I32P, _I32P, C.POINTER(C.c_uint64)],
@@ -576,6 +577,13 @@ class Mat:
                                      C.byref(st)))
         return g[:n], f[:n], st.as_dict()
 
+    def reorder_tile_order(self, width1: int = 1024, width2: int = 512):
+        n = self.info()[0]
+        perm = np.empty(max(n, 1), np.int32)
+        st = ReorderStats()
+        check(lib().mr_reorder_tile_order(self.h, width1, width2, ptr(perm, C.c_int32), C.byref(st)))
+        return perm[:n], st.as_dict()
+
     def reorder_degree(self):
         return self._reorder_plain(lib().mr_reorder_degree)
 
@@ -889,6 +897,9 @@ class Graph:
 
     def gorder5(self, vsize: int = 0, width1: int = 1, width2: int = 5):
         return self.mat.reorder_gorder5(vsize, width1, width2)
+
+    def order(self, width1: int, width2: int):
+        return self.mat.reorder_tile_order(width1, width2)
 
     def dbg(self):
         return self.mat.reorder_dbg()

@@ -14,6 +14,7 @@
  *   oracle_gorder1           src/class.cu:1825-2013
  *   oracle_gorder5           src/class.cu:2727-2873 (fixed batches, column-count
  *                            start scores, no hub caps, CPU phase on every batch)
+ *   oracle_tile_order        src/class.cu:3158-3365 (GRAPH::order)
  *
  * Divergence kept on purpose: for n < 32 the reference's first window reads
  * d_permutation past n (Oleng[0] = 32 is never clipped, class.cu:3003); here the
@@ -392,4 +393,72 @@ int oracle_gorder5(int32_t n, const int32_t* rp, const int32_t* ci, int32_t vsiz
     for (int32_t i = 0; i < n; ++i) perm[placement_final[i]] = i;
     free(tp); free(ti); free(score); free(keys); free(batch_of);
     return B;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GRAPH::order (src/class.cu:3158-3365): the tile-aware sequential order.
+ * curr counts the x entries (nonzeros of placed rows) laid out so far; a
+ * tile holds width1 of them.  Placing a row whose entries fit in the current
+ * tile adds one to every unplaced row sharing a column with it (columns of
+ * width2 or more entries skipped); a row that crosses into the next tile first
+ * resets (refresh) every row the old tile had raised, sets PMAX = 0, moves the
+ * tile start to the tile holding its last entry, and raises rows only for its
+ * entries inside that tile.  Start: first vertex with the longest row, raised
+ * by one. */
+int oracle_tile_order(int32_t n, const int32_t* rp, const int32_t* ci, int32_t width1, int32_t width2,
+                      uint32_t* order, int32_t* perm)
+{
+    if (n <= 0) return 0;
+    int32_t* tp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* ti = (int32_t*)malloc(sizeof(int32_t) * ((size_t)rp[n] + 1));
+    oracle_transpose(n, n, rp, ci, tp, ti);
+    int32_t id0 = 0;
+    for (int32_t v = 1; v < n; ++v)
+        if (rp[v + 1] - rp[v] > rp[id0 + 1] - rp[id0]) id0 = v;
+    uint8_t* sts = (uint8_t*)malloc((size_t)n);
+    int32_t* tile = (int32_t*)malloc(sizeof(int32_t) * (size_t)(width1 > 0 ? width1 : 1));
+    oheap h;
+    oheap_init(&h, n, (int64_t)width1 + 64);
+    for (int32_t v = 0; v < n; ++v) { sts[v] = 1; oheap_push(&h, v); }
+    oheap_modify(&h, id0, 1);
+    int64_t start = 0, curr = 0;
+    for (int32_t id = 0; h.cnt > 0; ++id) {
+        const int32_t node = oheap_pop(&h);
+        sts[node] = 0;
+        order[id] = (uint32_t)node;
+        const int32_t deg = rp[node + 1] - rp[node];
+        if (curr - start + deg < width1) {
+            for (int32_t i = 0; i < deg; ++i) {
+                const int32_t idx = ci[rp[node] + i];
+                tile[curr - start] = idx;
+                if (tp[idx + 1] - tp[idx] < width2)
+                    for (int32_t j = tp[idx]; j < tp[idx + 1]; ++j)
+                        if (sts[ti[j]]) oheap_modify(&h, ti[j], 1);
+                ++curr;
+            }
+        } else {
+            for (int64_t i = start; i < curr; ++i) {
+                const int32_t idx = tile[i - start];
+                if (tp[idx + 1] - tp[idx] < width2)
+                    for (int32_t j = tp[idx]; j < tp[idx + 1]; ++j)
+                        if (sts[ti[j]]) oheap_refresh(&h, ti[j]);
+            }
+            h.pmax = 0;
+            start = (curr + deg) / width1 * width1;
+            for (int32_t i = 0; i < deg; ++i) {
+                if (curr >= start) {
+                    const int32_t idx = ci[rp[node] + i];
+                    tile[curr - start] = idx;
+                    if (tp[idx + 1] - tp[idx] < width2)
+                        for (int32_t j = tp[idx]; j < tp[idx + 1]; ++j)
+                            if (sts[ti[j]]) oheap_modify(&h, ti[j], 1);
+                }
+                ++curr;
+            }
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) perm[order[i]] = i;
+    oheap_free(&h);
+    free(tp); free(ti); free(sts); free(tile);
+    return 0;
 }

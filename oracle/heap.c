@@ -112,6 +112,17 @@ void oheap_modify(oheap* h, int32_t v, int32_t val)
     }
 }
 
+/* Integer_Heap::refresh (integer_heap.cuh:112-120): a node with key1 > 0 is
+ * taken out and re-inserted with key1 = 0 (+ its pending key2) */
+void oheap_refresh(oheap* h, int32_t v)
+{
+    if (h->key1[v] > 0) {
+        hremove(h, v);
+        h->key1[v] = 0;
+        hinsert(h, v);
+    }
+}
+
 int32_t oheap_pop(oheap* h)
 {
     while (h->cnt > 0) {
@@ -142,7 +153,9 @@ int64_t oracle_heap_trace(int32_t n_nodes, int64_t buckets, const int32_t* ops,
         const int32_t code = ops[3 * i], node = ops[3 * i + 1], val = ops[3 * i + 2];
         if (code == 0) oheap_push(&h, node);
         else if (code == 1) oheap_modify(&h, node, val);
-        else out[k++] = oheap_pop(&h);
+        else if (code == 2) out[k++] = oheap_pop(&h);
+        else if (code == 3) oheap_refresh(&h, node);
+        else h.pmax = val;                /* code 4: the caller sets PMAX */
     }
     oheap_free(&h);
     return k;

@@ -43,6 +43,7 @@ void oheap_free(oheap* h);
 void oheap_push(oheap* h, int32_t v);
 void oheap_modify(oheap* h, int32_t v, int32_t val);
 int32_t oheap_pop(oheap* h);
+void oheap_refresh(oheap* h, int32_t v);
 /* scripted trace driver used by the golden-trace tests:
  * ops[i] = {code, node, val}: code 0 push(node), 1 modify(node,val), 2 pop.
  * Popped ids are written to out (returns count). */
@@ -57,6 +58,9 @@ int  oracle_check_perm(const int32_t* perm, int64_t n);            /* utils.cuh:
 /* GRAPH::gorder5 (class.cu:2727-2873); returns the batch count */
 int oracle_gorder5(int32_t n, const int32_t* rp, const int32_t* ci, int32_t vsize, int32_t width1,
                    int32_t width2, uint32_t* placement_gpu, uint32_t* placement_final, int32_t* perm);
+/* GRAPH::order (class.cu:3158-3365): order[pos] = vertex, perm[vertex] = pos */
+int oracle_tile_order(int32_t n, const int32_t* rp, const int32_t* ci, int32_t width1, int32_t width2,
+                      uint32_t* order, int32_t* perm);
 /* degree order and DBG (class.cu:1496-1549, 1683-1756): perm[old] = new */
 void oracle_order_degree(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* perm);
 void oracle_order_dbg(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* perm);

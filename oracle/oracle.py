@@ -30,6 +30,7 @@ _SIG = {
     "oracle_check_perm": (C.c_int, [_I32P, C.c_int64]),
     "oracle_permute": (None, [C.c_int32, _I32P, _I32P, _DP, _I32P, _I32P, _I32P, _DP]),
     "oracle_order_degree": (None, [C.c_int32, _I32P, _I32P, _I32P]),
+    "oracle_tile_order": (C.c_int, [C.c_int32, _I32P, _I32P, C.c_int32, C.c_int32, _U32P, _I32P]),
     "oracle_gorder5": (C.c_int, [C.c_int32, _I32P, _I32P, C.c_int32, C.c_int32, C.c_int32, _U32P, _U32P, _I32P]),
     "oracle_order_dbg": (None, [C.c_int32, _I32P, _I32P, _I32P]),
     "oracle_pgorder_schedule": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
@@ -127,6 +128,16 @@ def gorder5(n, rp, ci, vsize, width1=1, width2=5):
     lib().oracle_gorder5(n, _p(rp, C.c_int32), _p(ci, C.c_int32), vsize, width1, width2, _p(g, C.c_uint32),
                          _p(f, C.c_uint32), _p(perm, C.c_int32))
     return g[:n], f[:n], perm[:n]
+
+
+def tile_order(n, rp, ci, width1, width2):
+    """(order, perm) of GRAPH::order."""
+    rp, ci = _i32(rp), _i32(ci if len(ci) else np.zeros(1, np.int32))
+    order = np.zeros(max(n, 1), np.uint32)
+    perm = np.zeros(max(n, 1), np.int32)
+    lib().oracle_tile_order(n, _p(rp, C.c_int32), _p(ci, C.c_int32), width1, width2, _p(order, C.c_uint32),
+                            _p(perm, C.c_int32))
+    return order[:n], perm[:n]
 
 
 def order_degree(n, rp, ci, grouped=False):

@@ -8,6 +8,8 @@ Runs in the development container only (it needs /root/reference):
                        reference heap (include/integer_heap.cuh) produces;
   * predict_ref.json   feature vectors and the speedups of the reference's
                        rd_sampled_{merbit,csr}_model.hpp;
+  * heap_refresh_traces.json  traces with refresh and PMAX resets, the heap
+                       operations of the tile-aware order;
   * rcm_ref.json       small COO graphs and R_Cuthill_Mckee's permutation
                        (rcm/src/rcm.cpp, one OpenMP thread: its reversal loop
                        shares a temporary between threads);
@@ -86,6 +88,45 @@ def heap_traces(rng) -> list:
     return out
 
 
+def heap_refresh_traces(rng) -> list:
+    """Traces in the pattern of the tile-aware order (src/class.cu:3158-3365):
+    +1 modifies, refreshes of nodes with a positive key (in any order, some
+    twice) followed by PMAX = 0, and pops."""
+    out = []
+    for t in range(30):
+        n = int(rng.integers(2, 50))
+        buckets = 4096
+        ops = [(0, v, 0) for v in range(n)]
+        key = np.zeros(n, np.int64)
+        live = np.ones(n, bool)
+        s = int(rng.integers(0, n))
+        ops.append((1, s, 1))
+        key[s] += 1
+        while live.any():
+            r = rng.random()
+            cand = np.flatnonzero(live)
+            if r < 0.25:
+                ops.append((2, 0, 0))
+                popped = oracle.heap_trace(n, buckets, np.array(ops, np.int32))
+                live[:] = True
+                live[popped] = False
+                key[~live] = 0
+            elif r < 0.35:
+                hot = np.flatnonzero(live & (key > 0))
+                for v in rng.permutation(np.concatenate([hot, hot[: len(hot) // 2]])):
+                    ops.append((3, int(v), 0))
+                key[hot] = 0
+                ops.append((4, 0, 0))
+            else:
+                v = int(rng.choice(cand))
+                ops.append((1, v, 1))
+                key[v] += 1
+        text = f"{n} {buckets} {len(ops)}\n" + "\n".join(f"{a} {b} {c}" for a, b, c in ops) + "\n"
+        pops = [int(x) for x in run("heap_trace", text).split()]
+        out.append({"n": n, "buckets": buckets, "ops": ops, "pops": pops})
+    return out
+
+
 def predict_ref(rng) -> list:
     rows = []
     feats = [np.zeros(10), np.full(10, 0.1)]
@@ -127,6 +168,7 @@ def main() -> None:
     (HERE / "heap_traces.json").write_text(json.dumps(heap_traces(rng)))
     (HERE / "predict_ref.json").write_text(json.dumps(predict_ref(rng), indent=0))
     (HERE / "rcm_ref.json").write_text(json.dumps(rcm_ref(rng)))
+    (HERE / "heap_refresh_traces.json").write_text(json.dumps(heap_refresh_traces(rng)))
     shutil.copy(REF / "ridge/out/merbit/sampled/merbit_sampled_deploy_train_all_aug_model.json",
                 HERE / "model_merbit_sampled.json")
     shutil.copy(REF / "ridge/out/csr/sampled/csr_sampled_deploy_train_all_aug_model.json",

@@ -244,3 +244,17 @@ def test_gorder5_argument_limits(ctx, mr):
         with pytest.raises(mr.MRError):
             mat.reorder_gorder5(*args)
     mat.close()
+
+
+@pytest.mark.parametrize("scale,ef,w1,w2", [(10, 8, 1024, 512), (12, 16, 256, 64), (11, 4, 7, 1000)])
+def test_tile_order_bit_exact(ctx, mr, oracle, scale, ef, w1, w2):
+    rp, ci = rmat_csr(scale, ef, seed=scale)
+    n = len(rp) - 1
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    perm, st = mat.reorder_tile_order(w1, w2)
+    _, operm = oracle.tile_order(n, rp, ci, w1, w2)
+    assert np.array_equal(perm, operm)
+    grp, gci, _ = mat.export()
+    orp, oci, _ = oracle.permute(n, rp, ci, None, operm)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci)
+    mat.close()

@@ -21,6 +21,16 @@ def test_heap_matches_reference_traces(oracle):
         assert pops.tolist() == t["pops"], f"trace n={t['n']}"
 
 
+def test_heap_refresh_traces_match_reference(oracle):
+    """refresh and PMAX resets (the tile-aware order's heap operations)
+    against pop orders of the reference Integer_Heap."""
+    traces = json.loads((GOLDEN / "heap_refresh_traces.json").read_text())
+    assert len(traces) >= 20 and any(o[0] == 3 for t in traces for o in t["ops"])
+    for t in traces:
+        pops = oracle.heap_trace(t["n"], t["buckets"], np.array(t["ops"], np.int32))
+        assert pops.tolist() == t["pops"], f"trace n={t['n']}"
+
+
 def test_heap_hand_worked_order(oracle):
     # all in bucket 0: a push goes right after the head, so the list reads
     # 0, n-1, ..., 1; raising node 2 moves it to bucket 1 first

@@ -209,3 +209,46 @@ def test_gorder5_start_scores_are_column_counts(oracle):
     ci = np.array([1, 2, 3, 1, 2, 0], np.int32)
     g, _, _ = oracle.gorder5(4, rp, ci, 1, 1, 5)
     assert g[0] == 1
+
+
+def tile_scores(A, order, pid, width1, width2):
+    """Score of every vertex before pop `pid` of the tile-aware order: for
+    each x entry of the current tile whose column has fewer than width2
+    entries, one for every row holding that column.  The tile is replayed from
+    the row lengths of the rows placed so far."""
+    n = A.shape[0]
+    rowlen, collen = A.sum(1), A.sum(0)
+    start = curr = 0
+    tile = []
+    for q in range(pid):
+        v = int(order[q])
+        cols = np.flatnonzero(A[v])
+        deg = len(cols)
+        if curr - start + deg < width1:
+            tile += cols.tolist()
+            curr += deg
+        else:
+            start = (curr + deg) // width1 * width1
+            tile = [int(c) for i, c in enumerate(cols) if curr + i >= start]
+            curr += deg
+    w = np.zeros(n, np.int64)
+    for c in tile:
+        if collen[c] < width2:
+            w[c] += 1
+    return A @ w
+
+
+@pytest.mark.parametrize("seed,n,w1,w2", [(21, 60, 16, 100), (22, 200, 64, 6), (23, 150, 7, 1000), (24, 90, 1, 5)])
+def test_tile_order_pops_carry_max_score(oracle, seed, n, w1, w2):
+    rng = np.random.default_rng(seed)
+    rp, ci = random_csr(rng, n, max_row=8, empty_frac=0.2)
+    A = dense(n, rp, ci)
+    order, perm = oracle.tile_order(n, rp, ci, w1, w2)
+    assert oracle.check_perm(perm) and np.array_equal(perm[order.astype(np.int64)], np.arange(n))
+    assert order[0] == int(np.argmax(A.sum(1)))        # first longest row
+    placed = np.zeros(n, bool)
+    placed[order[0]] = True
+    for pid in range(1, n):
+        s = tile_scores(A, order, pid, w1, w2)
+        assert s[order[pid]] == s[~placed].max(), f"pop {pid}"
+        placed[order[pid]] = True
