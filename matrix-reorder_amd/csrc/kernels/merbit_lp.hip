@@ -57,6 +57,29 @@ template <bool NT> __device__ __forceinline__ u32x4 ld_vec(const u32x4* p)
     else return *p;
 }
 
+// the column and value words of lane `lane` in tile t
+template <typename VT, int S, bool NT>
+__device__ __forceinline__ void load_tile(const int32_t* __restrict__ pc, const VT* __restrict__ pv, int64_t t,
+                                          int lane, uint32_t* cw, uint32_t* vw)
+{
+    constexpr int T = 64 * S;
+    constexpr int CW = S;
+    constexpr int VW = S * (int)sizeof(VT) / 4;
+    const int64_t base = t * (int64_t)T + (int64_t)lane * S;
+    const u32x4* cp = reinterpret_cast<const u32x4*>(pc + base);
+    const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pv) + base * (int64_t)sizeof(VT));
+#pragma unroll
+    for (int q = 0; q < CW / 4; ++q) {
+        const u32x4 a = ld_vec<NT>(cp + q);
+        cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
+    }
+#pragma unroll
+    for (int q = 0; q < VW / 4; ++q) {
+        const u32x4 b = ld_vec<NT>(vp + q);
+        vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
+    }
+}
+
 template <typename VT, int S, int W, bool NT>
 __global__ __launch_bounds__(64 * W) void merbit_lp_kernel(int64_t n_tiles, const int32_t* __restrict__ pc,
                                                            const VT* __restrict__ pv, const int32_t* __restrict__ ty,
@@ -64,7 +87,6 @@ __global__ __launch_bounds__(64 * W) void merbit_lp_kernel(int64_t n_tiles, cons
                                                            int xc, float* __restrict__ y, float* __restrict__ carry,
                                                            int32_t* __restrict__ carry_row)
 {
-    constexpr int T = 64 * S;
     constexpr int CW = S;                          // column words per lane
     constexpr int VW = S * (int)sizeof(VT) / 4;    // value words per lane
     static_assert(CW % 4 == 0 && VW % 4 == 0, "lanes load whole 16-byte vectors");
@@ -76,29 +98,28 @@ __global__ __launch_bounds__(64 * W) void merbit_lp_kernel(int64_t n_tiles, cons
 
     for (int64_t t = (int64_t)blockIdx.x * W + wid; t < n_tiles; t += (int64_t)gridDim.x * W) {
         const int32_t y0 = ty[t], y1 = ty[t + 1];
-        const int64_t base = t * (int64_t)T + (int64_t)lane * S;
         uint32_t cw[CW], vw[VW];
-        const u32x4* cp = reinterpret_cast<const u32x4*>(pc + base);
-        const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pv) + base * (int64_t)sizeof(VT));
-#pragma unroll
-        for (int q = 0; q < CW / 4; ++q) {
-            const u32x4 a = ld_vec<NT>(cp + q);
-            cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
-        }
-#pragma unroll
-        for (int q = 0; q < VW / 4; ++q) {
-            const u32x4 b = ld_vec<NT>(vp + q);
-            vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
-        }
-        float p[S];
+        load_tile<VT, S, NT>(pc, pv, t, lane, cw, vw);
+        // x gathers into registers of their own for HBM and for LDS (one
+        // register per item would make every global load wait for the LDS
+        // read that last wrote it); the global ones are issued first
+        float xg[S], xl[S];
 #pragma unroll
         for (int j = 0; j < S; ++j) {
-            const int32_t c = (int32_t)cw[j];
-            float xv = 0.0f;
-            if (c >= 0) xv = x[c];
-            else if (c < kPadItem) xv = xcache[c & 0x7fffffff];
-            p[j] = value_at<VT>(vw, j) * xv;
+            xg[j] = 0.0f;
+            if ((int32_t)cw[j] >= 0) xg[j] = x[(int32_t)cw[j]];
         }
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            xl[j] = 0.0f;
+            if ((int32_t)cw[j] < kPadItem) xl[j] = xcache[cw[j] & 0x7fffffffu];
+        }
+        float xv[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) xv[j] = (int32_t)cw[j] >= 0 ? xg[j] : xl[j];
+        float p[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) p[j] = value_at<VT>(vw, j) * xv[j];
 
         if (y0 == y1) {
             // every item consumes a nonzero of row y0 (fast track)
