@@ -3,10 +3,10 @@
 Every ``csrc/kernels/*.hip`` file is compiled with hipcc for gfx950 and every
 ``csrc/*.cpp`` host file with g++ against the HIP runtime headers; the objects
 are linked into ``matrix-reorder_amd/lib/libmr.so``.  The optional CLI driver
-``cli/mr_main.cpp`` (the reference's main.cu workflow on the C++ mirror) is
-linked against it when ``with_cli`` is set.
+``cli/mr_main.cpp`` (the reference's main.cu workflow on the C++ mirror of its
+GRAPH interface, include/mr_graph.hpp) is linked against it into lib/mr_main.
 
-Usage: ``python matrix-reorder_amd/build.py [--jobs N] [--cli]``
+Usage: ``python matrix-reorder_amd/build.py [--jobs N] [--no-cli]``
 """
 from __future__ import annotations
 
@@ -52,7 +52,7 @@ def _compile(src: pathlib.Path) -> pathlib.Path:
     return obj
 
 
-def build(jobs: int | None = None, with_cli: bool = False) -> pathlib.Path:
+def build(jobs: int | None = None, with_cli: bool = True) -> pathlib.Path:
     OBJ.mkdir(exist_ok=True)
     LIB.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("kernels/*.hip")) + sorted(CSRC.glob("*.cpp"))
@@ -65,10 +65,9 @@ def build(jobs: int | None = None, with_cli: bool = False) -> pathlib.Path:
           *[str(o) for o in objs], "-lpthread", "-Wl,-soname,libmr.so"])
     os.replace(tmp, out)
     if with_cli:
-        cli = PKG / "cli" / "mr_main.cpp"
-        if cli.exists():
-            _run(["g++", *CXX_FLAGS, str(cli), "-o", str(LIB / "mr_main"), "-L" + str(LIB), "-lmr",
-                  "-Wl,-rpath,$ORIGIN", "-L" + str(ROCM / "lib"), "-lamdhip64"])
+        # the driver needs only mr.h / mr_graph.hpp and libmr (no HIP headers)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + str(ROOT / "include"), str(PKG / "cli" / "mr_main.cpp"),
+              "-o", str(LIB / "mr_main"), "-L" + str(LIB), "-lmr", "-Wl,-rpath,$ORIGIN"])
     return out
 
 
@@ -79,7 +78,7 @@ def clean() -> None:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
-    ap.add_argument("--cli", action="store_true")
+    ap.add_argument("--no-cli", action="store_true")
     a = ap.parse_args()
-    print(build(a.jobs, a.cli))
+    print(build(a.jobs, not a.no_cli))
     sys.exit(0)

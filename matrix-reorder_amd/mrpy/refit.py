@@ -100,8 +100,8 @@ def local_swap(n: int, rho: float, near: int, window: int, seed: int) -> np.ndar
 # ---------------------------------------------------------------- corpus
 def corpus():
     """Base matrices of the training set: (name, family, spec).  Sizes run
-    from 2^17 to 2^22 rows, so x runs from 0.5 MB (resident in L2) to 16 MB
-    (four times the L2 of one XCD)."""
+    from 2^17 to 2^24 rows, so x runs from 0.5 MB (resident in L2) to 64 MB
+    (twice the L2 of all eight XCDs)."""
     out = []
     skews = {"a57": (0.57, 0.19, 0.19), "a45": (0.45, 0.15, 0.15), "a65": (0.65, 0.15, 0.15)}
     for scale in (17, 19, 21):
@@ -123,6 +123,12 @@ def corpus():
     out.append(("rmat_s22_ef8_a65", "rmat", dict(scale=22, ef=8, abc=skews["a65"], sym=False)))
     out.append(("banded_n4194304_bw2048", "banded", dict(n=1 << 22, bw=2048, per_row=12)))
     out.append(("stencil27_160", "stencil", dict(g=160)))
+    # the scales of the reference's published R-MAT sweep (s24, symmetric)
+    out.append(("rmat_s23_ef8_a57", "rmat", dict(scale=23, ef=8, abc=skews["a57"], sym=False)))
+    out.append(("rmat_s23_ef16_a65", "rmat", dict(scale=23, ef=16, abc=skews["a65"], sym=False)))
+    out.append(("rmat_sym_s24_ef4_a57", "rmat", dict(scale=24, ef=4, abc=skews["a57"], sym=True)))
+    out.append(("rmat_sym_s24_ef4_a45", "rmat", dict(scale=24, ef=4, abc=skews["a45"], sym=True)))
+    out.append(("rmat_sym_s24_ef8_a65", "rmat", dict(scale=24, ef=8, abc=skews["a65"], sym=True)))
     return out
 
 
@@ -155,7 +161,7 @@ def _time(ctx, mat, xd, yd, kernel, reps, warmup):
     return best * 1e3            # microseconds
 
 
-def collect(out_path: str, limit: int = 0, reps: int = 20) -> None:
+def collect(out_path: str, limit: int = 0, reps: int = 20, skip: int = 0) -> None:
     import mrpy as mr
 
     ctx = mr.Ctx(0)
@@ -166,14 +172,14 @@ def collect(out_path: str, limit: int = 0, reps: int = 20) -> None:
             + [f"r{i + 1}" for i in range(FEATURES)] + [f"q{i + 1}" for i in range(FEATURES)])
     path = pathlib.Path(out_path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    items = corpus()
+    items = list(enumerate(corpus()))[skip:]
     if limit:
         items = items[:limit]
     t_start = time.perf_counter()
     with path.open("w", newline="") as fh:
         w = csv.writer(fh)
         w.writerow(cols)
-        for gi, (name, family, spec) in enumerate(items):
+        for k, (gi, (name, family, spec)) in enumerate(items):
             seed = 7001 + gi
             rp, ci = build(mr, family, spec, seed)
             n, nnz = len(rp) - 1, len(ci)
@@ -230,7 +236,7 @@ def collect(out_path: str, limit: int = 0, reps: int = 20) -> None:
             fh.flush()
             xd.free()
             yd.free()
-            print(f"[refit] {gi + 1}/{len(items)} {name}: n={n} nnz={nnz} pgorder {t_pg[0]:.1f} us "
+            print(f"[refit] {k + 1}/{len(items)} {name}: n={n} nnz={nnz} pgorder {t_pg[0]:.1f} us "
                   f"({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
     ctx.close()
 
@@ -364,6 +370,7 @@ def main(argv=None) -> int:
     c.add_argument("--out", required=True)
     c.add_argument("--limit", type=int, default=0, help="first N matrices of the corpus only")
     c.add_argument("--reps", type=int, default=20)
+    c.add_argument("--skip", type=int, default=0, help="start at this corpus index (seeds stay per index)")
     f = sub.add_parser("fit", help="fit the models on the CPU")
     f.add_argument("--csv", required=True)
     f.add_argument("--header", default="")
@@ -371,7 +378,7 @@ def main(argv=None) -> int:
     f.add_argument("--alpha", type=float, default=1.0)
     a = ap.parse_args(argv)
     if a.cmd == "collect":
-        collect(a.out, a.limit, a.reps)
+        collect(a.out, a.limit, a.reps, a.skip)
         return 0
     report, chosen = fit_shared(a.csv, a.alpha)
     if a.header:

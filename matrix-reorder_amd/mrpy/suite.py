@@ -20,6 +20,7 @@ beside the published origin / gorder1 times of the same id
 
 Usage, from the package directory:
     python -m mrpy.suite --ids 1-25 --out ../gpurun_out/suite/rmat_1_25.json
+    python -m mrpy.suite --summarize run1.json run2.json --out suite/rmat_s24_mi355x.json
 """
 from __future__ import annotations
 
@@ -114,12 +115,52 @@ def run(ids, out_path: str, width: int = 1024) -> list:
     return results
 
 
+def geomean(v) -> float:
+    v = [float(x) for x in v]
+    return float(np.exp(np.mean(np.log(v)))) if v else float("nan")
+
+
+def summarize(paths, out_path: str = "") -> dict:
+    """Merge run files into one record per id and the sweep-wide figures."""
+    rows = {}
+    for pth in paths:
+        for r in json.loads(pathlib.Path(pth).read_text()):
+            rows[r["id"]] = r
+    rs = [rows[k] for k in sorted(rows)]
+    summ = {
+        "matrices": len(rs),
+        "geomean_origin_gflops": geomean(r["origin_gflops"] for r in rs),
+        "geomean_pgorder_gflops": geomean(r["pgorder_gflops"] for r in rs),
+        "geomean_published_origin_gflops": geomean(r["published_origin_gflops"] for r in rs),
+        "geomean_published_gorder1_gflops": geomean(r["published_gorder1_gflops"] for r in rs),
+        "geomean_measured_speedup": geomean(r["measured_speedup"] for r in rs),
+        "geomean_published_speedup": geomean(r["published_speedup"] for r in rs),
+        "geomean_after_reorder_vs_published": geomean(r["pgorder_gflops"] / r["published_gorder1_gflops"] for r in rs),
+        "mean_pgorder_s": float(np.mean([r["pgorder_wall_ms"] for r in rs]) / 1e3),
+        "mean_published_gorder1_s": float(np.mean([r["published_gorder1_reorder_ms"] for r in rs]) / 1e3),
+        "median_abs_log_err_sm89_model": float(np.median([abs(np.log(r["predicted_speedup_sm89_model"] /
+                                                                     r["measured_speedup"])) for r in rs])),
+        "median_abs_log_err_mi355x_model": float(np.median([abs(np.log(r["predicted_speedup_mi355x_model"] /
+                                                                       r["measured_speedup"])) for r in rs])),
+    }
+    out = {"summary": summ, "matrices": rs}
+    if out_path:
+        pathlib.Path(out_path).write_text(json.dumps(out, indent=1) + "\n")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m mrpy.suite")
     ap.add_argument("--ids", default="1-75")
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", default="")
     ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--summarize", nargs="*", default=None, help="merge these run files instead of running")
     a = ap.parse_args(argv)
+    if a.summarize is not None:
+        print(json.dumps(summarize(a.summarize, a.out)["summary"], indent=1))
+        return 0
+    if not a.out:
+        ap.error("--out is required")
     run(parse_ids(a.ids), a.out, a.width)
     return 0
 

@@ -161,3 +161,20 @@ def test_serve_rhs_and_hash_host_side(mr):
     d = a.copy()
     d[[0, 1]] = d[[1, 0]]
     assert mr.serve_hash(d) != h
+
+
+def test_driver_usage_and_header_compile(tmp_path):
+    """lib/mr_main prints its usage without touching a device; the C++ GRAPH
+    mirror compiles on its own against mr.h."""
+    import pathlib
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    exe = root / "matrix-reorder_amd" / "lib" / "mr_main"
+    assert exe.exists()
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=30)
+    assert res.returncode == 2 and "usage" in res.stderr
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "mr_graph.hpp"\nint main() { mr::GRAPH<double> g; return g.nrow; }\n')
+    res = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I" + str(root / "include"),
+                          str(src)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
