@@ -11,30 +11,30 @@ namespace mi355x {
 
 constexpr int32_t kEdges[9] = {1, 32, 112, 256, 1024, 8192, 32768, 131072, 262144};
 constexpr int32_t kRdCap = 262144;
-constexpr double kMerbitB = 1.73328011948709754e-01;
+constexpr double kMerbitB = 1.63398337184711667e-01;
 constexpr double kMerbitW[10] = {
-    -3.02546653968843504e-01,
-    4.29554919397135970e-01,
-    -1.27908865587784082e-01,
-    -8.37731448790604016e-01,
-    2.39636422739685245e-01,
-    -1.49872799344546836e-01,
-    -6.02353580382591694e-02,
-    9.55997949159451355e-01,
-    4.50158192790066547e+00,
-    0.00000000000000000e+00};
-constexpr double kCsrB = 1.32175979119135034e-01;
+    -3.51794074582339744e-01,
+    5.31324813545729802e-01,
+    -1.66389724261768374e-01,
+    -7.98905254006095578e-01,
+    2.80750310336807640e-01,
+    -1.61563603057311711e-01,
+    -6.25217327073647466e-02,
+    9.76358860299390119e-01,
+    -4.03639884710281227e-02,
+    -6.99778603294693546e-01};
+constexpr double kCsrB = 1.24099095973469756e-01;
 constexpr double kCsrW[10] = {
-    -1.54358492973378864e-01,
-    -2.89846662499796629e-03,
-    -1.48951882984405981e-01,
-    -2.35116544816686479e-01,
-    -1.43034179613825557e-01,
-    -8.61775869231759528e-02,
-    -4.08774935846230175e-02,
-    7.93306515858193451e-01,
-    1.06863912464851332e+01,
-    0.00000000000000000e+00};
+    -2.03048208790278378e-01,
+    9.88761711003832283e-02,
+    -1.91238423157624932e-01,
+    -2.01382571619076994e-01,
+    -1.01575576950520288e-01,
+    -9.74694858397428116e-02,
+    -4.82297503596397087e-02,
+    8.30507451848058409e-01,
+    6.56543641088430385e-02,
+    -6.38010089589619467e-01};
 
 }  // namespace mi355x
 }  // namespace mr

@@ -85,10 +85,12 @@ def run(ids, out_path: str, width: int = 1024) -> list:
         yd = ctx.empty(n, np.float32)
         mat.merbit_autotune(5)
         rec["origin_ms"] = time_merbit(ctx, mr, mat, xd, yd)
-        sp, _, _ = mat.locality_predict(mr.MODEL_MERBIT)
-        sp_mi, _, _ = mat.locality_predict(mr.MODEL_MERBIT_MI355X)
+        sp, f_ref, _ = mat.locality_predict(mr.MODEL_MERBIT)
+        sp_mi, f_mi, _ = mat.locality_predict(mr.MODEL_MERBIT_MI355X)
         rec["predicted_speedup_sm89_model"] = sp
         rec["predicted_speedup_mi355x_model"] = sp_mi
+        rec["rd_features"] = [float(v) for v in f_ref]
+        rec["rd_features_mi355x_edges"] = [float(v) for v in f_mi]
         log(f"id {gid}: a={p['a']} ef={p['ef']} nnz={nnz} origin {rec['origin_ms'] * 1e3:.0f} us")
         t = time.perf_counter()
         _, st = mat.reorder_pgorder(width, 5)
