@@ -755,7 +755,8 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
     MR_HIP_TRY(hipEventCreate(&e0));
     MR_HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    int bs = m->merbit.sigma, bp = m->merbit.policy, bxs = m->merbit.xs, bxw = m->merbit.xw;
+    int bs = m->merbit.sigma, bp = m->merbit.policy, bxs = m->merbit.xs, bxw = m->merbit.xw,
+        bxc = m->merbit.xc_req;
     auto time_it = [&](float* us) -> int {
         MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
         MR_HIP_TRY(hipEventRecord(e0, s));
@@ -786,29 +787,37 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
         m->merbit.sigma = bs;
         rc = merbit_build(m, s);
     }
-    // wave tiles with the LDS x-cache: (items per lane, waves) x streaming hint
+    // wave tiles with the full LDS x-cache: (items per lane, waves) x streaming
+    // hint.  Smaller caches (more resident workgroups per CU) were slower on
+    // every matrix measured, so they are not searched.
     if (m->dtype != MR_F64) {
         const int shapes[5][2] = {{8, 16}, {16, 16}, {8, 8}, {16, 8}, {4, 16}};
+        const int xc0 = m->merbit.xc_req;
         for (int q = 0; q < 5 && rc == MR_OK; ++q) {
             if (shapes[q][0] == 4 && m->dtype != MR_F32) continue;
             m->merbit.xs = shapes[q][0];
             m->merbit.xw = shapes[q][1];
+            m->merbit.xc_req = -1;
             m->merbit.xc_built = false;
             for (int nt = 0; nt < 2 && rc == MR_OK; ++nt) {
                 m->merbit.policy = 4 | nt;
                 float us = 0;
                 rc = time_it(&us);
-                if (rc == MR_OK && us < best) { best = us; bp = 4 | nt; bxs = shapes[q][0]; bxw = shapes[q][1]; }
+                if (rc == MR_OK && us < best) {
+                    best = us; bp = 4 | nt; bxs = shapes[q][0]; bxw = shapes[q][1]; bxc = -1;
+                }
             }
         }
+        if (!(bp & 4)) m->merbit.xc_req = xc0;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc != MR_OK) return rc;
     m->merbit.policy = bp;
-    if (m->merbit.xs != bxs || m->merbit.xw != bxw) {
+    if (m->merbit.xs != bxs || m->merbit.xw != bxw || ((bp & 4) && m->merbit.xc_req != bxc)) {
         m->merbit.xs = bxs;
         m->merbit.xw = bxw;
+        if (bp & 4) m->merbit.xc_req = bxc;
         m->merbit.xc_built = false;
     }
     if (bp & 4) MR_TRY(merbit_xc_build(m, s));

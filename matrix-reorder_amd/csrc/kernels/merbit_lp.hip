@@ -23,6 +23,8 @@
 //  * the open row at a tile end goes to carry[t] / carry_row[t]; a wave-parallel
 //    fix-up adds each run of carries to its row in a fixed order, so results
 //    are bit-reproducible (no floating-point atomics).
+#include <algorithm>
+
 #include "dtypes.hpp"
 #include "launch.hpp"
 #include "ldst.hpp"
@@ -484,7 +486,15 @@ hipError_t launch_lp(int64_t n_rows, int64_t n_tiles, const int32_t* pc, const v
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    kern<<<(unsigned)grid, 64 * W, lds, s>>>(n_tiles, pc, static_cast<const VT*>(pv), ty,
+    // persistent workgroups: as many per CU as the LDS cache and the wave
+    // slots allow (one with the full cache; more when fewer columns are cached)
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 64 * W, lds) !=
+            hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int64_t want = (int64_t)grid * per_cu;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(want, (n_tiles + W - 1) / W));
+    kern<<<(unsigned)blocks, 64 * W, lds, s>>>(n_tiles, pc, static_cast<const VT*>(pv), ty,
                                              static_cast<const float*>(x), hot, xc, static_cast<float*>(y),
                                              static_cast<float*>(carry), carry_row);
     e = hipGetLastError();
