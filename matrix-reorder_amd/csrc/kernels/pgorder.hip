@@ -483,8 +483,7 @@ __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t
 __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict__ tp, const int32_t* __restrict__ ti,
                                                        const int32_t* __restrict__ live,
                                                        int32_t* __restrict__ h, const int32_t* __restrict__ ylist,
-                                                       uint32_t* __restrict__ score, int32_t* __restrict__ T,
-                                                       PgState* __restrict__ st, int nxt)
+                                                       uint32_t* __restrict__ score, const PgState* __restrict__ st)
 {
     // groups of 16 lanes, four listed columns per wave; rows(y) holds only
     // vertices still unplaced at the last compaction (live[y] entries)
@@ -503,15 +502,31 @@ __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict
         }
         for (int32_t c0 = zb; c0 < ze; c0 += 16) {
             const int32_t c = c0 + li;
-            bool first = false;
-            int32_t z = 0;
             if (c < ze) {
-                z = ti[c];
-                if (score[z] & kUnplaced) first = atomicAdd(&score[z], w) == kUnplaced;
+                const int32_t z = ti[c];
+                // the old value is not needed: a non-returning atomic, so the
+                // wave does not wait for it (the touched list is rebuilt by
+                // touched_kernel)
+                if (score[z] & kUnplaced) atomicAdd(&score[z], w);
             }
-            wave_append(first, z, T, &st->tcount[nxt]);
         }
         if (q < ny && li == 0) h[y] = 0;   // y is listed once: back to 0 for the next window
+    }
+}
+
+// the touched list of the next batch: every unplaced vertex whose count the
+// window made non-zero (all counts were 0 before the window)
+__global__ __launch_bounds__(256) void touched_kernel(int64_t n, const uint32_t* __restrict__ score,
+                                                      int32_t* __restrict__ T, PgState* __restrict__ st, int nxt)
+{
+    const int64_t lim = (n + 63) / 64 * 64;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < lim; v += (int64_t)gridDim.x * blockDim.x) {
+        bool pred = false;
+        if (v < n) {
+            const uint32_t sc = score[v];
+            pred = (sc & kUnplaced) && sc != kUnplaced;
+        }
+        wave_append(pred, (int32_t)v, T, &st->tcount[nxt]);
     }
 }
 
@@ -574,7 +589,8 @@ hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, co
     if (ol > 0) {
         window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st,
                                                     hub_cap);
-        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, ylist, score, Tl[nxt], st, nxt);
+        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, ylist, score, st);
+        touched_kernel<<<kGridStride, 256, 0, s>>>(n, score, Tl[nxt], st, nxt);
     }
     return hipGetLastError();
 }
