@@ -515,18 +515,39 @@ __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict
 }
 
 // the touched list of the next batch: every unplaced vertex whose count the
-// window made non-zero (all counts were 0 before the window)
+// window made non-zero (all counts were 0 before the window).  Each block
+// owns a contiguous id range, counts it, reserves its slice of the list with
+// one atomic, then writes it in id order (a counter shared by every wave
+// would serialise tens of thousands of atomics per batch).
+__device__ __forceinline__ bool is_touched(uint32_t sc) { return (sc & kUnplaced) && sc != kUnplaced; }
+
 __global__ __launch_bounds__(256) void touched_kernel(int64_t n, const uint32_t* __restrict__ score,
                                                       int32_t* __restrict__ T, PgState* __restrict__ st, int nxt)
 {
-    const int64_t lim = (n + 63) / 64 * 64;
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < lim; v += (int64_t)gridDim.x * blockDim.x) {
-        bool pred = false;
-        if (v < n) {
-            const uint32_t sc = score[v];
-            pred = (sc & kUnplaced) && sc != kUnplaced;
-        }
-        wave_append(pred, (int32_t)v, T, &st->tcount[nxt]);
+    __shared__ int s_w[4];
+    __shared__ int s_base;
+    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
+    if (lo >= hi) return;                       // uniform over the block
+    int cnt = 0;
+    for (int64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) cnt += is_touched(score[v]) ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        s_base = tot ? atomicAdd(&st->tcount[nxt], tot) : 0;
+    }
+    __syncthreads();
+    int run = s_base;
+    for (int64_t c = lo; c < hi; c += blockDim.x) {
+        const int64_t v = c + threadIdx.x;
+        const bool f = v < hi && is_touched(score[v]);
+        int rank = 0;
+        const int tot = block_rank(f, rank, s_w);
+        if (f) T[run + rank] = (int32_t)v;
+        run += tot;
     }
 }
 
