@@ -42,6 +42,8 @@ namespace {
 constexpr uint32_t kUnplaced = 0x80000000u;
 constexpr int kExactBins = 65536;
 constexpr int kBins = kExactBins + 16 * 64;
+constexpr int kCoarseShift = 9;                    // 512 count bins per coarse bin
+constexpr int kCoarse = (kBins + (1 << kCoarseShift) - 1) >> kCoarseShift;   // 130
 constexpr int kSmallT = 4096;     // sort T directly at or below this size
 constexpr int kChunk = 1024;      // ids per unplaced counter
 constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
@@ -130,6 +132,9 @@ __global__ void init_kernel(int64_t n, const int32_t* __restrict__ rp, uint32_t*
     }
 }
 
+// count histogram of T: hist[0, kBins) per count bin, hist[kBins + c] per
+// coarse bin c of 512 count bins (the threshold search reads the coarse bins
+// first)
 __global__ __launch_bounds__(256) void hist_kernel(const int32_t* __restrict__ T, const PgState* __restrict__ st,
                                                    int cur, const uint32_t* __restrict__ score,
                                                    int32_t* __restrict__ hist)
@@ -137,73 +142,98 @@ __global__ __launch_bounds__(256) void hist_kernel(const int32_t* __restrict__ T
     const int32_t tn = st->tcount[cur];
     if (tn <= kSmallT) return;
     __shared__ int32_t h[4096];
+    __shared__ int32_t hc[kCoarse];
     for (int i = threadIdx.x; i < 4096; i += blockDim.x) h[i] = 0;
+    for (int i = threadIdx.x; i < kCoarse; i += blockDim.x) hc[i] = 0;
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tn; i += (int64_t)gridDim.x * blockDim.x) {
         const int b = count_bin(score[T[i]] & ~kUnplaced);
         if (b < 4096) atomicAdd(&h[b], 1);
         else atomicAdd(&hist[b], 1);
+        atomicAdd(&hc[b >> kCoarseShift], 1);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 4096; i += blockDim.x)
         if (h[i]) atomicAdd(&hist[i], h[i]);
+    for (int i = threadIdx.x; i < kCoarse; i += blockDim.x)
+        if (hc[i]) atomicAdd(&hist[kBins + i], hc[i]);
 }
 
+// inclusive suffix sums of one value per thread over a 256-thread block
+// (thread t gets the sum over threads >= t)
+__device__ __forceinline__ int64_t block_suffix(int64_t v, int64_t* s_tot)
+{
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t o = __shfl_down(v, d, 64);
+        if (lane + d < 64) v += o;
+    }
+    if (lane == 0) s_tot[wid] = v;     // wave total
+    __syncthreads();
+    int64_t higher = 0;
+    for (int w = wid + 1; w < 4; ++w) higher += s_tot[w];
+    __syncthreads();
+    return v + higher;
+}
+
+// the threshold bin tb: the bins above it hold fewer than pl vertices, with it
+// at least pl.  Two coalesced passes: the coarse bins, then the 512 bins of
+// the coarse bin that holds the threshold.
 __global__ __launch_bounds__(256) void thresh_kernel(const int32_t* __restrict__ hist, PgState* __restrict__ st,
                                                      int cur, int32_t pl)
 {
     const int32_t tn = st->tcount[cur];
     if (tn <= kSmallT) return;
-    __shared__ int64_t s_sum[256];
-    __shared__ int64_t s_above[256];
-    constexpr int seg = (kBins + 255) / 256;
+    __shared__ int64_t s_tot[4];
+    __shared__ int s_c;
+    __shared__ int64_t s_above_c;
     const int t = threadIdx.x;
-    const int lo = t * seg, hi = min(kBins, lo + seg);
-    int64_t sum = 0;
-    for (int bnum = lo; bnum < hi; ++bnum) sum += hist[bnum];
-    // exclusive suffix sums over the 256 segments: wave scans from the top
-    {
-        const int lane = t & 63, wid = t >> 6;
-        int64_t v = sum;   // inclusive suffix inside the wave (lanes above this one)
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int64_t o = __shfl_down(v, d, 64);
-            if (lane + d < 64) v += o;
-        }
-        if (lane == 0) s_sum[wid] = v;   // wave total
-        __syncthreads();
-        int64_t higher = 0;
-        for (int w = wid + 1; w < 4; ++w) higher += s_sum[w];
-        s_above[t] = higher + v - sum;
-    }
-    __syncthreads();
     if (pl <= 0 || tn < pl) {
         if (t == 0) { st->tb = -1; st->above = tn; st->need = 0; }
         return;
     }
-    const int64_t above = s_above[t];
-    if (above < pl && above + sum >= pl) {
-        int64_t run = above;
-        for (int bnum = hi - 1; bnum >= lo; --bnum) {
-            const int32_t c = hist[bnum];
-            if (run + c >= pl) {
-                st->tb = bnum;
-                st->above = (int32_t)run;
-                st->need = (int32_t)(pl - run);
-                break;
-            }
-            run += c;
-        }
+    // coarse: the highest coarse bin c with suffix(c) >= pl
+    const int64_t c_cnt = t < kCoarse ? hist[kBins + t] : 0;
+    const int64_t c_suf = block_suffix(c_cnt, s_tot);
+    if (c_suf >= pl && c_suf - c_cnt < pl) {
+        s_c = t;
+        s_above_c = c_suf - c_cnt;
+    }
+    __syncthreads();
+    const int c = s_c;
+    const int64_t above_c = s_above_c;
+    // fine: two bins per thread inside coarse bin c, thread t owning bins
+    // lo + 2t and lo + 2t + 1
+    const int lo = c << kCoarseShift;
+    const int b0 = lo + 2 * t, b1 = b0 + 1;
+    const int64_t h0 = b0 < kBins ? hist[b0] : 0, h1 = b1 < kBins ? hist[b1] : 0;
+    const int64_t suf = block_suffix(h0 + h1, s_tot) + above_c;   // bins >= b0
+    // bin b1 first (the higher count), then b0
+    const int64_t above1 = suf - h0 - h1;
+    if (above1 < pl && above1 + h1 >= pl) {
+        st->tb = b1;
+        st->above = (int32_t)above1;
+        st->need = (int32_t)(pl - above1);
+    } else if (above1 + h1 < pl && suf >= pl) {
+        st->tb = b0;
+        st->above = (int32_t)(above1 + h1);
+        st->need = (int32_t)(pl - above1 - h1);
     }
 }
 
 __global__ __launch_bounds__(256) void collect_kernel(const int32_t* __restrict__ T, PgState* __restrict__ st,
                                                       int cur, const uint32_t* __restrict__ score,
                                                       int32_t* __restrict__ sure, int32_t* __restrict__ tie,
-                                                      int32_t pl)
+                                                      int32_t pl, int32_t* __restrict__ hist)
 {
     const int32_t tn = st->tcount[cur];
-    if (tn <= kSmallT || pl <= 0) return;
+    if (tn <= kSmallT) return;
+    // the histogram was consumed by thresh_kernel: clear it for the next batch
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < kBins + kCoarse;
+         i += (int64_t)gridDim.x * blockDim.x)
+        hist[i] = 0;
+    if (pl <= 0) return;
     const int tb = st->tb;
     const int64_t lim = ((int64_t)tn + 63) / 64 * 64;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
@@ -393,9 +423,8 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
         }
     }
     __syncthreads();
-    // clear per-batch scratch for the next batch
-    if (tn > kSmallT)
-        for (int i = tid; i < kBins; i += blockDim.x) hist[i] = 0;
+    // clear per-batch scratch for the next batch (the histogram is cleared by
+    // collect_kernel)
     if (tid == 0) {
         st->nsure = 0;
         st->ntie = 0;
@@ -417,11 +446,34 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
             s_len[w] = len;
         }
         __syncthreads();
-        if (tid == 0) {
-            int32_t run = 0;
-            for (int w = 0; w < ol; ++w) { wprefix[w] = run; run += s_len[w]; }
-            wprefix[ol] = run;
+        // exclusive prefix over ol <= 1024 lengths: four consecutive entries
+        // per thread, then a block scan of the per-thread sums
+        int32_t mine[4];
+        int32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int w = 4 * tid + j;
+            mine[j] = w < ol ? s_len[w] : 0;
+            sum += mine[j];
         }
+        const int lane = tid & 63, wid = tid >> 6;
+        int32_t inc = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) s_w[wid] = inc;
+        __syncthreads();
+        int32_t run = inc - sum;
+        for (int w = 0; w < wid; ++w) run += s_w[w];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int w = 4 * tid + j;
+            if (w < ol) wprefix[w] = run;
+            run += mine[j];
+        }
+        if (4 * tid < ol && 4 * tid + 4 >= ol) wprefix[ol] = run;   // the thread holding entry ol - 1
     } else if (tid == 0) {
         wprefix[0] = 0;
     }
@@ -603,7 +655,7 @@ hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, co
     const int nxt = cur ^ 1;
     hist_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, hist);
     thresh_kernel<<<1, 256, 0, s>>>(hist, st, cur, pl);
-    collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl);
+    collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl, hist);
     final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, hist, chunk_left, placement, ps, pl, rp,
                                    os, ol, wprefix, hub_cap);
     reset_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score);
@@ -635,7 +687,7 @@ hipError_t pg_live_init(int64_t n, const int32_t* tp, int32_t* live, hipStream_t
     return hipGetLastError();
 }
 
-int pg_hist_bins() { return kBins; }
+int pg_hist_bins() { return kBins + kCoarse; }
 
 }  // namespace k
 }  // namespace mr
