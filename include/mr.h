@@ -153,7 +153,9 @@ int mr_permute_vec(mr_mat m, int dir, const void* in, void* out, mr_stream s);
 typedef struct {
     int32_t width, window, batches;
     double gpu_ms;       /* GPU placement loop (incl. transpose)                 */
-    double cpu_ms;       /* host phase (per-batch Gorder / whole CPU order)      */
+    double cpu_ms;       /* host phase (per-batch Gorder / whole CPU order); for
+                            pGOrder / gorder5 it overlaps the GPU loop and is the
+                            mean busy time of the host threads               */
     double relabel_ms;   /* device relabel P A P^T + MERBIT rebuild               */
     double total_ms;
 } mr_reorder_stats;

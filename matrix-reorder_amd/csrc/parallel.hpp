@@ -2,6 +2,8 @@
 // (generators, CSR assembly, the pGOrder CPU phase).  No OpenMP runtime is
 // linked into libmr, so it can share a process with any other OpenMP user.
 #pragma once
+#include <sched.h>
+
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
@@ -21,7 +23,13 @@ inline int& host_thread_cap()
 inline int host_threads()
 {
     static const int n = [] {
-        int h = (int)std::thread::hardware_concurrency();
+        // the CPUs this process may run on (a container's share), not every
+        // CPU of the machine
+        int h = 0;
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) h = CPU_COUNT(&set);
+        if (h <= 0) h = (int)std::thread::hardware_concurrency();
         if (h <= 0) h = 4;
         if (const char* e = std::getenv("MR_HOST_THREADS")) {
             const int v = std::atoi(e);

@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 
 #include "bucket_heap.hpp"
@@ -343,31 +345,8 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     MR_HIP_TRY(k::pg_init(n, P.start_columns ? tp.as<int32_t>() : m->rp.as<int32_t>(), score.as<uint32_t>(),
                           T0.as<int32_t>(), st, chunk.as<int32_t>(), s));
     int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
-    int cur = 0;
-    for (int32_t b = 0; b < B; ++b) {
-        if (b > 0 && b % compact_every == 0)
-            MR_HIP_TRY(k::pg_compact(n, tp.as<int32_t>(), ti_live.as<int32_t>(), live.as<int32_t>(),
-                                     score.as<uint32_t>(), s));
-        MR_HIP_TRY(k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(),
-                               ti_live.as<int32_t>(), score.as<uint32_t>(), Tl, st, sure.as<int32_t>(),
-                               tie.as<int32_t>(), hist.as<int32_t>(), chunk.as<int32_t>(), placement.as<uint32_t>(),
-                               wprefix.as<int32_t>(), hcol.as<int32_t>(), ylist.as<int32_t>(), live.as<int32_t>(),
-                               P.Ps[(size_t)b], P.Pl[(size_t)b], P.Os[(size_t)b], P.Ol[(size_t)b], P.hub_cap, s));
-        cur ^= 1;
-    }
-    std::vector<uint32_t> order((size_t)n);
-    MR_HIP_TRY(hipMemcpyAsync(order.data(), placement.p, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
-    MR_HIP_TRY(hipStreamSynchronize(s));
-    const double gpu_ms = ms_since(t_gpu);
-    if (gpu_order) std::memcpy(gpu_order, order.data(), sizeof(uint32_t) * (size_t)n);
-    {
-        std::vector<int32_t> chk(order.begin(), order.end());
-        if (mr_check_perm(chk.data(), n) != MR_OK)
-            return set_error(MR_ERR_NUMERIC, "Gorder GPU loop produced an invalid placement");
-    }
 
-    // ---- CPU phase: order_on_cpu per slice ----
-    const auto t_cpu = clk::now();
+    // ---- host copies for the CPU phase (the matrix does not change) ----
     std::vector<int32_t> hrp((size_t)n + 1), hci((size_t)std::max<int64_t>(nnz, 1)), htp((size_t)n + 1),
         hti((size_t)std::max<int64_t>(nnz, 1));
     MR_HIP_TRY(hipMemcpyAsync(hrp.data(), m->rp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
@@ -377,31 +356,148 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
         MR_HIP_TRY(hipMemcpyAsync(hti.data(), ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
     }
     MR_HIP_TRY(hipStreamSynchronize(s));
+
+    // ---- CPU phase overlapped with the GPU loop ----
+    // A slice is final once the GPU has placed every batch that writes into
+    // it; the loop copies the placement to pinned host memory every kChunk
+    // batches and records an event, and host threads refine each slice
+    // (order_on_cpu) as soon as the event covering it has completed.
+    struct Pinned {
+        uint32_t* p = nullptr;
+        ~Pinned() { if (p) hipHostFree(p); }
+    } ghost;
+    MR_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ghost.p), sizeof(uint32_t) * (size_t)n, hipHostMallocDefault));
+    std::vector<uint32_t> order((size_t)n);
     const int32_t C = (int32_t)P.Cs.size();
+    std::vector<int32_t> ready((size_t)C, 0);   // batch after which slice c is final
+    for (int32_t c = 0, b2 = 0; c < C; ++c) {
+        const int64_t last = (int64_t)P.Cs[(size_t)c] + P.Cl[(size_t)c] - 1;
+        while (b2 + 1 < B && (int64_t)P.Ps[(size_t)b2] + P.Pl[(size_t)b2] - 1 < last) ++b2;
+        ready[(size_t)c] = b2;
+    }
     std::vector<int32_t> local_idx((size_t)n, -1);
-    for (int32_t c = 0; c < C; ++c)
-        for (int32_t i = 0; i < P.Cl[(size_t)c]; ++i) local_idx[(size_t)order[(size_t)(P.Cs[(size_t)c] + i)]] = i;
     PgCpuShared g{hrp.data(), hci.data(), htp.data(), hti.data(), local_idx.data(), P.cpu_window,
                   (int32_t)(std::sqrt((double)n) / 4)};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::pair<int32_t, hipEvent_t>> published;   // (last batch copied, event)
+    bool abort_all = false;
     std::atomic<int32_t> next{0};
-    const int nth = std::max(1, std::min<int>(host_threads(), C));
+    std::atomic<int64_t> cpu_busy_us{0};
+    const int device = m->ctx->device;
+    auto worker = [&] {
+        hipSetDevice(device);
+        PgCpuWorker w;
+        w.yslot.assign((size_t)n, -1);
+        for (int32_t c = next++; c < C; c = next++) {
+            hipEvent_t ev = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] {
+                    if (abort_all) return true;
+                    for (const auto& pe : published)
+                        if (pe.first >= ready[(size_t)c]) { ev = pe.second; return true; }
+                    return false;
+                });
+                if (abort_all) return;
+            }
+            hipEventSynchronize(ev);
+            const auto t0 = clk::now();
+            const int32_t cs = P.Cs[(size_t)c], cl = P.Cl[(size_t)c];
+            for (int32_t i = 0; i < cl; ++i) {
+                order[(size_t)(cs + i)] = ghost.p[cs + i];
+                local_idx[(size_t)ghost.p[cs + i]] = i;
+            }
+            order_batch(g, w, order.data() + cs, cl);
+            cpu_busy_us += (int64_t)(ms_since(t0) * 1000.0);
+        }
+    };
+    // one core stays with the thread that launches the GPU loop
+    const int nth = C > 0 ? std::max(1, std::min<int>(host_threads() - 1, C)) : 0;
     std::vector<std::thread> pool;
-    for (int t = 0; t < nth && C > 0; ++t)
-        pool.emplace_back([&] {
-            PgCpuWorker w;
-            w.yslot.assign((size_t)n, -1);
-            for (int32_t c = next++; c < C; c = next++)
-                order_batch(g, w, order.data() + P.Cs[(size_t)c], P.Cl[(size_t)c]);
-        });
-    for (auto& th : pool) th.join();
-    const double cpu_ms = ms_since(t_cpu);
+    for (int t = 0; t < nth; ++t) pool.emplace_back(worker);
+    auto finish = [&](bool ok) {
+        if (!ok) {
+            std::lock_guard<std::mutex> lk(mu);
+            abort_all = true;
+        }
+        cv.notify_all();
+        for (auto& th : pool) th.join();
+        pool.clear();
+        for (auto& pe : published) hipEventDestroy(pe.second);
+        published.clear();
+    };
+    constexpr int32_t kChunk = 32;
+    int64_t copied = 0;
+    auto publish = [&](int32_t b) -> int {
+        // batch 0 of the gorder6 schedule keeps |P| = 32 even when n < 32
+        const int64_t upto = std::min<int64_t>((int64_t)P.Ps[(size_t)b] + P.Pl[(size_t)b], n);
+        if (upto > copied) {
+            const hipError_t e = hipMemcpyAsync(ghost.p + copied, placement.as<uint32_t>() + copied,
+                                                sizeof(uint32_t) * (size_t)(upto - copied), hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) return set_error(MR_ERR_HIP, "placement copy failed: %s", hipGetErrorString(e));
+            copied = upto;
+        }
+        hipEvent_t ev;
+        // blocking sync: waiting host threads sleep instead of spinning beside
+        // the thread that launches the loop
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventBlockingSync) != hipSuccess ||
+            hipEventRecord(ev, s) != hipSuccess)
+            return set_error(MR_ERR_HIP, "event for the placement copy failed");
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            published.emplace_back(b, ev);
+        }
+        cv.notify_all();
+        return MR_OK;
+    };
+
+    int cur = 0;
+    int rc = MR_OK;
+    for (int32_t b = 0; b < B && rc == MR_OK; ++b) {
+        hipError_t e = hipSuccess;
+        if (b > 0 && b % compact_every == 0)
+            e = k::pg_compact(n, tp.as<int32_t>(), ti_live.as<int32_t>(), live.as<int32_t>(), score.as<uint32_t>(), s);
+        if (e == hipSuccess)
+            e = k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(), ti_live.as<int32_t>(),
+                            score.as<uint32_t>(), Tl, st, sure.as<int32_t>(), tie.as<int32_t>(), hist.as<int32_t>(),
+                            chunk.as<int32_t>(), placement.as<uint32_t>(), wprefix.as<int32_t>(), hcol.as<int32_t>(),
+                            ylist.as<int32_t>(), live.as<int32_t>(), P.Ps[(size_t)b], P.Pl[(size_t)b],
+                            P.Os[(size_t)b], P.Ol[(size_t)b], P.hub_cap, s);
+        if (e != hipSuccess) rc = set_error(MR_ERR_HIP, "pGOrder batch %d failed: %s", b, hipGetErrorString(e));
+        cur ^= 1;
+        if (rc == MR_OK && (b % kChunk == kChunk - 1 || b == B - 1)) rc = publish(b);
+    }
+    if (rc == MR_OK && hipStreamSynchronize(s) != hipSuccess) rc = set_error(MR_ERR_HIP, "pGOrder loop failed");
+    const double gpu_ms = ms_since(t_gpu);
+    if (rc == MR_OK) {
+        std::vector<int32_t> chk(ghost.p, ghost.p + n);
+        if (mr_check_perm(chk.data(), n) != MR_OK)
+            rc = set_error(MR_ERR_NUMERIC, "Gorder GPU loop produced an invalid placement");
+    }
+    const auto t_tail = clk::now();
+    finish(rc == MR_OK);
+    if (rc != MR_OK) return rc;
+    const double tail_ms = ms_since(t_tail);
+    // positions outside every CPU slice keep the GPU placement
+    {
+        std::vector<uint8_t> in_slice((size_t)n, 0);
+        for (int32_t c = 0; c < C; ++c)
+            std::fill(in_slice.begin() + P.Cs[(size_t)c], in_slice.begin() + P.Cs[(size_t)c] + P.Cl[(size_t)c], 1);
+        for (int32_t i = 0; i < n; ++i)
+            if (!in_slice[(size_t)i]) order[(size_t)i] = ghost.p[i];
+    }
+    if (gpu_order) std::memcpy(gpu_order, ghost.p, sizeof(uint32_t) * (size_t)n);
     if (final_order) std::memcpy(final_order, order.data(), sizeof(uint32_t) * (size_t)n);
     if (stats) {
         stats->batches = B;
         stats->gpu_ms = gpu_ms;
-        stats->cpu_ms = cpu_ms;
+        // host refinement: summed busy time over the threads that overlapped
+        // the GPU loop; the part after the loop is in total_ms
+        stats->cpu_ms = (double)cpu_busy_us.load() / 1000.0 / std::max(1, nth);
         stats->total_ms = ms_since(t_all);
     }
+    (void)tail_ms;
     return MR_OK;
 }
 
