@@ -17,8 +17,10 @@ struct PgState {
 };
 
 int pg_hist_bins();
+// start counts, the first touched list and its count histogram (hist must be
+// zero on entry; each batch clears it and builds the next batch's)
 hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, PgState* st, int32_t* chunk_left,
-                   hipStream_t s);
+                   int32_t* hist, hipStream_t s);
 // one batch: select |P| (pl) at ps, then score the window [os, os+ol)
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,

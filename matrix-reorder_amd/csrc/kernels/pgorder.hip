@@ -17,8 +17,9 @@
 // The reference's per-batch full sort and full reset are O(n) per batch
 // (quadratic overall).  After a reset only vertices touched by the window have
 // a non-zero count, so this build keeps the touched set T of the last window
-// (appended on the 0 -> 1 transition of a count) and selects exactly:
-//   |T| <= 4096       : one workgroup sorts T in LDS;
+// (the unplaced vertices whose count it made non-zero, listed by one scan of
+// the counts) and selects exactly:
+//   |T| <= 4096       : one workgroup sorts T (a register bitonic network);
 //   otherwise         : a count histogram (exact below 65536) finds the
 //                       threshold bin, candidates above it are collected, and
 //                       the tie bin is resolved by an id histogram when it is
@@ -26,9 +27,12 @@
 //   |T| < |P(b)|      : the rest comes from untouched unplaced vertices in
 //                       ascending id order (count 0 ties), found by a
 //                       workgroup scan of per-1024-id unplaced counters.
-// Only T is reset between batches.  Batch 0 starts from count = the start
-// lengths (row lengths for gorder6, column counts for gorder5) with T = all
-// vertices whose start length is non-zero.  The placement is identical to
+// Only T is reset between batches.  A batch is five dependent launches:
+// candidates (with the threshold search), selection + placement (one
+// workgroup), the window's column pass (with the reset of T's counts), its row
+// pass, and the next touched list (with its count histogram).  Batch 0 starts
+// from count = the start lengths (row lengths for gorder6, column counts for
+// gorder5) with T = all vertices whose start length is non-zero.  The placement is identical to
 // the reference's for every input (the oracle restates the full-sort loop and
 // the tests compare the two bit for bit).
 #include "dtypes.hpp"
@@ -49,6 +53,7 @@ constexpr int kChunk = 1024;      // ids per unplaced counter
 constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
 constexpr int kWindowGrid = 2048; // blocks of the window-score kernel (8192 waves)
 constexpr int kSureCap = 4096;    // capacity of the candidates-above-threshold list
+constexpr int kTouchedPer = 16;   // ids per thread of the touched-list kernel
 
 __device__ __forceinline__ int count_bin(uint32_t c)
 {
@@ -79,19 +84,85 @@ __device__ __forceinline__ void wave_append(bool pred, int32_t v, int32_t* list,
     if (pred && at < cap) list[at] = v;
 }
 
-__device__ void lds_bitonic_asc(uint64_t* a, int P)
+// Ascending bitonic sort of 256 * R keys held in registers by a 256-thread
+// block: wave w holds elements e = w * 64R + 64r + lane in v[r].  A
+// compare-exchange at distance j < 64 is a lane shuffle, 64 <= j < 64R
+// swaps two registers of the same lane, and only j >= 64R (the last
+// log2(4) merge levels) goes through LDS with a block barrier -- three
+// barrier stages instead of one per stage of the network.
+__device__ __forceinline__ uint64_t cx_keep(uint64_t x, uint64_t y, bool keep_min)
 {
-    for (int k2 = 2; k2 <= P; k2 <<= 1) {
+    const bool lt = x < y;
+    return keep_min ? (lt ? x : y) : (lt ? y : x);
+}
+
+template <int R, int JJ>
+__device__ __forceinline__ void sort_lane_stage(uint64_t (&v)[R], int k2, int ebase)
+{
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (r & JJ) continue;
+        const bool up = ((ebase + 64 * r) & k2) == 0;
+        const uint64_t a = v[r], b = v[r + JJ];
+        const bool sw = (a > b) == up;
+        v[r] = sw ? b : a;
+        v[r + JJ] = sw ? a : b;
+    }
+}
+
+template <int R>
+__device__ void block_sort_regs(uint64_t (&v)[R], uint64_t* x)
+{
+    constexpr int S = 64 * R, N = 256 * R;
+    const int lane = threadIdx.x & 63;
+    const int ebase = (threadIdx.x >> 6) * S + lane;
+    for (int k2 = 2; k2 <= N; k2 <<= 1) {
         for (int j = k2 >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P / 2; i += blockDim.x) {
-                const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                const int hi = lo | j;
-                const uint64_t x = a[lo], y = a[hi];
-                const bool up = (lo & k2) == 0;
-                if ((x > y) == up) { a[lo] = y; a[hi] = x; }
+            if (j >= S) {
+                __syncthreads();   // earlier reads of x are done
+#pragma unroll
+                for (int r = 0; r < R; ++r) x[ebase + 64 * r] = v[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = ebase + 64 * r;
+                    v[r] = cx_keep(v[r], x[e ^ j], ((e & j) == 0) == ((e & k2) == 0));
+                }
+            } else if (j >= 64) {
+                const int jj = j >> 6;
+                if constexpr (R > 1) if (jj == 1) sort_lane_stage<R, 1>(v, k2, ebase);
+                if constexpr (R > 2) if (jj == 2) sort_lane_stage<R, 2>(v, k2, ebase);
+                if constexpr (R > 4) if (jj == 4) sort_lane_stage<R, 4>(v, k2, ebase);
+                if constexpr (R > 8) if (jj == 8) sort_lane_stage<R, 8>(v, k2, ebase);
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t y = __shfl_xor(v[r], j, 64);
+                    v[r] = cx_keep(v[r], y, lower == (((ebase + 64 * r) & k2) == 0));
+                }
             }
-            __syncthreads();
         }
+    }
+}
+
+// sort keys[0, cnt) (cnt <= 256 R) and write the low words of the `take`
+// smallest to sel[base, base + take); keys[] is the LDS exchange buffer
+template <int R>
+__device__ void sort_take_regs(uint64_t* keys, int cnt, int take, int32_t* sel, int base)
+{
+    const int ebase = (threadIdx.x >> 6) * (64 * R) + (threadIdx.x & 63);
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        v[r] = e < cnt ? keys[e] : ~0ull;
+    }
+    block_sort_regs<R>(v, keys);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        if (e < take) sel[base + e] = (int32_t)(v[r] & 0xffffffffu);
     }
 }
 
@@ -132,9 +203,9 @@ __global__ void init_kernel(int64_t n, const int32_t* __restrict__ rp, uint32_t*
     }
 }
 
-// count histogram of T: hist[0, kBins) per count bin, hist[kBins + c] per
-// coarse bin c of 512 count bins (the threshold search reads the coarse bins
-// first)
+// count histogram of the first touched list: hist[0, kBins) per count bin,
+// hist[kBins + c] per coarse bin c of 512 count bins (the threshold search
+// reads the coarse bins first); later lists get theirs from touched_kernel
 __global__ __launch_bounds__(256) void hist_kernel(const int32_t* __restrict__ T, const PgState* __restrict__ st,
                                                    int cur, const uint32_t* __restrict__ score,
                                                    int32_t* __restrict__ hist)
@@ -177,21 +248,25 @@ __device__ __forceinline__ int64_t block_suffix(int64_t v, int64_t* s_tot)
     return v + higher;
 }
 
-// the threshold bin tb: the bins above it hold fewer than pl vertices, with it
-// at least pl.  Two coalesced passes: the coarse bins, then the 512 bins of
-// the coarse bin that holds the threshold.
-__global__ __launch_bounds__(256) void thresh_kernel(const int32_t* __restrict__ hist, PgState* __restrict__ st,
-                                                     int cur, int32_t pl)
+// The threshold bin tb: the bins above it hold fewer than pl vertices, with it
+// at least pl.  Two coalesced passes over the histogram: the coarse bins,
+// then the 512 bins of the coarse bin that holds the threshold.  Every
+// thread of the block returns the same (tb, above, need).
+struct Thresh {
+    int32_t tb, above, need;
+};
+
+__device__ Thresh find_threshold(const int32_t* __restrict__ hist, int32_t tn, int32_t pl)
 {
-    const int32_t tn = st->tcount[cur];
-    if (tn <= kSmallT) return;
     __shared__ int64_t s_tot[4];
     __shared__ int s_c;
     __shared__ int64_t s_above_c;
+    __shared__ Thresh s_th;
     const int t = threadIdx.x;
-    if (pl <= 0 || tn < pl) {
-        if (t == 0) { st->tb = -1; st->above = tn; st->need = 0; }
-        return;
+    if (pl <= 0 || tn < pl) return Thresh{-1, tn, 0};
+    if (t == 0) {
+        s_c = 0;
+        s_th = Thresh{-1, tn, 0};
     }
     // coarse: the highest coarse bin c with suffix(c) >= pl
     const int64_t c_cnt = t < kCoarse ? hist[kBins + t] : 0;
@@ -211,30 +286,32 @@ __global__ __launch_bounds__(256) void thresh_kernel(const int32_t* __restrict__
     const int64_t suf = block_suffix(h0 + h1, s_tot) + above_c;   // bins >= b0
     // bin b1 first (the higher count), then b0
     const int64_t above1 = suf - h0 - h1;
-    if (above1 < pl && above1 + h1 >= pl) {
-        st->tb = b1;
-        st->above = (int32_t)above1;
-        st->need = (int32_t)(pl - above1);
-    } else if (above1 + h1 < pl && suf >= pl) {
-        st->tb = b0;
-        st->above = (int32_t)(above1 + h1);
-        st->need = (int32_t)(pl - above1 - h1);
-    }
+    if (above1 < pl && above1 + h1 >= pl)
+        s_th = Thresh{b1, (int32_t)above1, (int32_t)(pl - above1)};
+    else if (above1 + h1 < pl && suf >= pl)
+        s_th = Thresh{b0, (int32_t)(above1 + h1), (int32_t)(pl - above1 - h1)};
+    __syncthreads();
+    return s_th;
 }
 
+// candidates of a large touched list: every block finds the threshold bin
+// itself from the histogram (a few KB read per block, no launch of its own),
+// block 0 records it for final_kernel, and the vertices above it (sure) and
+// in it (tie) are listed
 __global__ __launch_bounds__(256) void collect_kernel(const int32_t* __restrict__ T, PgState* __restrict__ st,
                                                       int cur, const uint32_t* __restrict__ score,
                                                       int32_t* __restrict__ sure, int32_t* __restrict__ tie,
-                                                      int32_t pl, int32_t* __restrict__ hist)
+                                                      int32_t pl, const int32_t* __restrict__ hist)
 {
     const int32_t tn = st->tcount[cur];
-    if (tn <= kSmallT) return;
-    // the histogram was consumed by thresh_kernel: clear it for the next batch
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < kBins + kCoarse;
-         i += (int64_t)gridDim.x * blockDim.x)
-        hist[i] = 0;
-    if (pl <= 0) return;
-    const int tb = st->tb;
+    if (tn <= kSmallT || pl <= 0) return;
+    const Thresh th = find_threshold(hist, tn, pl);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->tb = th.tb;
+        st->above = th.above;
+        st->need = th.need;
+    }
+    const int tb = th.tb;
     const int64_t lim = ((int64_t)tn + 63) / 64 * 64;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t v = 0;
@@ -248,12 +325,69 @@ __global__ __launch_bounds__(256) void collect_kernel(const int32_t* __restrict_
     }
 }
 
-// one workgroup: final selection, fill with untouched vertices, placement,
-// and clearing of the per-batch scratch
+// The ids a[0, na) then b[0, cnt - na) (cnt <= 256 R), keyed by (count
+// descending, id ascending) from score[], sorted, and the `take` first written
+// to sel[base, base + take).  Every global load is issued unconditionally at a
+// clamped, in-bounds index (a[0], b[0] and score[0] exist), so the R loads of
+// each kind go out together: one workgroup waits two memory latencies, not
+// two per element.
+// An all-ones / all-zero mask the optimiser cannot see through.  Gating a
+// loaded value with it keeps the load unconditional: a plain select on the
+// value lets the compiler branch around the load instead, one memory round
+// trip per element.
+__device__ __forceinline__ uint32_t opaque_mask(bool f)
+{
+    uint32_t m = f ? 0xffffffffu : 0u;
+    asm volatile("" : "+v"(m));
+    return m;
+}
+
+template <int R>
+__device__ void select_sorted_r(int cnt, int take, const int32_t* __restrict__ a, int na,
+                                const int32_t* __restrict__ b, const uint32_t* __restrict__ score, uint64_t* x,
+                                int32_t* sel, int base)
+{
+    const int ebase = (threadIdx.x >> 6) * (64 * R) + (threadIdx.x & 63);
+    // the masks first: the empty asm statements bound the scheduler's regions
+    uint32_t am[R], okm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        am[r] = opaque_mask(e < na);
+        okm[r] = opaque_mask(e < cnt);
+    }
+    uint32_t id[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        const uint32_t ia = (uint32_t)a[e < na ? e : 0];
+        const uint32_t ib = (uint32_t)b[(e >= na && e < cnt) ? e - na : 0];
+        id[r] = ((ia & am[r]) | (ib & ~am[r])) & okm[r];   // 0 for padding
+    }
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t sc = score[id[r]];
+        // padding: count 0 and id ~0, the key ~0 (sorts last)
+        v[r] = sel_key((sc & ~kUnplaced) & okm[r], id[r] | ~okm[r]);
+    }
+    block_sort_regs<R>(v, x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        if (e < take) sel[base + e] = (int32_t)(v[r] & 0xffffffffu);
+    }
+}
+
+// One workgroup: final selection, fill with untouched vertices, placement,
+// clearing of the per-batch scratch and the prefix of the window's row
+// lengths.  Each phase issues its global loads together (a single workgroup
+// is latency-bound: a chain of dependent loads per element would cost one
+// memory round trip each).
 __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __restrict__ T, int cur,
                                                     PgState* __restrict__ st, uint32_t* __restrict__ score,
                                                     const int32_t* __restrict__ sure, const int32_t* __restrict__ tie,
-                                                    int32_t* __restrict__ hist, int32_t* __restrict__ chunk_left,
+                                                    int32_t* __restrict__ chunk_left,
                                                     uint32_t* __restrict__ placement, int32_t ps, int32_t pl,
                                                     const int32_t* __restrict__ rp, int32_t os, int32_t ol,
                                                     int32_t* __restrict__ wprefix, int32_t hub_cap)
@@ -264,8 +398,12 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
     __shared__ int32_t s_nsel;
     __shared__ int32_t s_bins[4096];
     __shared__ int32_t s_misc[4];
-    const int tid = threadIdx.x;
+    __shared__ uint64_t s_ball[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // every bookkeeping field this kernel reads, loaded once (tb and need are
+    // meaningful only after collect_kernel ran, i.e. for a large list)
     const int32_t tn = st->tcount[cur];
+    const int32_t nsure = st->nsure, ntie = st->ntie, tb = st->tb, need = st->need, lo_chunk0 = st->lo_chunk;
     const int32_t want = pl > 0 ? (pl < 1024 ? pl : 1024) : 0;
     if (tid == 0) {
         s_nsel = 0;
@@ -274,14 +412,27 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
     }
     __syncthreads();
 
-    auto sort_take = [&](int cnt, int take) {
-        int P = 2;
-        while (P < cnt) P <<= 1;
-        for (int i = cnt + tid; i < P; i += blockDim.x) keys[i] = ~0ull;
-        __syncthreads();
-        lds_bitonic_asc(keys, P);
+    // the `take` best of a[0, na) ++ b[0, cnt - na) appended to sel
+    auto select_sorted = [&](int cnt, int take, const int32_t* a, int na, const int32_t* b) {
         const int base = s_nsel;
-        for (int i = tid; i < take; i += blockDim.x) sel[base + i] = (int32_t)(keys[i] & 0xffffffffu);
+        if (cnt <= 256) select_sorted_r<1>(cnt, take, a, na, b, score, keys, sel, base);
+        else if (cnt <= 512) select_sorted_r<2>(cnt, take, a, na, b, score, keys, sel, base);
+        else if (cnt <= 1024) select_sorted_r<4>(cnt, take, a, na, b, score, keys, sel, base);
+        else if (cnt <= 2048) select_sorted_r<8>(cnt, take, a, na, b, score, keys, sel, base);
+        else select_sorted_r<16>(cnt, take, a, na, b, score, keys, sel, base);
+        __syncthreads();
+        if (tid == 0) s_nsel = base + take;
+        __syncthreads();
+    };
+    // the `take` smallest of keys[0, cnt) (cnt <= kSmallT) appended to sel
+    auto sort_take = [&](int cnt, int take) {
+        __syncthreads();
+        const int base = s_nsel;
+        if (cnt <= 256) sort_take_regs<1>(keys, cnt, take, sel, base);
+        else if (cnt <= 512) sort_take_regs<2>(keys, cnt, take, sel, base);
+        else if (cnt <= 1024) sort_take_regs<4>(keys, cnt, take, sel, base);
+        else if (cnt <= 2048) sort_take_regs<8>(keys, cnt, take, sel, base);
+        else sort_take_regs<16>(keys, cnt, take, sel, base);
         __syncthreads();
         if (tid == 0) s_nsel = base + take;
         __syncthreads();
@@ -289,32 +440,16 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
 
     if (want > 0) {
         if (tn <= kSmallT) {
-            for (int i = tid; i < tn; i += blockDim.x) {
-                const int32_t v = T[i];
-                keys[i] = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
-            }
-            __syncthreads();
-            sort_take(tn, tn < want ? tn : want);
+            select_sorted(tn, tn < want ? tn : want, T, tn, T);
         } else {
-            const int32_t ns = min(st->nsure, kSureCap), nt = st->ntie, need = st->need;
+            const int32_t ns = min(nsure, kSureCap), nt = ntie;
             if (ns + nt <= kSmallT) {
-                for (int i = tid; i < ns + nt; i += blockDim.x) {
-                    const int32_t v = i < ns ? sure[i] : tie[i - ns];
-                    keys[i] = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
-                }
-                __syncthreads();
-                const int take = ns + nt < want ? ns + nt : want;
-                sort_take(ns + nt, take);
+                select_sorted(ns + nt, ns + nt < want ? ns + nt : want, sure, ns, tie);
             } else {
                 // all sure candidates first (they beat every tie)
-                for (int i = tid; i < ns; i += blockDim.x) {
-                    const int32_t v = sure[i];
-                    keys[i] = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
-                }
-                __syncthreads();
-                sort_take(ns, ns);
+                select_sorted(ns, ns, sure, ns, sure);
                 // the `need` best ties
-                if (st->tb < kExactBins) {
+                if (tb < kExactBins) {
                     // equal counts: the `need` smallest ids, via an id histogram
                     int sh = 0;
                     while (((n - 1) >> sh) >= 4096) ++sh;
@@ -345,7 +480,7 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
                     const int cnt = s_misc[1];
                     if (cnt <= kSmallT) sort_take(cnt, need);
                 }
-                if (st->tb >= kExactBins || (st->tb < kExactBins && s_misc[1] > kSmallT)) {
+                if (tb >= kExactBins || s_misc[1] > kSmallT) {
                     // rare general case: repeated maximum over the tie list
                     uint64_t last = 0;
                     for (int r = 0; r < need; ++r) {
@@ -386,35 +521,95 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
         atomicSub(&chunk_left[v / kChunk], 1);
     }
     __syncthreads();
-    // fill with untouched unplaced vertices in ascending id order
+    // fill with untouched unplaced vertices in ascending id order.  Each pass
+    // reads the counters of the next 256 chunks at once and scans the first
+    // four that still hold unplaced vertices, one wave per chunk (16
+    // consecutive ids per lane), ranked in chunk and id order.
     int missing = want - nsel_t;
+    int filled = 0;
     if (missing > 0) {
         const int64_t nchunks = (n + kChunk - 1) / kChunk;
-        int64_t c = st->lo_chunk;
-        int filled = 0;
+        int64_t c = lo_chunk0;
+        int64_t lo_new = -1;   // first chunk with unplaced vertices
         while (missing > 0 && c < nchunks) {
-            const int left = chunk_left[c];
+            const int64_t cc = c + tid;
+            const bool nz = cc < nchunks && chunk_left[cc] > 0;
+            const uint64_t m = __ballot(nz);
+            if (lane == 0) s_ball[wid] = m;
             __syncthreads();
-            if (left == 0) {
-                if (tid == 0 && st->lo_chunk == c) st->lo_chunk = (int32_t)(c + 1);
-                __syncthreads();
-                ++c;
-                continue;
+            int pos[4] = {0, 0, 0, 0};
+            int np = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint64_t bb = s_ball[q];
+#pragma unroll
+                for (int k2 = 0; k2 < 4; ++k2) {
+                    const bool take_bit = bb != 0 && np < 4;
+                    const int at = q * 64 + __ffsll((unsigned long long)bb) - 1;
+                    if (take_bit) {
+                        pos[0] = np == 0 ? at : pos[0];
+                        pos[1] = np == 1 ? at : pos[1];
+                        pos[2] = np == 2 ? at : pos[2];
+                        pos[3] = np == 3 ? at : pos[3];
+                        ++np;
+                        bb &= bb - 1;
+                    }
+                }
             }
-            const int64_t base = c * kChunk;
-            for (int sub = 0; sub < kChunk && missing > 0; sub += 256) {
-                const int64_t id = base + sub + tid;
-                const bool f = id < n && score[id] == kUnplaced;
-                int rank = 0;
-                const int tot = block_rank(f, rank, s_w);
-                if (f && rank < missing) sel[nsel_t + filled + rank] = (int32_t)id;
-                const int got = tot < missing ? tot : missing;
-                filled += got;
-                missing -= got;
+            if (lo_new < 0 && np > 0) lo_new = c + pos[0];
+            const int my = wid == 0 ? pos[0] : wid == 1 ? pos[1] : wid == 2 ? pos[2] : pos[3];
+            uint32_t flags = 0;
+            int64_t idb = 0;
+            if (wid < np) {
+                idb = (c + my) * kChunk + lane * 16;
+                uint32_t sc[16];
+                if (idb + 16 <= n) {
+                    const uint4* p4 = reinterpret_cast<const uint4*>(score + idb);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint4 u = p4[q];
+                        sc[4 * q] = u.x;
+                        sc[4 * q + 1] = u.y;
+                        sc[4 * q + 2] = u.z;
+                        sc[4 * q + 3] = u.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) sc[j] = idb + j < n ? score[idb + j] : 0u;
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) flags |= (sc[j] == kUnplaced) ? (1u << j) : 0u;
             }
-            ++c;
+            const int cnt = __popc(flags);
+            int inc = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += o;
+            }
+            if (lane == 63) s_w[wid] = inc;
+            __syncthreads();
+            int off = inc - cnt, tot = 0;
+            for (int w = 0; w < 4; ++w) {
+                off += w < wid ? s_w[w] : 0;
+                tot += s_w[w];
+            }
+            int r = off;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if (flags & (1u << j)) {
+                    if (r < missing) sel[nsel_t + filled + r] = (int32_t)(idb + j);
+                    ++r;
+                }
+            }
+            __syncthreads();   // s_ball and s_w are rewritten by the next pass
+            const int got = tot < missing ? tot : missing;
+            filled += got;
+            missing -= got;
+            c = np == 4 ? c + pos[3] + 1 : c + 256;
         }
-        __syncthreads();
+        if (lo_new < 0) lo_new = c < nchunks ? c : nchunks;
+        if (tid == 0 && lo_new != lo_chunk0) st->lo_chunk = (int32_t)lo_new;
         for (int i = tid; i < filled; i += blockDim.x) {
             const int32_t v = sel[nsel_t + i];
             score[v] = 0u;
@@ -424,7 +619,7 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
     }
     __syncthreads();
     // clear per-batch scratch for the next batch (the histogram is cleared by
-    // collect_kernel)
+    // window_y_kernel)
     if (tid == 0) {
         st->nsure = 0;
         st->ntie = 0;
@@ -432,31 +627,34 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
         st->tcount[cur ^ 1] = 0;
     }
     // prefix sums of the (x, y) pair counts of the window O(b): wprefix[w] =
-    // pairs of window entries before w (rows at or above the hub cap count 0)
-    __syncthreads();
+    // pairs of window entries before w (rows at or above the hub cap count 0).
+    // ol <= 1024: four consecutive entries per thread.  When the window is the
+    // batch just placed (the steady state of the gorder6 schedule) its
+    // vertices are still in sel[].
     if (ol > 0) {
-        int32_t* s_len = s_bins;   // reuse: ol <= 1024 entries
-        for (int w = tid; w < ol; w += blockDim.x) {
-            const uint32_t xu = placement[os + w];
-            int32_t len = 0;
-            if (xu < (uint32_t)n) {
-                len = rp[xu + 1] - rp[xu];
-                if (len >= hub_cap) len = 0;
-            }
-            s_len[w] = len;
-        }
-        __syncthreads();
-        // exclusive prefix over ol <= 1024 lengths: four consecutive entries
-        // per thread, then a block scan of the per-thread sums
-        int32_t mine[4];
-        int32_t sum = 0;
+        const bool from_sel = os == ps && ol <= nsel_t + filled;
+        int32_t xs[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int w = 4 * tid + j;
-            mine[j] = w < ol ? s_len[w] : 0;
-            sum += mine[j];
+            const int wc = w < ol ? w : 0;
+            xs[j] = from_sel ? sel[wc] : (int32_t)placement[os + wc];
         }
-        const int lane = tid & 63, wid = tid >> 6;
+        int32_t mine[4];
+        int32_t sum = 0;
+        uint32_t okm[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)   // unplaced positions hold ~0
+            okm[j] = opaque_mask(4 * tid + j < ol && (uint32_t)xs[j] < (uint32_t)n);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int32_t xi = (int32_t)((uint32_t)xs[j] & okm[j]);
+            const int32_t r0 = rp[xi], r1 = rp[xi + 1];
+            int32_t len = (int32_t)((uint32_t)(r1 - r0) & okm[j]);
+            if (len >= hub_cap) len = 0;
+            mine[j] = len;
+            sum += len;
+        }
         int32_t inc = sum;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -479,16 +677,6 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
     }
 }
 
-__global__ void reset_kernel(const int32_t* __restrict__ T, const PgState* __restrict__ st, int cur,
-                             uint32_t* __restrict__ score)
-{
-    const int32_t tn = st->tcount[cur];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tn; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t v = T[i];
-        if (score[v] != 0u) score[v] = kUnplaced;
-    }
-}
-
 // window score (gorder6_kernel3, class.cu:2917-2953).  The reference adds one
 // to every unplaced z for every 2-hop path x -> y -> z (x in the window with a
 // row below the hub cap, y in cols(x) with a column below the cap, z in
@@ -499,20 +687,31 @@ __global__ void reset_kernel(const int32_t* __restrict__ T, const PgState* __res
 // lists y when h[y] leaves 0; (2) one wave per listed y walks rows(y) once and
 // adds h[y].  Integer sums: identical counts, far fewer updates when window
 // rows share columns (hubs).
+// The kernel also resets the counts of the batch's touched list to the status
+// bit (placed vertices keep 0) and clears the count histogram, both consumed
+// by the selection: the window adds to zero counts.
 __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t* __restrict__ window, int32_t ol,
                                                        const int32_t* __restrict__ wprefix,
                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                        const int32_t* __restrict__ tp, int32_t* __restrict__ h,
                                                        int32_t* __restrict__ ylist, PgState* __restrict__ st,
-                                                       int32_t hub_cap)
+                                                       int32_t hub_cap, const int32_t* __restrict__ T, int cur,
+                                                       uint32_t* __restrict__ score, int32_t* __restrict__ hist)
 {
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t threads = (int64_t)gridDim.x * blockDim.x;
+    const int32_t tn = st->tcount[cur];
+    for (int64_t i = gtid; i < tn; i += threads) {
+        const int32_t v = T[i];
+        if (score[v] != 0u) score[v] = kUnplaced;
+    }
+    for (int64_t i = gtid; i < kBins + kCoarse; i += threads) hist[i] = 0;
     __shared__ int32_t s_pre[1025];
     for (int i = threadIdx.x; i <= ol; i += blockDim.x) s_pre[i] = wprefix[i];
     __syncthreads();
     const int32_t total = s_pre[ol];
-    const int64_t threads = (int64_t)gridDim.x * blockDim.x;
     const int64_t lim = ((int64_t)total + 63) / 64 * 64;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < lim; p += threads) {
+    for (int64_t p = gtid; p < lim; p += threads) {
         bool first = false;
         int32_t y = 0;
         if (p < total) {
@@ -566,41 +765,80 @@ __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict
     }
 }
 
-// the touched list of the next batch: every unplaced vertex whose count the
-// window made non-zero (all counts were 0 before the window).  Each block
-// owns a contiguous id range, counts it, reserves its slice of the list with
-// one atomic, then writes it in id order (a counter shared by every wave
-// would serialise tens of thousands of atomics per batch).
+// The touched list of the next batch: every unplaced vertex whose count the
+// window made non-zero (all counts were 0 before the window), and its count
+// histogram for the next selection (as hist_kernel builds it).  Each thread
+// owns kTouchedPer consecutive ids, read once as 16-byte vectors and kept in
+// registers; a block scan orders the block's entries, one atomic reserves the
+// block's slice of the list (a counter shared by every wave would serialise
+// tens of thousands of atomics per batch).
 __device__ __forceinline__ bool is_touched(uint32_t sc) { return (sc & kUnplaced) && sc != kUnplaced; }
 
 __global__ __launch_bounds__(256) void touched_kernel(int64_t n, const uint32_t* __restrict__ score,
-                                                      int32_t* __restrict__ T, PgState* __restrict__ st, int nxt)
+                                                      int32_t* __restrict__ T, PgState* __restrict__ st, int nxt,
+                                                      int32_t* __restrict__ hist)
 {
     __shared__ int s_w[4];
-    __shared__ int s_base;
-    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
-    const int64_t lo = (int64_t)blockIdx.x * per;
-    const int64_t hi = lo + per < n ? lo + per : n;
-    if (lo >= hi) return;                       // uniform over the block
-    int cnt = 0;
-    for (int64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) cnt += is_touched(score[v]) ? 1 : 0;
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = cnt;
+    __shared__ int s_base, s_tot;
+    __shared__ int32_t h[4096];
+    __shared__ int32_t hc[kCoarse];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t lo = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kTouchedPer;
+    uint32_t sc[kTouchedPer];
+    if (lo + kTouchedPer <= n) {
+        const uint4* p4 = reinterpret_cast<const uint4*>(score + lo);   // lo is a multiple of 16 ids
+#pragma unroll
+        for (int q = 0; q < kTouchedPer / 4; ++q) {
+            const uint4 u = p4[q];
+            sc[4 * q] = u.x;
+            sc[4 * q + 1] = u.y;
+            sc[4 * q + 2] = u.z;
+            sc[4 * q + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int k2 = 0; k2 < kTouchedPer; ++k2) sc[k2] = lo + k2 < n ? score[lo + k2] : 0u;
+    }
+    uint32_t flags = 0;
+#pragma unroll
+    for (int k2 = 0; k2 < kTouchedPer; ++k2) flags |= is_touched(sc[k2]) ? (1u << k2) : 0u;
+    const int cnt = __popc(flags);
+    // block exclusive scan of the per-thread counts
+    int inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) s_w[wid] = inc;
     __syncthreads();
+    int off = inc - cnt;
+    for (int w = 0; w < wid; ++w) off += s_w[w];
     if (threadIdx.x == 0) {
         const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
         s_base = tot ? atomicAdd(&st->tcount[nxt], tot) : 0;
+        s_tot = tot;
+    }
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) h[i] = 0;
+    for (int i = threadIdx.x; i < kCoarse; i += blockDim.x) hc[i] = 0;
+    __syncthreads();
+    if (s_tot == 0) return;   // uniform over the block
+    int at = s_base + off;
+#pragma unroll
+    for (int k2 = 0; k2 < kTouchedPer; ++k2) {
+        if (flags & (1u << k2)) {
+            T[at++] = (int32_t)(lo + k2);
+            const int b = count_bin(sc[k2] & ~kUnplaced);
+            if (b < 4096) atomicAdd(&h[b], 1);
+            else atomicAdd(&hist[b], 1);
+            atomicAdd(&hc[b >> kCoarseShift], 1);
+        }
     }
     __syncthreads();
-    int run = s_base;
-    for (int64_t c = lo; c < hi; c += blockDim.x) {
-        const int64_t v = c + threadIdx.x;
-        const bool f = v < hi && is_touched(score[v]);
-        int rank = 0;
-        const int tot = block_rank(f, rank, s_w);
-        if (f) T[run + rank] = (int32_t)v;
-        run += tot;
-    }
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+    for (int i = threadIdx.x; i < kCoarse; i += blockDim.x)
+        if (hc[i]) atomicAdd(&hist[kBins + i], hc[i]);
 }
 
 __global__ void live_init_kernel(int64_t n, const int32_t* __restrict__ tp, int32_t* __restrict__ live)
@@ -638,32 +876,36 @@ __global__ __launch_bounds__(256) void compact_kernel(int64_t n, const int32_t* 
 }  // namespace
 
 hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, PgState* st, int32_t* chunk_left,
-                   hipStream_t s)
+                   int32_t* hist, hipStream_t s)
 {
     int64_t g = (n + 255) / 256;
     if (g > 4096) g = 4096;
     if (g < 1) g = 1;
     init_kernel<<<(unsigned)g, 256, 0, s>>>(n, rp, score, t0, st, chunk_left);
+    hist_kernel<<<kGridStride, 256, 0, s>>>(t0, st, 0, score, hist);
     return hipGetLastError();
 }
 
+// Five dependent launches per batch: candidates (with the threshold search),
+// the one-workgroup selection and placement, the window's column pass (with
+// the reset of the batch's counts), its row pass, and the next touched list
+// (with its histogram).
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
                     int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
                     int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap, hipStream_t s)
 {
     const int nxt = cur ^ 1;
-    hist_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, hist);
-    thresh_kernel<<<1, 256, 0, s>>>(hist, st, cur, pl);
     collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl, hist);
-    final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, hist, chunk_left, placement, ps, pl, rp,
-                                   os, ol, wprefix, hub_cap);
-    reset_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score);
+    final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, chunk_left, placement, ps, pl, rp, os, ol,
+                                   wprefix, hub_cap);
+    window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st, hub_cap,
+                                                Tl[cur], cur, score, hist);
     if (ol > 0) {
-        window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st,
-                                                    hub_cap);
         window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, ylist, score, st);
-        touched_kernel<<<kGridStride, 256, 0, s>>>(n, score, Tl[nxt], st, nxt);
+        const int64_t per_block = 256 * (int64_t)kTouchedPer;
+        const int64_t g = n > 0 ? (n + per_block - 1) / per_block : 1;
+        touched_kernel<<<(unsigned)g, 256, 0, s>>>(n, score, Tl[nxt], st, nxt, hist);
     }
     return hipGetLastError();
 }

@@ -343,7 +343,7 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     MR_HIP_TRY(k::pg_live_init(n, tp.as<int32_t>(), live.as<int32_t>(), s));
     const int32_t compact_every = std::max<int32_t>(32, B / 32);
     MR_HIP_TRY(k::pg_init(n, P.start_columns ? tp.as<int32_t>() : m->rp.as<int32_t>(), score.as<uint32_t>(),
-                          T0.as<int32_t>(), st, chunk.as<int32_t>(), s));
+                          T0.as<int32_t>(), st, chunk.as<int32_t>(), hist.as<int32_t>(), s));
     int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
 
     // ---- host copies for the CPU phase (the matrix does not change) ----
