@@ -346,22 +346,34 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
                           T0.as<int32_t>(), st, chunk.as<int32_t>(), hist.as<int32_t>(), s));
     int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
 
-    // ---- host copies for the CPU phase (the matrix does not change) ----
-    std::vector<int32_t> hrp((size_t)n + 1), hci((size_t)std::max<int64_t>(nnz, 1)), htp((size_t)n + 1),
-        hti((size_t)std::max<int64_t>(nnz, 1));
-    MR_HIP_TRY(hipMemcpyAsync(hrp.data(), m->rp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
-    MR_HIP_TRY(hipMemcpyAsync(htp.data(), tp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    // ---- host copies of both adjacency lists for the CPU phase ----
+    // Pinned, so each is one DMA transfer queued on the stream ahead of the
+    // loop (the matrix does not change).  Every placement event the refining
+    // threads wait for comes later in the same stream, so no other wait is
+    // needed before they read the lists.
+    struct PinnedLists {
+        int32_t* p = nullptr;
+        ~PinnedLists() { if (p) hipHostFree(p); }
+    } lists;
+    const size_t nnz_a = (size_t)std::max<int64_t>(nnz, 1);
+    MR_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&lists.p),
+                             sizeof(int32_t) * (2 * ((size_t)n + 1) + 2 * nnz_a), hipHostMallocDefault));
+    int32_t* const hrp = lists.p;
+    int32_t* const htp = hrp + ((size_t)n + 1);
+    int32_t* const hci = htp + ((size_t)n + 1);
+    int32_t* const hti = hci + nnz_a;
+    MR_HIP_TRY(hipMemcpyAsync(hrp, m->rp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipMemcpyAsync(htp, tp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
     if (nnz > 0) {
-        MR_HIP_TRY(hipMemcpyAsync(hci.data(), m->ci.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
-        MR_HIP_TRY(hipMemcpyAsync(hti.data(), ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipMemcpyAsync(hci, m->ci.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipMemcpyAsync(hti, ti.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
     }
-    MR_HIP_TRY(hipStreamSynchronize(s));
 
     // ---- CPU phase overlapped with the GPU loop ----
     // A slice is final once the GPU has placed every batch that writes into
     // it; the loop copies the placement to pinned host memory every kChunk
     // batches and records an event, and host threads refine each slice
-    // (order_on_cpu) as soon as the event covering it has completed.
+    // (order_on_cpu) as soon as the placement covering it has landed.
     struct Pinned {
         uint32_t* p = nullptr;
         ~Pinned() { if (p) hipHostFree(p); }
@@ -376,32 +388,50 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
         ready[(size_t)c] = b2;
     }
     std::vector<int32_t> local_idx((size_t)n, -1);
-    PgCpuShared g{hrp.data(), hci.data(), htp.data(), hti.data(), local_idx.data(), P.cpu_window,
+    PgCpuShared g{hrp, hci, htp, hti, local_idx.data(), P.cpu_window,
                   (int32_t)(std::sqrt((double)n) / 4)};
     std::mutex mu;
     std::condition_variable cv;
     std::vector<std::pair<int32_t, hipEvent_t>> published;   // (last batch copied, event)
     bool abort_all = false;
+    int32_t done_batch = -1;   // the placement up to this batch is in ghost
     std::atomic<int32_t> next{0};
     std::atomic<int64_t> cpu_busy_us{0};
     const int device = m->ctx->device;
-    auto worker = [&] {
+    // One thread waits for the copy events in order and publishes how far the
+    // placement has landed; the refining threads make no HIP call (thousands
+    // of event waits from every thread contend in the runtime with the thread
+    // that launches the loop).
+    auto watcher = [&] {
         hipSetDevice(device);
+        size_t next_ev = 0;
+        for (;;) {
+            std::pair<int32_t, hipEvent_t> pe;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return abort_all || next_ev < published.size(); });
+                if (abort_all) return;
+                pe = published[next_ev++];
+            }
+            const bool ok = hipEventSynchronize(pe.second) == hipSuccess;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (ok) done_batch = pe.first;
+                else abort_all = true;
+            }
+            cv.notify_all();
+            if (!ok || pe.first >= B - 1) return;
+        }
+    };
+    auto worker = [&] {
         PgCpuWorker w;
         w.yslot.assign((size_t)n, -1);
         for (int32_t c = next++; c < C; c = next++) {
-            hipEvent_t ev = nullptr;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] {
-                    if (abort_all) return true;
-                    for (const auto& pe : published)
-                        if (pe.first >= ready[(size_t)c]) { ev = pe.second; return true; }
-                    return false;
-                });
+                cv.wait(lk, [&] { return abort_all || done_batch >= ready[(size_t)c]; });
                 if (abort_all) return;
             }
-            hipEventSynchronize(ev);
             const auto t0 = clk::now();
             const int32_t cs = P.Cs[(size_t)c], cl = P.Cl[(size_t)c];
             for (int32_t i = 0; i < cl; ++i) {
@@ -415,6 +445,7 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     // one core stays with the thread that launches the GPU loop
     const int nth = C > 0 ? std::max(1, std::min<int>(host_threads() - 1, C)) : 0;
     std::vector<std::thread> pool;
+    if (nth > 0) pool.emplace_back(watcher);
     for (int t = 0; t < nth; ++t) pool.emplace_back(worker);
     auto finish = [&](bool ok) {
         if (!ok) {
@@ -478,6 +509,7 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     const auto t_tail = clk::now();
     finish(rc == MR_OK);
     if (rc != MR_OK) return rc;
+    if (abort_all) return set_error(MR_ERR_HIP, "waiting for the pGOrder placement copy failed");
     const double tail_ms = ms_since(t_tail);
     // positions outside every CPU slice keep the GPU placement
     {
