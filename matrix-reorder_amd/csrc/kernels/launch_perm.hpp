@@ -28,12 +28,11 @@ hipError_t perm_sort_class(int cls, int32_t count, const int32_t* rows, const in
                            const int32_t* inv, const int32_t* new_rp, int32_t* new_ci, void* new_val,
                            uint64_t* scratch, const int64_t* scratch_off, hipStream_t s);
 
-hipError_t iota_i32(int64_t n, int32_t* out, hipStream_t s);
+// inv[perm[i]] = i; *bad becomes 1 unless perm is a permutation of [0, n)
+// (out-of-range entries are not written)
+hipError_t invert_perm(int64_t n, const int32_t* perm, int32_t* inv, int32_t* bad, hipStream_t s);
 // column counts of a CSR (cnt zeroed by the caller)
 hipError_t transpose_count(int64_t nnz, const int32_t* ci, int32_t* cnt, hipStream_t s);
-// row ids into column segments [tp[c], tp[c+1]) in arbitrary order (cursor zeroed)
-hipError_t transpose_fill(int64_t n_rows, const int32_t* rp, const int32_t* ci, const int32_t* tp, int32_t* cursor,
-                          int32_t* ti, hipStream_t s);
 
 }  // namespace k
 }  // namespace mr

@@ -244,31 +244,28 @@ hipError_t sort_class_t(int cls, int32_t count, const int32_t* rows, const int32
     return hipGetLastError();
 }
 
-__global__ void iota_kernel(int64_t n, int32_t* __restrict__ out)
+// inv[perm[i]] = i; an out-of-range entry sets *bad instead of writing
+__global__ void invert_kernel(int64_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ inv,
+                              int32_t* __restrict__ bad)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = perm[i];
+        if ((uint32_t)p < (uint32_t)n) inv[p] = (int32_t)i;
+        else *bad = 1;
+    }
+}
+
+// an entry of inv still at its -1 fill means perm repeats a value
+__global__ void inverse_holes_kernel(int64_t n, const int32_t* __restrict__ inv, int32_t* __restrict__ bad)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (int32_t)i;
+        if (inv[i] < 0) *bad = 1;
 }
 
 __global__ void col_count_kernel(int64_t nnz, const int32_t* __restrict__ ci, int32_t* __restrict__ cnt)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
         atomicAdd(&cnt[ci[i]], 1);
-}
-
-// scatter row ids into column segments (unordered inside a segment)
-__global__ void transpose_fill_kernel(int64_t n_rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                      const int32_t* __restrict__ tp, int32_t* __restrict__ cursor,
-                                      int32_t* __restrict__ ti)
-{
-    const int64_t lanes = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t g = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 8; g < n_rows; g += lanes / 8) {
-        const int32_t b = rp[g], e = rp[g + 1];
-        for (int32_t q = b + (threadIdx.x & 7); q < e; q += 8) {
-            const int32_t c = ci[q];
-            ti[tp[c] + atomicAdd(&cursor[c], 1)] = (int32_t)g;
-        }
-    }
 }
 
 }  // namespace
@@ -315,12 +312,16 @@ hipError_t perm_sort_class(int cls, int32_t count, const int32_t* rows, const in
     }
 }
 
-hipError_t iota_i32(int64_t n, int32_t* out, hipStream_t s)
+hipError_t invert_perm(int64_t n, const int32_t* perm, int32_t* inv, int32_t* bad, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     int64_t g = (n + 255) / 256;
     if (g > 65536) g = 65536;
-    iota_kernel<<<(unsigned)g, 256, 0, s>>>(n, out);
+    hipError_t e = hipMemsetAsync(inv, 0xff, sizeof(int32_t) * (size_t)n, s);
+    if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(int32_t), s);
+    if (e != hipSuccess) return e;
+    invert_kernel<<<(unsigned)g, 256, 0, s>>>(n, perm, inv, bad);
+    inverse_holes_kernel<<<(unsigned)g, 256, 0, s>>>(n, inv, bad);
     return hipGetLastError();
 }
 
@@ -330,16 +331,6 @@ hipError_t transpose_count(int64_t nnz, const int32_t* ci, int32_t* cnt, hipStre
     int64_t g = (nnz + 255) / 256;
     if (g > 65536) g = 65536;
     col_count_kernel<<<(unsigned)g, 256, 0, s>>>(nnz, ci, cnt);
-    return hipGetLastError();
-}
-
-hipError_t transpose_fill(int64_t n_rows, const int32_t* rp, const int32_t* ci, const int32_t* tp, int32_t* cursor,
-                          int32_t* ti, hipStream_t s)
-{
-    if (n_rows <= 0) return hipSuccess;
-    int64_t g = (n_rows * 8 + 255) / 256;
-    if (g > 65536) g = 65536;
-    transpose_fill_kernel<<<(unsigned)g, 256, 0, s>>>(n_rows, rp, ci, tp, cursor, ti);
     return hipGetLastError();
 }
 

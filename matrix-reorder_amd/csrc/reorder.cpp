@@ -19,6 +19,7 @@
 #include "kernels/launch_degree.hpp"
 #include "kernels/launch_perm.hpp"
 #include "kernels/launch_pg.hpp"
+#include "kernels/launch_transpose.hpp"
 #include "parallel.hpp"
 #include "reorder.hpp"
 
@@ -94,23 +95,10 @@ int gpu_transpose(mr_mat m, DevBuf& tp, DevBuf& ti, hipStream_t s)
     const int64_t n = m->n_rows, nc = m->n_cols, nnz = m->nnz;
     MR_TRY(tp.alloc(sizeof(int32_t) * (size_t)(nc + 1)));
     MR_TRY(ti.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
-    DevBuf cnt, cursor, raw, ident, dummy, ws;
-    MR_TRY(cnt.alloc(sizeof(int32_t) * (size_t)(nc + 1)));
-    MR_TRY(cursor.alloc(sizeof(int32_t) * (size_t)(nc + 1)));
-    MR_TRY(raw.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
-    MR_TRY(ident.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(std::max(n, nc), 1)));
-    MR_TRY(dummy.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
-    MR_TRY(ws.alloc(k::scan_workspace_bytes(nc + 1)));
-    MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
-    MR_HIP_TRY(hipMemsetAsync(cursor.p, 0, cursor.bytes, s));
-    MR_HIP_TRY(k::transpose_count(nnz, m->ci.as<int32_t>(), cnt.as<int32_t>(), s));
-    MR_HIP_TRY(k::exclusive_scan_i32(cnt.as<int32_t>(), nc + 1, tp.as<int32_t>(), nullptr, ws.p, s));
-    MR_HIP_TRY(k::transpose_fill(n, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(),
-                                 cursor.as<int32_t>(), raw.as<int32_t>(), s));
-    MR_HIP_TRY(k::iota_i32(std::max(n, nc), ident.as<int32_t>(), s));
-    // sort each column's row list: rows of (tp, raw) mapped through identity
-    MR_TRY(sort_rows(nc, nnz, tp.as<int32_t>(), raw.as<int32_t>(), raw.p, 4, ident.as<int32_t>(),
-                     ident.as<int32_t>(), tp.as<int32_t>(), ti.as<int32_t>(), dummy.p, s));
+    DevBuf ws;
+    MR_TRY(ws.alloc(k::transpose_ws_bytes(nnz, nc)));
+    MR_HIP_TRY(k::transpose_sorted(n, nc, nnz, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(),
+                                   ti.as<int32_t>(), ws.p, ws.bytes, s));
     return MR_OK;
 }
 
@@ -120,8 +108,6 @@ int relabel(mr_mat m, const std::vector<int32_t>& perm, double* ms_out)
     const int64_t n = m->n_rows, nnz = m->nnz;
     hipStream_t s = m->ctx->stream;
     if (n == 0) return MR_OK;
-    std::vector<int32_t> inv((size_t)n);
-    for (int64_t i = 0; i < n; ++i) inv[(size_t)perm[(size_t)i]] = (int32_t)i;
     DevBuf d_perm, d_inv, d_len, d_newrp, d_newci, d_newval, ws;
     MR_TRY(d_perm.alloc(sizeof(int32_t) * (size_t)n));
     MR_TRY(d_inv.alloc(sizeof(int32_t) * (size_t)n));
@@ -131,9 +117,20 @@ int relabel(mr_mat m, const std::vector<int32_t>& perm, double* ms_out)
     MR_TRY(d_newval.alloc(dtype_size(m->dtype) * (size_t)(nnz + kPad)));
     MR_TRY(ws.alloc(k::scan_workspace_bytes(n)));
     MR_HIP_TRY(hipMemcpyAsync(d_perm.p, perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
-    MR_HIP_TRY(hipMemcpyAsync(d_inv.p, inv.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
-    MR_HIP_TRY(hipMemsetAsync(d_newci.p, 0, d_newci.bytes, s));
-    MR_HIP_TRY(hipMemsetAsync(d_newval.p, 0, d_newval.bytes, s));
+    {
+        DevBuf d_bad;
+        MR_TRY(d_bad.alloc(sizeof(int32_t)));
+        MR_HIP_TRY(k::invert_perm(n, d_perm.as<int32_t>(), d_inv.as<int32_t>(), d_bad.as<int32_t>(), s));
+        int32_t bad = 0;
+        MR_HIP_TRY(hipMemcpyAsync(&bad, d_bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+        if (bad) return set_error(MR_ERR_INVALID_ARG, "relabel: the order is not a permutation of [0, %lld)",
+                                  (long long)n);
+    }
+    // only the padding past nnz needs a value: every entry below it is written
+    MR_HIP_TRY(hipMemsetAsync(d_newci.as<int32_t>() + nnz, 0, sizeof(int32_t) * (size_t)kPad, s));
+    MR_HIP_TRY(hipMemsetAsync(static_cast<char*>(d_newval.p) + dtype_size(m->dtype) * (size_t)nnz, 0,
+                              dtype_size(m->dtype) * (size_t)kPad, s));
     MR_HIP_TRY(k::perm_row_lengths(n, m->rp.as<int32_t>(), d_inv.as<int32_t>(), d_len.as<int32_t>(), s));
     MR_HIP_TRY(k::exclusive_scan_i32(d_len.as<int32_t>(), n, d_newrp.as<int32_t>(), nullptr, ws.p, s));
     const int32_t tail = (int32_t)nnz;
