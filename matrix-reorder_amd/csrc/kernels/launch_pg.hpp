@@ -11,12 +11,15 @@ struct PgState {
     int32_t tcount[2];   // sizes of the two touched-vertex lists
     int32_t nsure, ntie; // candidate list sizes
     int32_t tb, above, need;
-    int32_t ycount;      // distinct columns listed by the window's first pass
+    int32_t ycount;      // work items listed by the window's column pass
     int32_t lo_chunk;    // first 1024-id chunk that may hold unplaced vertices
     int32_t pad2[7];
 };
 
 int pg_hist_bins();
+// capacity of the window's work-item list (items of at most 64 row-list
+// entries, at most one partial item per column)
+int64_t pg_item_capacity(int64_t n, int64_t nnz);
 // start counts, the first touched list and its count histogram (hist must be
 // zero on entry; each batch clears it and builds the next batch's)
 hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, PgState* st, int32_t* chunk_left,
@@ -24,7 +27,7 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 // one batch: select |P| (pl) at ps, then score the window [os, os+ol)
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
-                    int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
+                    int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, uint64_t* items,
                     int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap, hipStream_t s);
 // live[y] = full length of rows(y); then, from time to time, rows(y) keeps
 // only unplaced vertices (in place, first live[y] entries of the segment)

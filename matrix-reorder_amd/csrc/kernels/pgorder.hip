@@ -54,6 +54,7 @@ constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
 constexpr int kWindowGrid = 2048; // blocks of the window-score kernel (8192 waves)
 constexpr int kSureCap = 4096;    // capacity of the candidates-above-threshold list
 constexpr int kTouchedPer = 16;   // ids per thread of the touched-list kernel
+constexpr int kItem = 64;         // row-list entries per work item of the window's row pass
 
 __device__ __forceinline__ int count_bin(uint32_t c)
 {
@@ -684,18 +685,23 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
 // count[z] += h[y] * [z in rows(y)] with h[y] = #window rows holding y, so the
 // work is done in two passes: (1) every (x, y) pair adds one to h[y] -- pairs
 // spread over a fixed grid by a prefix sum of the window's row lengths -- and
-// lists y when h[y] leaves 0; (2) one wave per listed y walks rows(y) once and
-// adds h[y].  Integer sums: identical counts, far fewer updates when window
-// rows share columns (hubs).
-// The kernel also resets the counts of the batch's touched list to the status
-// bit (placed vertices keep 0) and clears the count histogram, both consumed
-// by the selection: the window adds to zero counts.
+// when h[y] leaves 0, rows(y) is listed as work items of at most kItem
+// entries; (2) one 16-lane group per item adds h[y] over its entries.
+// Integer sums: identical counts, far fewer updates when window rows share
+// columns (hubs).  Splitting long lists into items spreads them over the
+// whole grid: a batch lists a few thousand columns, most of whose entries sit
+// in lists of hundreds, so one group per list would leave the walk bound by
+// the longest lists.
+//
+// A work item is (chunk << 32) | y: entries [chunk * kItem, (chunk + 1) * kItem)
+// of the live part of rows(y).
 __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t* __restrict__ window, int32_t ol,
                                                        const int32_t* __restrict__ wprefix,
                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                        const int32_t* __restrict__ tp, int32_t* __restrict__ h,
-                                                       int32_t* __restrict__ ylist, PgState* __restrict__ st,
-                                                       int32_t hub_cap, const int32_t* __restrict__ T, int cur,
+                                                       const int32_t* __restrict__ live, uint64_t* __restrict__ items,
+                                                       PgState* __restrict__ st, int32_t hub_cap,
+                                                       const int32_t* __restrict__ T, int cur,
                                                        uint32_t* __restrict__ score, int32_t* __restrict__ hist)
 {
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -727,29 +733,47 @@ __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t
                 if (tp[y + 1] - tp[y] < hub_cap) first = atomicAdd(&h[y], 1) == 0;
             }
         }
-        wave_append(first, y, ylist, &st->ycount);
+        // the first pair of a column lists its row list as work items (one
+        // at least: the item of chunk 0 also resets h[y] after the pass)
+        const int32_t nit = first ? max(1, (live[y] + kItem - 1) / kItem) : 0;
+        int32_t inc = nit;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t o = __shfl_up(inc, d, 64);
+            if ((int)(threadIdx.x & 63) >= d) inc += o;
+        }
+        const int32_t wtot = __shfl(inc, 63, 64);
+        if (wtot > 0) {
+            int32_t base = 0;
+            if ((threadIdx.x & 63) == 0) base = atomicAdd(&st->ycount, wtot);
+            base = __shfl(base, 0, 64);
+            for (int32_t j = 0; j < nit; ++j)
+                items[base + inc - nit + j] = ((uint64_t)(uint32_t)j << 32) | (uint32_t)y;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict__ tp, const int32_t* __restrict__ ti,
-                                                       const int32_t* __restrict__ live,
-                                                       int32_t* __restrict__ h, const int32_t* __restrict__ ylist,
+                                                       const int32_t* __restrict__ live, const int32_t* __restrict__ h,
+                                                       const uint64_t* __restrict__ items,
                                                        uint32_t* __restrict__ score, const PgState* __restrict__ st)
 {
-    // groups of 16 lanes, four listed columns per wave; rows(y) holds only
+    // groups of 16 lanes, four work items per wave; rows(y) holds only
     // vertices still unplaced at the last compaction (live[y] entries)
-    const int32_t ny = st->ycount;
+    const int32_t nit = st->ycount;
     const int lane = threadIdx.x & 63, li = lane & 15;
     const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 4);
-    for (int64_t q0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 4; q0 < ny; q0 += groups) {
+    for (int64_t q0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 4; q0 < nit; q0 += groups) {
         const int64_t q = q0 + (lane >> 4);
-        int32_t zb = 0, ze = 0, y = 0;
+        int32_t zb = 0, ze = 0;
         uint32_t w = 0;
-        if (q < ny) {
-            y = ylist[q];
+        if (q < nit) {
+            const uint64_t it = items[q];
+            const int32_t y = (int32_t)(uint32_t)it, chunk = (int32_t)(it >> 32);
             w = (uint32_t)h[y];
-            zb = tp[y];
-            ze = zb + live[y];
+            const int32_t b0 = tp[y], e0 = b0 + live[y];
+            zb = b0 + chunk * kItem;
+            ze = zb + kItem < e0 ? zb + kItem : e0;
         }
         for (int32_t c0 = zb; c0 < ze; c0 += 16) {
             const int32_t c = c0 + li;
@@ -761,7 +785,6 @@ __global__ __launch_bounds__(256) void window_z_kernel(const int32_t* __restrict
                 if (score[z] & kUnplaced) atomicAdd(&score[z], w);
             }
         }
-        if (q < ny && li == 0) h[y] = 0;   // y is listed once: back to 0 for the next window
     }
 }
 
@@ -776,8 +799,16 @@ __device__ __forceinline__ bool is_touched(uint32_t sc) { return (sc & kUnplaced
 
 __global__ __launch_bounds__(256) void touched_kernel(int64_t n, const uint32_t* __restrict__ score,
                                                       int32_t* __restrict__ T, PgState* __restrict__ st, int nxt,
-                                                      int32_t* __restrict__ hist)
+                                                      int32_t* __restrict__ hist, int32_t* __restrict__ hcol,
+                                                      const uint64_t* __restrict__ items)
 {
+    // the window's column weights go back to 0 (the first item of each
+    // listed column resets it; the row pass that read them has ended)
+    const int32_t nit = st->ycount;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nit; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t it = items[i];
+        if ((it >> 32) == 0) hcol[(int32_t)(uint32_t)it] = 0;
+    }
     __shared__ int s_w[4];
     __shared__ int s_base, s_tot;
     __shared__ int32_t h[4096];
@@ -892,20 +923,20 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 // (with its histogram).
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
-                    int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, int32_t* ylist,
+                    int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, uint64_t* items,
                     int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap, hipStream_t s)
 {
     const int nxt = cur ^ 1;
     collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl, hist);
     final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, chunk_left, placement, ps, pl, rp, os, ol,
                                    wprefix, hub_cap);
-    window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, ylist, st, hub_cap,
-                                                Tl[cur], cur, score, hist);
+    window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, live, items, st,
+                                                hub_cap, Tl[cur], cur, score, hist);
     if (ol > 0) {
-        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, ylist, score, st);
+        window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, items, score, st);
         const int64_t per_block = 256 * (int64_t)kTouchedPer;
         const int64_t g = n > 0 ? (n + per_block - 1) / per_block : 1;
-        touched_kernel<<<(unsigned)g, 256, 0, s>>>(n, score, Tl[nxt], st, nxt, hist);
+        touched_kernel<<<(unsigned)g, 256, 0, s>>>(n, score, Tl[nxt], st, nxt, hist, h, items);
     }
     return hipGetLastError();
 }
@@ -930,6 +961,8 @@ hipError_t pg_live_init(int64_t n, const int32_t* tp, int32_t* live, hipStream_t
 }
 
 int pg_hist_bins() { return kBins + kCoarse; }
+
+int64_t pg_item_capacity(int64_t n, int64_t nnz) { return n + nnz / kItem + 64; }
 
 }  // namespace k
 }  // namespace mr

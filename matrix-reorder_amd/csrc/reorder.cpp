@@ -312,7 +312,7 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     const auto t_gpu = clk::now();
     DevBuf tp, ti;
     MR_TRY(gpu_transpose(m, tp, ti, s));
-    DevBuf score, T0, T1, sure, tie, hist, chunk, placement, state, wprefix, hcol, ylist;
+    DevBuf score, T0, T1, sure, tie, hist, chunk, placement, state, wprefix, hcol, items;
     MR_TRY(score.alloc(sizeof(uint32_t) * (size_t)n));
     MR_TRY(T0.alloc(sizeof(int32_t) * (size_t)n));
     MR_TRY(T1.alloc(sizeof(int32_t) * (size_t)n));
@@ -324,7 +324,7 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     MR_TRY(state.alloc(sizeof(k::PgState)));
     MR_TRY(wprefix.alloc(sizeof(int32_t) * 1025));
     MR_TRY(hcol.alloc(sizeof(int32_t) * (size_t)n));
-    MR_TRY(ylist.alloc(sizeof(int32_t) * (size_t)n));
+    MR_TRY(items.alloc(sizeof(uint64_t) * (size_t)k::pg_item_capacity(n, nnz)));
     MR_HIP_TRY(hipMemsetAsync(hcol.p, 0, hcol.bytes, s));
     MR_HIP_TRY(hipMemsetAsync(hist.p, 0, hist.bytes, s));
     MR_HIP_TRY(hipMemsetAsync(state.p, 0, state.bytes, s));
@@ -490,7 +490,7 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
             e = k::pg_batch(n, cur, m->rp.as<int32_t>(), m->ci.as<int32_t>(), tp.as<int32_t>(), ti_live.as<int32_t>(),
                             score.as<uint32_t>(), Tl, st, sure.as<int32_t>(), tie.as<int32_t>(), hist.as<int32_t>(),
                             chunk.as<int32_t>(), placement.as<uint32_t>(), wprefix.as<int32_t>(), hcol.as<int32_t>(),
-                            ylist.as<int32_t>(), live.as<int32_t>(), P.Ps[(size_t)b], P.Pl[(size_t)b],
+                            items.as<uint64_t>(), live.as<int32_t>(), P.Ps[(size_t)b], P.Pl[(size_t)b],
                             P.Os[(size_t)b], P.Ol[(size_t)b], P.hub_cap, s);
         if (e != hipSuccess) rc = set_error(MR_ERR_HIP, "pGOrder batch %d failed: %s", b, hipGetErrorString(e));
         cur ^= 1;
