@@ -49,6 +49,7 @@ constexpr int kBins = kExactBins + 16 * 64;
 constexpr int kCoarseShift = 9;                    // 512 count bins per coarse bin
 constexpr int kCoarse = (kBins + (1 << kCoarseShift) - 1) >> kCoarseShift;   // 130
 constexpr int kSmallT = 4096;     // sort T directly at or below this size
+constexpr int kSmallIds = 8192;   // ids sorted at once (32-bit keys in the same LDS)
 constexpr int kChunk = 1024;      // ids per unplaced counter
 constexpr int kGridStride = 1024; // blocks of the grid-stride kernels
 constexpr int kWindowGrid = 2048; // blocks of the window-score kernel (8192 waves)
@@ -90,29 +91,31 @@ __device__ __forceinline__ void wave_append(bool pred, int32_t v, int32_t* list,
 // compare-exchange at distance j < 64 is a lane shuffle, 64 <= j < 64R
 // swaps two registers of the same lane, and only j >= 64R (the last
 // log2(4) merge levels) goes through LDS with a block barrier -- three
-// barrier stages instead of one per stage of the network.
-__device__ __forceinline__ uint64_t cx_keep(uint64_t x, uint64_t y, bool keep_min)
+// barrier stages instead of one per stage of the network.  K is uint64_t
+// (count, id) selection keys or uint32_t ids.
+template <typename K>
+__device__ __forceinline__ K cx_keep(K x, K y, bool keep_min)
 {
     const bool lt = x < y;
     return keep_min ? (lt ? x : y) : (lt ? y : x);
 }
 
-template <int R, int JJ>
-__device__ __forceinline__ void sort_lane_stage(uint64_t (&v)[R], int k2, int ebase)
+template <typename K, int R, int JJ>
+__device__ __forceinline__ void sort_lane_stage(K (&v)[R], int k2, int ebase)
 {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (r & JJ) continue;
         const bool up = ((ebase + 64 * r) & k2) == 0;
-        const uint64_t a = v[r], b = v[r + JJ];
+        const K a = v[r], b = v[r + JJ];
         const bool sw = (a > b) == up;
         v[r] = sw ? b : a;
         v[r + JJ] = sw ? a : b;
     }
 }
 
-template <int R>
-__device__ void block_sort_regs(uint64_t (&v)[R], uint64_t* x)
+template <typename K, int R>
+__device__ void block_sort_regs(K (&v)[R], K* x)
 {
     constexpr int S = 64 * R, N = 256 * R;
     const int lane = threadIdx.x & 63;
@@ -131,39 +134,41 @@ __device__ void block_sort_regs(uint64_t (&v)[R], uint64_t* x)
                 }
             } else if (j >= 64) {
                 const int jj = j >> 6;
-                if constexpr (R > 1) if (jj == 1) sort_lane_stage<R, 1>(v, k2, ebase);
-                if constexpr (R > 2) if (jj == 2) sort_lane_stage<R, 2>(v, k2, ebase);
-                if constexpr (R > 4) if (jj == 4) sort_lane_stage<R, 4>(v, k2, ebase);
-                if constexpr (R > 8) if (jj == 8) sort_lane_stage<R, 8>(v, k2, ebase);
+                if constexpr (R > 1) if (jj == 1) sort_lane_stage<K, R, 1>(v, k2, ebase);
+                if constexpr (R > 2) if (jj == 2) sort_lane_stage<K, R, 2>(v, k2, ebase);
+                if constexpr (R > 4) if (jj == 4) sort_lane_stage<K, R, 4>(v, k2, ebase);
+                if constexpr (R > 8) if (jj == 8) sort_lane_stage<K, R, 8>(v, k2, ebase);
+                if constexpr (R > 16) if (jj == 16) sort_lane_stage<K, R, 16>(v, k2, ebase);
             } else {
                 const bool lower = (lane & j) == 0;
+                // all shuffles first: they are independent of each other
+                K y[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint64_t y = __shfl_xor(v[r], j, 64);
-                    v[r] = cx_keep(v[r], y, lower == (((ebase + 64 * r) & k2) == 0));
-                }
+                for (int r = 0; r < R; ++r) y[r] = __shfl_xor(v[r], j, 64);
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = cx_keep(v[r], y[r], lower == (((ebase + 64 * r) & k2) == 0));
             }
         }
     }
 }
 
-// sort keys[0, cnt) (cnt <= 256 R) and write the low words of the `take`
+// sort keys[0, cnt) (cnt <= 256 R) and write the low 32 bits of the `take`
 // smallest to sel[base, base + take); keys[] is the LDS exchange buffer
-template <int R>
-__device__ void sort_take_regs(uint64_t* keys, int cnt, int take, int32_t* sel, int base)
+template <typename K, int R>
+__device__ void sort_take_regs(K* keys, int cnt, int take, int32_t* sel, int base)
 {
     const int ebase = (threadIdx.x >> 6) * (64 * R) + (threadIdx.x & 63);
-    uint64_t v[R];
+    K v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = ebase + 64 * r;
-        v[r] = e < cnt ? keys[e] : ~0ull;
+        v[r] = e < cnt ? keys[e] : ~(K)0;
     }
-    block_sort_regs<R>(v, keys);
+    block_sort_regs<K, R>(v, keys);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = ebase + 64 * r;
-        if (e < take) sel[base + e] = (int32_t)(v[r] & 0xffffffffu);
+        if (e < take) sel[base + e] = (int32_t)(uint32_t)v[r];
     }
 }
 
@@ -372,11 +377,35 @@ __device__ void select_sorted_r(int cnt, int take, const int32_t* __restrict__ a
         // padding: count 0 and id ~0, the key ~0 (sorts last)
         v[r] = sel_key((sc & ~kUnplaced) & okm[r], id[r] | ~okm[r]);
     }
-    block_sort_regs<R>(v, x);
+    block_sort_regs<uint64_t, R>(v, x);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = ebase + 64 * r;
         if (e < take) sel[base + e] = (int32_t)(v[r] & 0xffffffffu);
+    }
+}
+
+// The `take` smallest of the ids a[0, cnt) (cnt <= 256 R), ascending, to
+// sel[base, base + take): ties of one exact count are ordered by id alone, so
+// 32-bit keys do (half the shuffles and compares of the 64-bit keys).
+template <int R>
+__device__ void select_ids_r(int cnt, int take, const int32_t* __restrict__ a, uint32_t* x, int32_t* sel, int base)
+{
+    const int ebase = (threadIdx.x >> 6) * (64 * R) + (threadIdx.x & 63);
+    uint32_t okm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) okm[r] = opaque_mask(ebase + 64 * r < cnt);
+    uint32_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        v[r] = ((uint32_t)a[e < cnt ? e : 0] & okm[r]) | ~okm[r];   // padding ~0 sorts last
+    }
+    block_sort_regs<uint32_t, R>(v, x);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = ebase + 64 * r;
+        if (e < take) sel[base + e] = (int32_t)v[r];
     }
 }
 
@@ -401,10 +430,10 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
     __shared__ int32_t s_misc[4];
     __shared__ uint64_t s_ball[4];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    // every bookkeeping field this kernel reads, loaded once (tb and need are
+    // every bookkeeping field this kernel reads, loaded once (tb is
     // meaningful only after collect_kernel ran, i.e. for a large list)
     const int32_t tn = st->tcount[cur];
-    const int32_t nsure = st->nsure, ntie = st->ntie, tb = st->tb, need = st->need, lo_chunk0 = st->lo_chunk;
+    const int32_t nsure = st->nsure, ntie = st->ntie, tb = st->tb, lo_chunk0 = st->lo_chunk;
     const int32_t want = pl > 0 ? (pl < 1024 ? pl : 1024) : 0;
     if (tid == 0) {
         s_nsel = 0;
@@ -425,15 +454,31 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
         if (tid == 0) s_nsel = base + take;
         __syncthreads();
     };
-    // the `take` smallest of keys[0, cnt) (cnt <= kSmallT) appended to sel
-    auto sort_take = [&](int cnt, int take) {
+    // the `take` smallest ids among ids32[0, cnt) (cnt <= kSmallIds, ids
+    // staged in LDS) appended to sel
+    uint32_t* const ids32 = reinterpret_cast<uint32_t*>(keys);
+    auto sort_take_ids = [&](int cnt, int take) {
         __syncthreads();
         const int base = s_nsel;
-        if (cnt <= 256) sort_take_regs<1>(keys, cnt, take, sel, base);
-        else if (cnt <= 512) sort_take_regs<2>(keys, cnt, take, sel, base);
-        else if (cnt <= 1024) sort_take_regs<4>(keys, cnt, take, sel, base);
-        else if (cnt <= 2048) sort_take_regs<8>(keys, cnt, take, sel, base);
-        else sort_take_regs<16>(keys, cnt, take, sel, base);
+        if (cnt <= 256) sort_take_regs<uint32_t, 1>(ids32, cnt, take, sel, base);
+        else if (cnt <= 512) sort_take_regs<uint32_t, 2>(ids32, cnt, take, sel, base);
+        else if (cnt <= 1024) sort_take_regs<uint32_t, 4>(ids32, cnt, take, sel, base);
+        else if (cnt <= 2048) sort_take_regs<uint32_t, 8>(ids32, cnt, take, sel, base);
+        else if (cnt <= 4096) sort_take_regs<uint32_t, 16>(ids32, cnt, take, sel, base);
+        else sort_take_regs<uint32_t, 32>(ids32, cnt, take, sel, base);
+        __syncthreads();
+        if (tid == 0) s_nsel = base + take;
+        __syncthreads();
+    };
+    // the `take` smallest of the ids a[0, cnt) (cnt <= kSmallIds) appended to sel
+    auto select_ids = [&](int cnt, int take, const int32_t* a) {
+        const int base = s_nsel;
+        if (cnt <= 256) select_ids_r<1>(cnt, take, a, ids32, sel, base);
+        else if (cnt <= 512) select_ids_r<2>(cnt, take, a, ids32, sel, base);
+        else if (cnt <= 1024) select_ids_r<4>(cnt, take, a, ids32, sel, base);
+        else if (cnt <= 2048) select_ids_r<8>(cnt, take, a, ids32, sel, base);
+        else if (cnt <= 4096) select_ids_r<16>(cnt, take, a, ids32, sel, base);
+        else select_ids_r<32>(cnt, take, a, ids32, sel, base);
         __syncthreads();
         if (tid == 0) s_nsel = base + take;
         __syncthreads();
@@ -444,70 +489,116 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
             select_sorted(tn, tn < want ? tn : want, T, tn, T);
         } else {
             const int32_t ns = min(nsure, kSureCap), nt = ntie;
-            if (ns + nt <= kSmallT) {
-                select_sorted(ns + nt, ns + nt < want ? ns + nt : want, sure, ns, tie);
-            } else {
-                // all sure candidates first (they beat every tie)
+            // every sure candidate beats every tie, so the ties fill what is
+            // left: take_t = need of the threshold search
+            const int32_t take_t = min(nt, max(0, want - ns));
+            bool general = false;
+            if (tb < kExactBins) {
+                // an exact threshold bin: the ties share one count and come
+                // in id order after the sorted sure candidates
                 select_sorted(ns, ns, sure, ns, sure);
-                // the `need` best ties
-                if (tb < kExactBins) {
-                    // equal counts: the `need` smallest ids, via an id histogram
+                if (take_t > 0 && nt <= kSmallIds) {
+                    select_ids(nt, take_t, tie);
+                } else if (take_t > 0) {
+                    // the take_t smallest of many ids: an id histogram finds
+                    // the bin holding the take_t-th, the ids up to it are
+                    // sorted
                     int sh = 0;
                     while (((n - 1) >> sh) >= 4096) ++sh;
                     for (int i = tid; i < 4096; i += blockDim.x) s_bins[i] = 0;
                     __syncthreads();
-                    for (int i = tid; i < nt; i += blockDim.x) atomicAdd(&s_bins[tie[i] >> sh], 1);
-                    __syncthreads();
-                    if (tid == 0) {
-                        int run = 0, tbin = 4095;
-                        for (int bb = 0; bb < 4096; ++bb) {
-                            if (run + s_bins[bb] >= need) { tbin = bb; break; }
-                            run += s_bins[bb];
+                    for (int i0 = 0; i0 < nt; i0 += 8 * 256) {
+                        int32_t v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int i = i0 + u * 256 + tid;
+                            v[u] = tie[i < nt ? i : 0];
                         }
-                        s_misc[0] = tbin;
-                        s_misc[1] = 0;
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (i0 + u * 256 + tid < nt) atomicAdd(&s_bins[v[u] >> sh], 1);
+                    }
+                    __syncthreads();
+                    // thread t owns bins [16t, 16t + 16): block scan of its sum
+                    int own = 0;
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) own += s_bins[16 * tid + q];
+                    int incl = own;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const int o = __shfl_up(incl, d, 64);
+                        if (lane >= d) incl += o;
+                    }
+                    if (lane == 63) s_w[wid] = incl;
+                    if (tid == 0) s_misc[1] = 0;
+                    __syncthreads();
+                    int before = incl - own;
+                    for (int w = 0; w < wid; ++w) before += s_w[w];
+                    if (before < take_t && before + own >= take_t) {
+                        int run = before;
+                        for (int q = 0; q < 16; ++q) {
+                            if (run + s_bins[16 * tid + q] >= take_t) {
+                                s_misc[0] = 16 * tid + q;
+                                break;
+                            }
+                            run += s_bins[16 * tid + q];
+                        }
                     }
                     __syncthreads();
                     const int tbin = s_misc[0];
-                    // ids in bins <= tbin: fewer than need + 2^sh; sort them by id
-                    for (int i = tid; i < nt; i += blockDim.x) {
-                        const int32_t v = tie[i];
-                        if ((v >> sh) <= tbin) {
-                            const int pos = atomicAdd(&s_misc[1], 1);
-                            if (pos < kSmallT) keys[pos] = (uint64_t)(uint32_t)v;
+                    // ids in bins <= tbin: fewer than take_t + 2^sh
+                    for (int i0 = 0; i0 < nt; i0 += 8 * 256) {
+                        int32_t v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int i = i0 + u * 256 + tid;
+                            v[u] = tie[i < nt ? i : 0];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            if (i0 + u * 256 + tid < nt && (v[u] >> sh) <= tbin) {
+                                const int pos = atomicAdd(&s_misc[1], 1);
+                                if (pos < kSmallIds) ids32[pos] = (uint32_t)v[u];
+                            }
                         }
                     }
                     __syncthreads();
                     const int cnt = s_misc[1];
-                    if (cnt <= kSmallT) sort_take(cnt, need);
+                    if (cnt <= kSmallIds) sort_take_ids(cnt, take_t);
+                    else general = true;
                 }
-                if (tb >= kExactBins || s_misc[1] > kSmallT) {
-                    // rare general case: repeated maximum over the tie list
-                    uint64_t last = 0;
-                    for (int r = 0; r < need; ++r) {
-                        uint64_t best = ~0ull;
-                        for (int i = tid; i < nt; i += blockDim.x) {
-                            const int32_t v = tie[i];
-                            const uint64_t kk = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
-                            if ((r == 0 || kk > last) && kk < best) best = kk;
-                        }
-                        for (int off = 32; off > 0; off >>= 1) {
-                            const uint64_t o = __shfl_xor(best, off, 64);
-                            best = o < best ? o : best;
-                        }
-                        if ((tid & 63) == 0) keys[tid >> 6] = best;
-                        __syncthreads();
-                        if (tid == 0) {
-                            uint64_t bm = keys[0];
-                            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = keys[w] < bm ? keys[w] : bm;
-                            sel[s_nsel] = (int32_t)(bm & 0xffffffffu);
-                            s_nsel = s_nsel + 1;
-                            keys[8] = bm;
-                        }
-                        __syncthreads();
-                        last = keys[8];
-                        __syncthreads();
+            } else if (ns + nt <= kSmallT) {
+                select_sorted(ns + nt, ns + nt < want ? ns + nt : want, sure, ns, tie);
+            } else {
+                select_sorted(ns, ns, sure, ns, sure);
+                general = true;
+            }
+            if (general) {
+                // rare: repeated minimum of the (count, id) keys over the tie list
+                uint64_t last = 0;
+                for (int r = 0; r < take_t; ++r) {
+                    uint64_t best = ~0ull;
+                    for (int i = tid; i < nt; i += blockDim.x) {
+                        const int32_t v = tie[i];
+                        const uint64_t kk = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
+                        if ((r == 0 || kk > last) && kk < best) best = kk;
                     }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const uint64_t o = __shfl_xor(best, off, 64);
+                        best = o < best ? o : best;
+                    }
+                    if ((tid & 63) == 0) keys[tid >> 6] = best;
+                    __syncthreads();
+                    if (tid == 0) {
+                        uint64_t bm = keys[0];
+                        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = keys[w] < bm ? keys[w] : bm;
+                        sel[s_nsel] = (int32_t)(bm & 0xffffffffu);
+                        s_nsel = s_nsel + 1;
+                        keys[8] = bm;
+                    }
+                    __syncthreads();
+                    last = keys[8];
+                    __syncthreads();
                 }
             }
         }
