@@ -79,6 +79,37 @@ def pgorder_cases():
     # hundreds of batches: the column lists are compacted every 32 batches
     out.append(("rmat_s13_w32_compacted", *rmat_csr(13, 16, seed=13), 32))
     out.append(("rmat_s15_w64_compacted", *rmat_csr(15, 16, seed=15), 64))
+    out.extend(selection_path_cases())
+    return out
+
+
+def selection_path_cases():
+    """Graphs built to reach the rarer selection paths of the GPU loop.
+
+    ties: row 0 points at columns 1-3, each held by 9000 one-entry rows, so a
+    window with row 0 gives ~27000 touched vertices in three equal-count
+    groups -- more than 8192 ties of an exact count (the id-histogram path).
+    dense: 6000 rows share 64 columns; once a 1024-row window holds them, the
+    other rows reach count 64 * 1024 = 65536, beyond the exact bins, with
+    more than 4096 candidates (the repeated-minimum path).  dense_mixed adds
+    1000 rows holding one of those columns: the same inexact bin with at most
+    4096 candidates (one sort of the candidates).
+    """
+    rng = np.random.default_rng(7)
+    n = 30000
+    rows = [np.zeros(0, np.int64) for _ in range(n)]
+    rows[0] = np.array([1, 2, 3])
+    zs = rng.permutation(np.arange(10, n))[:27000]
+    for g in range(3):
+        for z in zs[g * 9000:(g + 1) * 9000]:
+            rows[z] = np.array([1 + g])
+    out = [("ties_30000_w1024", *csr_rows(rows, n), 1024)]
+    n = 6200
+    block = np.arange(6100, 6164)
+    rows = [block] * 6000 + [np.zeros(0, np.int64)] * 200
+    out.append(("dense_6000x64_w1024", *csr_rows(rows, n), 1024))
+    rows = [block] * 5000 + [block[:1]] * 1000 + [np.zeros(0, np.int64)] * 200
+    out.append(("dense_mixed_w1024", *csr_rows(rows, n), 1024))
     return out
 
 
@@ -217,6 +248,7 @@ def gorder5_cases():
             np.sort(rng.choice(n, int(rng.integers(0, 6)), replace=False)) for i in range(n)]
     out.append(("hubs_30000", *csr_rows(rows, n), 1024, 1, 5))
     out.append(("rmat_s13_v32_compacted", *rmat_csr(13, 16, seed=13), 32, 1, 5))
+    out.extend((name, rp, ci, W, 1, 5) for name, rp, ci, W in selection_path_cases())
     return out
 
 
