@@ -22,11 +22,20 @@ hipError_t perm_class_count(int64_t n, const int32_t* new_len, int32_t* counts, 
 // scatter row ids into class lists starting at offs[c]; cursor[kPermClasses] zeroed by caller
 hipError_t perm_class_fill(int64_t n, const int32_t* new_len, const int32_t* offs, int32_t* cursor,
                            int32_t* lists, hipStream_t s);
-// relabel + sort the rows of one class; scratch/scratch_off only for class 8
+// relabel + sort the rows of one class below the last
 hipError_t perm_sort_class(int cls, int32_t count, const int32_t* rows, const int32_t* rp,
                            const int32_t* ci, const void* val, int val_bytes, const int32_t* perm,
                            const int32_t* inv, const int32_t* new_rp, int32_t* new_ci, void* new_val,
-                           uint64_t* scratch, const int64_t* scratch_off, hipStream_t s);
+                           hipStream_t s);
+// the last class (rows longer than 4096): out[i] = new_len[rows[i]]; then,
+// with boff the exclusive prefix of those lengths (total entries), one
+// stable radix sort of all their entries
+hipError_t perm_long_lengths(int32_t count, const int32_t* rows, const int32_t* new_len, int32_t* out, hipStream_t s);
+size_t perm_long_ws_bytes(int64_t total, int32_t count, int64_t n);
+hipError_t perm_sort_long(int32_t count, const int32_t* rows, const int32_t* rp, const int32_t* ci, const void* val,
+                          int val_bytes, const int32_t* perm, const int32_t* inv, const int32_t* new_rp,
+                          int32_t* new_ci, void* new_val, const int64_t* boff, int64_t total, int64_t n, void* ws,
+                          size_t ws_bytes, hipStream_t s);
 
 // inv[perm[i]] = i; *bad becomes 1 unless perm is a permutation of [0, n)
 // (out-of-range entries are not written)

@@ -13,9 +13,12 @@
 //                group of 2..64 lanes of a wave64 (xor shuffles), 64/w rows
 //                per wave
 //   class 7      len <= 4096: one workgroup, bitonic sort in LDS
-//   class 8      longer: one workgroup, bitonic sort in a global scratch row
+//   class 8      longer: all such rows at once, one device-wide stable radix
+//                sort of (row slot, new column) keys with positions as values
 // No floating point is touched: the output is bit-identical to a sequential
 // relabel + stable sort.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "dtypes.hpp"
 #include "launch_perm.hpp"
 
@@ -180,38 +183,66 @@ __global__ __launch_bounds__(256) void sort_block_kernel(int32_t count, const in
     }
 }
 
-template <typename E>
-__global__ __launch_bounds__(256) void sort_big_kernel(int32_t count, const int32_t* __restrict__ rows,
-                                                       const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                                       const E* __restrict__ val, const int32_t* __restrict__ perm,
-                                                       const int32_t* __restrict__ inv,
-                                                       const int32_t* __restrict__ new_rp,
-                                                       int32_t* __restrict__ new_ci, E* __restrict__ new_val,
-                                                       uint64_t* __restrict__ scratch,
-                                                       const int64_t* __restrict__ scratch_off)
+// long rows: key = (slot << cbits) | new column, value = position in the old
+// row; one block per row
+__global__ __launch_bounds__(256) void long_keys_kernel(int32_t count, const int32_t* __restrict__ rows,
+                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                        const int32_t* __restrict__ perm,
+                                                        const int32_t* __restrict__ inv,
+                                                        const int64_t* __restrict__ boff, int cbits,
+                                                        uint64_t* __restrict__ keys, int32_t* __restrict__ pos)
 {
-    const int32_t p = rows[blockIdx.x], o = inv[p];
+    const int32_t i = blockIdx.x;
+    const int32_t p = rows[i], o = inv[p];
     const int32_t ob = rp[o], len = rp[o + 1] - ob;
-    int P = 8192;
-    while (P < len) P <<= 1;
-    uint64_t* a = scratch + scratch_off[blockIdx.x];
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-        a[i] = (i < len) ? (((uint64_t)(uint32_t)perm[ci[ob + i]] << 32) | (uint32_t)i) : ~0ull;
-    __syncthreads();
-    block_bitonic(a, P);
-    const int32_t nb = new_rp[p];
-    for (int i = threadIdx.x; i < len; i += blockDim.x) {
-        const uint64_t key = a[i];
-        new_ci[nb + i] = (int32_t)(key >> 32);
-        new_val[nb + i] = val[ob + (int32_t)(key & 0xffffffffu)];
+    const int64_t base = boff[i];
+    for (int32_t t = threadIdx.x; t < len; t += blockDim.x) {
+        keys[base + t] = ((uint64_t)(uint32_t)i << cbits) | (uint32_t)perm[ci[ob + t]];
+        pos[base + t] = t;
     }
+}
+
+// slot i's sorted entries sit at [boff[i], boff[i] + len): columns and values
+template <typename E>
+__global__ __launch_bounds__(256) void long_write_kernel(int32_t count, const int32_t* __restrict__ rows,
+                                                         const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ inv,
+                                                         const int32_t* __restrict__ new_rp,
+                                                         const int64_t* __restrict__ boff,
+                                                         const uint64_t* __restrict__ keys,
+                                                         const int32_t* __restrict__ pos, const E* __restrict__ val,
+                                                         uint64_t cmask, int32_t* __restrict__ new_ci,
+                                                         E* __restrict__ new_val)
+{
+    const int32_t i = blockIdx.x;
+    const int32_t p = rows[i], o = inv[p];
+    const int32_t ob = rp[o], len = rp[o + 1] - ob;
+    const int32_t nb = new_rp[p];
+    const int64_t base = boff[i];
+    for (int32_t t = threadIdx.x; t < len; t += blockDim.x) {
+        new_ci[nb + t] = (int32_t)(keys[base + t] & cmask);
+        new_val[nb + t] = val[ob + pos[base + t]];
+    }
+}
+
+__global__ void gather_len_kernel(int32_t count, const int32_t* __restrict__ rows, const int32_t* __restrict__ new_len,
+                                  int32_t* __restrict__ out)
+{
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        out[i] = new_len[rows[i]];
+}
+
+unsigned bits_for(int64_t v)   // bits to hold 0..v
+{
+    unsigned b = 1;
+    while (b < 63 && ((int64_t)1 << b) <= v) ++b;
+    return b;
 }
 
 template <typename E>
 hipError_t sort_class_t(int cls, int32_t count, const int32_t* rows, const int32_t* rp, const int32_t* ci,
                         const void* val, const int32_t* perm, const int32_t* inv, const int32_t* new_rp,
-                        int32_t* new_ci, void* new_val, uint64_t* scratch, const int64_t* scratch_off,
-                        hipStream_t s)
+                        int32_t* new_ci, void* new_val, hipStream_t s)
 {
     const E* v = static_cast<const E*>(val);
     E* nv = static_cast<E*>(new_val);
@@ -234,10 +265,6 @@ hipError_t sort_class_t(int cls, int32_t count, const int32_t* rows, const int32
     case 6: small(std::integral_constant<int, 64>{}); break;
     case 7:
         sort_block_kernel<E><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, v, perm, inv, new_rp, new_ci, nv);
-        break;
-    case 8:
-        sort_big_kernel<E><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, v, perm, inv, new_rp, new_ci, nv,
-                                                            scratch, scratch_off);
         break;
     default: return hipErrorInvalidValue;
     }
@@ -300,16 +327,83 @@ hipError_t perm_class_fill(int64_t n, const int32_t* new_len, const int32_t* off
 
 hipError_t perm_sort_class(int cls, int32_t count, const int32_t* rows, const int32_t* rp, const int32_t* ci,
                            const void* val, int val_bytes, const int32_t* perm, const int32_t* inv,
-                           const int32_t* new_rp, int32_t* new_ci, void* new_val, uint64_t* scratch,
-                           const int64_t* scratch_off, hipStream_t s)
+                           const int32_t* new_rp, int32_t* new_ci, void* new_val, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
+    if (cls < 0 || cls >= kPermClasses - 1) return hipErrorInvalidValue;   // the long class: perm_sort_long
     switch (val_bytes) {
-    case 2: return sort_class_t<uint16_t>(cls, count, rows, rp, ci, val, perm, inv, new_rp, new_ci, new_val, scratch, scratch_off, s);
-    case 4: return sort_class_t<uint32_t>(cls, count, rows, rp, ci, val, perm, inv, new_rp, new_ci, new_val, scratch, scratch_off, s);
-    case 8: return sort_class_t<uint64_t>(cls, count, rows, rp, ci, val, perm, inv, new_rp, new_ci, new_val, scratch, scratch_off, s);
+    case 2: return sort_class_t<uint16_t>(cls, count, rows, rp, ci, val, perm, inv, new_rp, new_ci, new_val, s);
+    case 4: return sort_class_t<uint32_t>(cls, count, rows, rp, ci, val, perm, inv, new_rp, new_ci, new_val, s);
+    case 8: return sort_class_t<uint64_t>(cls, count, rows, rp, ci, val, perm, inv, new_rp, new_ci, new_val, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t perm_long_lengths(int32_t count, const int32_t* rows, const int32_t* new_len, int32_t* out, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    int64_t g = ((int64_t)count + 255) / 256;
+    if (g > 4096) g = 4096;
+    gather_len_kernel<<<(unsigned)g, 256, 0, s>>>(count, rows, new_len, out);
+    return hipGetLastError();
+}
+
+static size_t long_sort_temp(int64_t total, unsigned bits)
+{
+    size_t tmp = 0;
+    rocprim::radix_sort_pairs(nullptr, tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr, (const int32_t*)nullptr,
+                              (int32_t*)nullptr, (size_t)total, 0u, bits, (hipStream_t)0);
+    return tmp;
+}
+
+size_t perm_long_ws_bytes(int64_t total, int32_t count, int64_t n)
+{
+    const unsigned bits = bits_for(count > 0 ? count - 1 : 0) + bits_for(n > 0 ? n - 1 : 0);
+    return (2 * sizeof(uint64_t) + 2 * sizeof(int32_t)) * (size_t)(total > 0 ? total : 1) +
+           long_sort_temp(total, bits) + 256;
+}
+
+hipError_t perm_sort_long(int32_t count, const int32_t* rows, const int32_t* rp, const int32_t* ci, const void* val,
+                          int val_bytes, const int32_t* perm, const int32_t* inv, const int32_t* new_rp,
+                          int32_t* new_ci, void* new_val, const int64_t* boff, int64_t total, int64_t n, void* ws,
+                          size_t ws_bytes, hipStream_t s)
+{
+    if (count <= 0 || total <= 0) return hipSuccess;
+    const unsigned cbits = bits_for(n > 0 ? n - 1 : 0);
+    const unsigned bits = bits_for(count - 1) + cbits;
+    uint64_t* keys = static_cast<uint64_t*>(ws);
+    uint64_t* keys_out = keys + total;
+    int32_t* pos = reinterpret_cast<int32_t*>(keys_out + total);
+    int32_t* pos_out = pos + total;
+    void* tmp = pos_out + total;
+    size_t tmp_bytes = long_sort_temp(total, bits);
+    if (ws_bytes < (2 * sizeof(uint64_t) + 2 * sizeof(int32_t)) * (size_t)total + tmp_bytes) return hipErrorInvalidValue;
+    long_keys_kernel<<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, perm, inv, boff, (int)cbits, keys, pos);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, static_cast<const uint64_t*>(keys), keys_out,
+                                  static_cast<const int32_t*>(pos), pos_out, (size_t)total, 0u, bits, s);
+    if (e != hipSuccess) return e;
+    const uint64_t cmask = ((uint64_t)1 << cbits) - 1;
+    switch (val_bytes) {
+    case 2:
+        long_write_kernel<uint16_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, inv, new_rp, boff, keys_out, pos_out,
+                                                                    static_cast<const uint16_t*>(val), cmask, new_ci,
+                                                                    static_cast<uint16_t*>(new_val));
+        break;
+    case 4:
+        long_write_kernel<uint32_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, inv, new_rp, boff, keys_out, pos_out,
+                                                                    static_cast<const uint32_t*>(val), cmask, new_ci,
+                                                                    static_cast<uint32_t*>(new_val));
+        break;
+    case 8:
+        long_write_kernel<uint64_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, inv, new_rp, boff, keys_out, pos_out,
+                                                                    static_cast<const uint64_t*>(val), cmask, new_ci,
+                                                                    static_cast<uint64_t*>(new_val));
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t invert_perm(int64_t n, const int32_t* perm, int32_t* inv, int32_t* bad, hipStream_t s)

@@ -57,32 +57,34 @@ int sort_rows(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, cons
     MR_HIP_TRY(hipMemcpyAsync(d_offs.p, offs, sizeof(offs), hipMemcpyHostToDevice, s));
     MR_HIP_TRY(k::perm_class_fill(n, d_len.as<int32_t>(), d_offs.as<int32_t>(), d_cursor.as<int32_t>(),
                                   d_lists.as<int32_t>(), s));
-    // scratch rows for the longest class
-    DevBuf d_scratch, d_soff;
+    // the longest class: lengths of its rows only, their offsets, one sort
     const int32_t nbig = counts[k::kPermClasses - 1];
+    DevBuf d_boff, d_lws;
+    std::vector<int64_t> boff;   // kept until the end: its upload is asynchronous
+    int64_t big_total = 0;
+    const int32_t* big_rows = d_lists.as<int32_t>() + offs[k::kPermClasses - 1];
     if (nbig > 0) {
-        std::vector<int32_t> rows((size_t)nbig), lens((size_t)n);
-        MR_HIP_TRY(hipMemcpyAsync(rows.data(), d_lists.as<int32_t>() + offs[k::kPermClasses - 1],
-                                  sizeof(int32_t) * (size_t)nbig, hipMemcpyDeviceToHost, s));
-        MR_HIP_TRY(hipMemcpyAsync(lens.data(), d_len.p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+        DevBuf d_blen;
+        MR_TRY(d_blen.alloc(sizeof(int32_t) * (size_t)nbig));
+        MR_HIP_TRY(k::perm_long_lengths(nbig, big_rows, d_len.as<int32_t>(), d_blen.as<int32_t>(), s));
+        std::vector<int32_t> blen((size_t)nbig);
+        MR_HIP_TRY(hipMemcpyAsync(blen.data(), d_blen.p, sizeof(int32_t) * (size_t)nbig, hipMemcpyDeviceToHost, s));
         MR_HIP_TRY(hipStreamSynchronize(s));
-        std::vector<int64_t> soff((size_t)nbig);
-        int64_t tot = 0;
+        boff.resize((size_t)nbig);
         for (int32_t i = 0; i < nbig; ++i) {
-            int64_t P = 8192;
-            while (P < lens[(size_t)rows[(size_t)i]]) P <<= 1;
-            soff[(size_t)i] = tot;
-            tot += P;
+            boff[(size_t)i] = big_total;
+            big_total += blen[(size_t)i];
         }
-        MR_TRY(d_scratch.alloc(sizeof(uint64_t) * (size_t)tot));
-        MR_TRY(d_soff.alloc(sizeof(int64_t) * (size_t)nbig));
-        MR_HIP_TRY(hipMemcpyAsync(d_soff.p, soff.data(), sizeof(int64_t) * (size_t)nbig, hipMemcpyHostToDevice, s));
-        MR_HIP_TRY(hipStreamSynchronize(s));
+        MR_TRY(d_boff.alloc(sizeof(int64_t) * (size_t)nbig));
+        MR_HIP_TRY(hipMemcpyAsync(d_boff.p, boff.data(), sizeof(int64_t) * (size_t)nbig, hipMemcpyHostToDevice, s));
+        MR_TRY(d_lws.alloc(k::perm_long_ws_bytes(big_total, nbig, n)));
     }
-    for (int c = 0; c < k::kPermClasses; ++c)
+    for (int c = 0; c < k::kPermClasses - 1; ++c)
         MR_HIP_TRY(k::perm_sort_class(c, counts[c], d_lists.as<int32_t>() + offs[c], rp, ci, val, val_bytes, perm,
-                                      inv, new_rp, new_ci, new_val, d_scratch.as<uint64_t>(),
-                                      d_soff.as<int64_t>(), s));
+                                      inv, new_rp, new_ci, new_val, s));
+    if (nbig > 0)
+        MR_HIP_TRY(k::perm_sort_long(nbig, big_rows, rp, ci, val, val_bytes, perm, inv, new_rp, new_ci, new_val,
+                                     d_boff.as<int64_t>(), big_total, n, d_lws.p, d_lws.bytes, s));
     MR_HIP_TRY(hipStreamSynchronize(s));
     (void)nnz;
     return MR_OK;

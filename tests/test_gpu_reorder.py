@@ -51,6 +51,35 @@ def test_permute_bit_exact(ctx, mr, oracle, case):
     mat.close()
 
 
+@pytest.mark.parametrize("store", ["F64", "F32", "BF16"])
+def test_permute_long_rows_all_widths(ctx, mr, oracle, store):
+    """Several rows past 4096 entries (sorted together by one radix sort),
+    repeated columns inside them (equal new columns keep their input order),
+    and 8-, 4- and 2-byte values (values exact in every width)."""
+    rng = np.random.default_rng(57)
+    n = 30000
+    long_len = {11: 4097, 500: 9000, 8191: 20000, 20000: 5000, 29999: 12000}
+    rows = []
+    for i in range(n):
+        if i in long_len:
+            r = np.sort(rng.choice(n, long_len[i], replace=True))   # repeats kept
+        else:
+            r = np.sort(rng.choice(n, int(rng.integers(0, 40)), replace=False))
+        rows.append(r)
+    rp, ci = csr_rows(rows, n)
+    val = rng.integers(-64, 64, len(ci)) / 8.0
+    perm = rng.permutation(n).astype(np.int32)
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, store=getattr(mr, store))
+    mat.permute(perm)
+    grp, gci, gval = mat.export()
+    orp, oci, oval = oracle.permute(n, rp, ci, val, perm)
+    assert np.array_equal(grp, orp) and np.array_equal(gci, oci)
+    if store == "BF16":   # exported as raw bfloat16 bits
+        gval = (gval.astype(np.uint32) << 16).view(np.float32)
+    assert np.array_equal(np.asarray(gval, np.float64), oval)
+    mat.close()
+
+
 def test_permute_rejects_non_permutation(ctx, mr):
     rp, ci = random_csr(np.random.default_rng(1), 10)
     mat = mr.Mat.from_csr(ctx, rp, ci)
