@@ -130,33 +130,39 @@ int merbit_xc_build(mr_mat m, hipStream_t s)
     int max_lds = 0;
     MR_HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->ctx->device));
     // as many hot columns as the workgroup's LDS holds
-    int64_t cap = max_lds > 512 ? ((int64_t)max_lds / 4 - 64) / 64 * 64 : 0;
+    const int64_t cap_lds = max_lds > 512 ? ((int64_t)max_lds / 4 - 64) / 64 * 64 : 0;
+    int64_t cap = cap_lds;
     if (p.xc_req >= 0) cap = std::min<int64_t>(cap, p.xc_req);
     cap = std::max<int64_t>(0, std::min<int64_t>(cap, m->n_cols));
     const int64_t nc = std::max<int64_t>(m->n_cols, 1);
-    // column degrees -> the `cap` columns with most nonzeros (ties: lower id)
-    std::vector<int32_t> deg((size_t)nc, 0);
-    if (m->nnz > 0 && cap > 0) {
-        DevBuf cnt;
-        MR_TRY(cnt.alloc(sizeof(int32_t) * (size_t)nc));
-        MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
-        MR_HIP_TRY(k::transpose_count(m->nnz, m->ci.as<int32_t>(), cnt.as<int32_t>(), s));
-        MR_HIP_TRY(hipMemcpyAsync(deg.data(), cnt.p, sizeof(int32_t) * (size_t)nc, hipMemcpyDeviceToHost, s));
-        MR_HIP_TRY(hipStreamSynchronize(s));
+    // column degrees -> the columns with most nonzeros (ties: lower id), once
+    // per structure for the largest cache; smaller caches take a prefix
+    if (!m->hot_valid) {
+        m->hot_order.clear();
+        m->hot_deg.clear();
+        const int64_t keep = std::min<int64_t>(cap_lds, m->n_cols);
+        if (m->nnz > 0 && keep > 0) {
+            std::vector<int32_t> deg((size_t)nc, 0);
+            DevBuf cnt;
+            MR_TRY(cnt.alloc(sizeof(int32_t) * (size_t)nc));
+            MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, cnt.bytes, s));
+            MR_HIP_TRY(k::transpose_count(m->nnz, m->ci.as<int32_t>(), cnt.as<int32_t>(), s));
+            MR_HIP_TRY(hipMemcpyAsync(deg.data(), cnt.p, sizeof(int32_t) * (size_t)nc, hipMemcpyDeviceToHost, s));
+            MR_HIP_TRY(hipStreamSynchronize(s));
+            std::vector<int32_t> idx((size_t)nc);
+            for (int64_t i = 0; i < nc; ++i) idx[(size_t)i] = (int32_t)i;
+            auto hotter = [&](int32_t a, int32_t b) {
+                return deg[(size_t)a] != deg[(size_t)b] ? deg[(size_t)a] > deg[(size_t)b] : a < b;
+            };
+            std::partial_sort(idx.begin(), idx.begin() + keep, idx.end(), hotter);
+            m->hot_order.assign(idx.begin(), idx.begin() + keep);
+            m->hot_deg.resize((size_t)keep);
+            for (int64_t i = 0; i < keep; ++i) m->hot_deg[(size_t)i] = deg[(size_t)idx[(size_t)i]];
+        }
+        m->hot_valid = true;
     }
-    std::vector<int32_t> idx((size_t)nc);
-    for (int64_t i = 0; i < nc; ++i) idx[(size_t)i] = (int32_t)i;
-    auto hotter = [&](int32_t a, int32_t b) {
-        return deg[(size_t)a] != deg[(size_t)b] ? deg[(size_t)a] > deg[(size_t)b] : a < b;
-    };
-    int64_t xc = cap;
-    if (xc > 0) {
-        std::partial_sort(idx.begin(), idx.begin() + xc, idx.end(), hotter);
-        while (xc > 0 && deg[(size_t)idx[(size_t)xc - 1]] == 0) --xc;   // never cache an unused column
-    }
-    std::vector<int32_t> map((size_t)nc);
-    for (int64_t i = 0; i < nc; ++i) map[(size_t)i] = (int32_t)i;
-    for (int64_t sl = 0; sl < xc; ++sl) map[(size_t)idx[(size_t)sl]] = (int32_t)(0x80000000u | (uint32_t)sl);
+    int64_t xc = std::min<int64_t>(cap, (int64_t)m->hot_order.size());
+    while (xc > 0 && m->hot_deg[(size_t)xc - 1] == 0) --xc;   // never cache an unused column
     const int64_t T = 64LL * p.xs;
     p.xtiles = (p.items + T - 1) / T;
     const int64_t padded = std::max<int64_t>(p.xtiles * T, T);
@@ -169,9 +175,10 @@ int merbit_xc_build(mr_mat m, hipStream_t s)
     MR_TRY(p.xty.alloc(sizeof(int32_t) * (size_t)(p.xtiles + 1)));
     MR_TRY(p.xcarry.alloc(sizeof(float) * (size_t)std::max<int64_t>(p.xtiles, 1)));
     MR_TRY(p.xcarry_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(p.xtiles, 1)));
-    MR_HIP_TRY(hipMemcpyAsync(dmap.p, map.data(), sizeof(int32_t) * (size_t)nc, hipMemcpyHostToDevice, s));
     if (xc > 0)
-        MR_HIP_TRY(hipMemcpyAsync(p.hot.p, idx.data(), sizeof(int32_t) * (size_t)xc, hipMemcpyHostToDevice, s));
+        MR_HIP_TRY(hipMemcpyAsync(p.hot.p, m->hot_order.data(), sizeof(int32_t) * (size_t)xc, hipMemcpyHostToDevice,
+                                  s));
+    MR_HIP_TRY(k::merbit_lp_map(nc, p.hot.as<int32_t>(), (int)xc, dmap.as<int32_t>(), s));
     MR_HIP_TRY(k::merbit_lp_build(m->dtype, m->n_rows, m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p,
                                   dmap.as<int32_t>(), p.items, padded, p.pc.as<int32_t>(), p.pv.p, s));
     MR_HIP_TRY(k::merbit_coords(m->n_rows, m->nnz, m->rp.as<int32_t>(), T, p.xtiles, tx.as<int32_t>(),

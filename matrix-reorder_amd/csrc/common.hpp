@@ -143,6 +143,10 @@ struct mr_mat_s {
     double avg_row = 0.0;
     mr::DevBuf long_rows;    // rows above 4096 nonzeros (CSR vector kernel)
     int32_t n_long = -1;     // -1: not computed for the current structure
+    // the columns with most nonzeros (ties: lower id), as many as an LDS x-cache
+    // can hold, and their counts; hot_valid = computed for the current structure
+    std::vector<int32_t> hot_order, hot_deg;
+    bool hot_valid = false;
     // plan-cache entry of the last cached reorder, and the fingerprint of
     // perm at that point (a later relabel makes the entry stale)
     std::string plan_path;

@@ -81,6 +81,9 @@ hipError_t merbit_spmv(int dt, int sigma, int policy, int64_t n_rows, int64_t n_
 // (S, W) in {(8,16), (16,16), (8,8), (16,8), (4,16)} (S = 4 for F32 only)
 size_t merbit_lp_lds_bytes(int xc);
 // map == NULL: plain columns; pv == NULL: the value stream is left alone
+// column map of the lane-padded stream: map[c] = c, except map[hot[i]] =
+// i | 1 << 31 for the xc LDS-cached columns
+hipError_t merbit_lp_map(int64_t nc, const int32_t* hot, int xc, int32_t* map, hipStream_t s);
 hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int32_t* ci, const void* val,
                            const int32_t* map, int64_t items, int64_t padded, int32_t* pc, void* pv, hipStream_t s);
 // SpMM on the lane-padded tiles for kk <= 16 columns: pcm = plain-column

@@ -555,6 +555,32 @@ hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t 
     });
 }
 
+namespace {
+
+__global__ void lp_map_identity_kernel(int64_t nc, int32_t* __restrict__ map)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x)
+        map[i] = (int32_t)i;
+}
+
+__global__ void lp_map_hot_kernel(const int32_t* __restrict__ hot, int xc, int32_t* __restrict__ map)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < xc; i += gridDim.x * blockDim.x)
+        map[hot[i]] = (int32_t)(0x80000000u | (uint32_t)i);
+}
+
+}  // namespace
+
+hipError_t merbit_lp_map(int64_t nc, const int32_t* hot, int xc, int32_t* map, hipStream_t s)
+{
+    if (nc <= 0) return hipSuccess;
+    int64_t g = (nc + 255) / 256;
+    if (g > 16384) g = 16384;
+    lp_map_identity_kernel<<<(unsigned)g, 256, 0, s>>>(nc, map);
+    if (xc > 0) lp_map_hot_kernel<<<(unsigned)((xc + 255) / 256), 256, 0, s>>>(hot, xc, map);
+    return hipGetLastError();
+}
+
 size_t merbit_lp_lds_bytes(int xc) { return sizeof(float) * (size_t)((xc + 63) / 64 * 64 + 64); }
 
 hipError_t merbit_lp_build(int dt, int64_t n_rows, const int32_t* rp, const int32_t* ci, const void* val,
