@@ -900,6 +900,10 @@ __global__ __launch_bounds__(256) void touched_kernel(int64_t n, const uint32_t*
         const uint64_t it = items[i];
         if ((it >> 32) == 0) hcol[(int32_t)(uint32_t)it] = 0;
     }
+    // a window that listed nothing changed no count: every count is back at
+    // zero after the reset, so the next list is empty (its size was cleared
+    // by final_kernel) and its histogram stays clear -- no scan of n counts
+    if (nit == 0) return;
     __shared__ int s_w[4];
     __shared__ int s_base, s_tot;
     __shared__ int32_t h[4096];
