@@ -126,6 +126,17 @@ int mr_csr_create(mr_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz,
  * to 1 when force_ones != 0. The stored CSR is the reference's multiplied
  * matrix (rows over the file's column index). */
 int mr_mtx_load(mr_ctx ctx, const char* path, int store_dtype, int force_ones, mr_mat* out);
+/* Graph files of the reference's GAP-benchmark reader (parsb/include/
+ * reader.h:50-162 and 334-400, built as builder.h:216-415 builds them), chosen
+ * by suffix: .el / .net ("u v" pairs), .wel ("u v w"), .gr (DIMACS "a u v w"
+ * lines, 1-based), .graph (METIS adjacency lines, 1-based; header format 1 =
+ * weighted), .sg / .wsg (GAPBS serialized CSR, read as stored).  Text
+ * formats: n = largest id + 1, one edge per (u, v) (the smallest weight),
+ * self-loops dropped, and with symmetrize != 0 every missing reverse edge
+ * added with the same weight.  Row u of the CSR holds the out-neighbours of
+ * u; values are the weights (1 for unweighted files), stored as store_dtype.
+ * A malformed file is MR_ERR_IO; another suffix is MR_ERR_UNSUPPORTED. */
+int mr_gap_load(mr_ctx ctx, const char* path, int symmetrize, int store_dtype, mr_mat* out);
 int mr_mat_destroy(mr_mat m);
 int mr_mat_info(mr_mat m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz, int* dtype);
 /* copy the (current, possibly relabelled) CSR back to the host; val in the

@@ -247,10 +247,22 @@ private:
     }
     std::ofstream* env_ = nullptr;
 
+    // GAP-benchmark suffixes go through mr_gap_load (their weights kept, 1 for
+    // unweighted formats); everything else is MatrixMarket, values forced to 1
+    static bool gap_suffix(const std::string& p)
+    {
+        const std::string::size_type dot = p.rfind('.');
+        if (dot == std::string::npos) return false;
+        const std::string s = p.substr(dot);
+        return s == ".el" || s == ".net" || s == ".wel" || s == ".gr" || s == ".graph" || s == ".sg" || s == ".wsg";
+    }
+
     void load(const ENV& _env, const std::string& path)
     {
         env_ = &_env.FSTR;
-        if (!MR_GRAPH_OK(mr_mtx_load(_env.ctx, path.c_str(), kDtype, 1, &mat))) return;
+        const int rc0 = gap_suffix(path) ? mr_gap_load(_env.ctx, path.c_str(), 0, kDtype, &mat)
+                                         : mr_mtx_load(_env.ctx, path.c_str(), kDtype, 1, &mat);
+        if (!MR_GRAPH_OK(rc0)) return;
         int64_t r = 0, c = 0, z = 0;
         int dt = 0;
         MR_GRAPH_OK(mr_mat_info(mat, &r, &c, &z, &dt));

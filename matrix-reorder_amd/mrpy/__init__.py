@@ -120,6 +120,7 @@ SIGNATURES = {
     "mr_gen_banded": [C.c_int64, C.c_int, C.c_int, C.c_uint64, _I32P, _I32P, _I64P],
     "mr_csr_create": [_P, C.c_int64, C.c_int64, C.c_int64, _I32P, _I32P, _P, C.c_int, C.c_int, C.POINTER(_P)],
     "mr_mtx_load": [_P, C.c_char_p, C.c_int, C.c_int, C.POINTER(_P)],
+    "mr_gap_load": [_P, C.c_char_p, C.c_int, C.c_int, C.POINTER(_P)],
     "mr_mat_destroy": [_P],
     "mr_mat_info": [_P, _I64P, _I64P, _I64P, C.POINTER(C.c_int)],
     "mr_mat_export": [_P, _I32P, _I32P, _P],
@@ -497,6 +498,13 @@ class Mat:
     def load_mtx(cls, ctx: Ctx, path: str, store=F32, force_ones=True):
         h = C.c_void_p()
         check(lib().mr_mtx_load(ctx.h, str(path).encode(), store, int(force_ones), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load_gap(cls, ctx: Ctx, path: str, symmetrize: bool = False, store=F32):
+        """A GAP-benchmark graph file (.el .net .wel .gr .graph .sg .wsg)."""
+        h = C.c_void_p()
+        check(lib().mr_gap_load(ctx.h, str(path).encode(), int(symmetrize), store, C.byref(h)))
         return cls(ctx, h)
 
     def close(self):

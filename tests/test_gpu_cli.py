@@ -103,3 +103,17 @@ def test_driver_runs_every_type(tmp_path):
                 assert float(f) >= 0.0, (typ, last)
     res = run_main(tmp_path, 99, "origin")
     assert res.returncode == 1 and "not found" in res.stderr
+
+
+def test_driver_reads_gap_edge_list(tmp_path):
+    """A config entry naming a GAP edge list (.el) goes through mr_gap_load."""
+    assert MR_MAIN.exists(), "build() links lib/mr_main"
+    rp, ci = rmat_csr(11, 8, seed=21)
+    n = len(rp) - 1
+    lines = [f"{i} {int(ci[q])}\n" for i in range(n) for q in range(rp[i], rp[i + 1])]
+    (tmp_path / "g.el").write_text("".join(lines))
+    (tmp_path / "config.txt").write_text(f"3\t{tmp_path / 'g.el'}\t{n}\t{n}\t{len(lines)}\n")
+    for typ in ("origin", "gorder6"):
+        res = run_main(tmp_path, 3, typ)
+        assert res.returncode == 0, (typ, res.stdout, res.stderr)
+        assert "total error = 0" in res.stdout, (typ, res.stdout)
