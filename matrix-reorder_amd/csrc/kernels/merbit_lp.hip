@@ -300,29 +300,53 @@ __global__ __launch_bounds__(64 * W) void merbit_lp_mm_kernel(int64_t n_tiles, c
         int row = inc - cnt;
         bool has_end = false;
         int head_row = 0;
+        // chunks of CH items: the chunk's X rows are gathered before it is
+        // walked -- global rows first, into registers of their own, then the
+        // LDS rows -- so the gathers of a chunk are in flight together
+        // instead of one per item between the row-end branches (k = 16 at
+        // s22: 859 -> 834 us).  Launches of 16 waves cap a lane at 128
+        // registers: their longest walks (S * G = 64) keep one item per chunk.
+        constexpr int CH = (SG <= 32 || W <= 8) ? (SG < 4 ? SG : 4) : 1;
 #pragma unroll
-        for (int j = 0; j < SG; ++j) {
-            const int32_t col = (int32_t)cw[j];
-            if (col == kRowEnd) {
-                if (!has_end) {
+        for (int j0 = 0; j0 < SG; j0 += CH) {
+            float4 xg[CH], xl[CH];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) head[e] = acc[e];
-                    head_row = row;
-                    has_end = true;
-                } else if (own) {
-                    float* yr = Y + (int64_t)(y0 + row) * ldy + 4 * q;
+            for (int c = 0; c < CH; ++c) {
+                const int32_t col = (int32_t)cw[j0 + c];
+                xg[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (own && col >= xr) xg[c] = *reinterpret_cast<const float4*>(X + (int64_t)col * ldx + 4 * q);
+            }
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (4 * q + e < kk) yr[e] = acc[e];
+            for (int c = 0; c < CH; ++c) {
+                const int32_t col = (int32_t)cw[j0 + c];
+                xl[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (own && col >= 0 && col < xr)
+                    xl[c] = *reinterpret_cast<const float4*>(xrows + (int64_t)col * K + 4 * q);
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int j = j0 + c;
+                const int32_t col = (int32_t)cw[j];
+                if (col == kRowEnd) {
+                    if (!has_end) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) head[e] = acc[e];
+                        head_row = row;
+                        has_end = true;
+                    } else if (own) {
+                        float* yr = Y + (int64_t)(y0 + row) * ldy + 4 * q;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (4 * q + e < kk) yr[e] = acc[e];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[e] = 0.0f;
+                    ++row;
+                } else if (col >= 0 && own) {
+                    const float v = value_at<VT>(vw, j);
+                    const float4 xv = col < xr ? xl[c] : xg[c];
+                    acc[0] += v * xv.x; acc[1] += v * xv.y; acc[2] += v * xv.z; acc[3] += v * xv.w;
                 }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[e] = 0.0f;
-                ++row;
-            } else if (col >= 0 && own) {
-                const float v = value_at<VT>(vw, j);
-                const float4 xv = col < xr ? *reinterpret_cast<const float4*>(xrows + (int64_t)col * K + 4 * q)
-                                           : *reinterpret_cast<const float4*>(X + (int64_t)col * ldx + 4 * q);
-                acc[0] += v * xv.x; acc[1] += v * xv.y; acc[2] += v * xv.z; acc[3] += v * xv.w;
             }
         }
 
@@ -511,7 +535,15 @@ int merbit_lp_mm_chunk(int kk) { return kk <= 4 ? 4 : kk <= 8 ? 8 : 16; }
 
 // shapes whose group registers fit without spilling (16 items per lane with
 // 16 waves leaves 128 VGPRs: only one lane per item fits)
-bool merbit_lp_mm_fits(int S, int W, int kk) { return !(S == 16 && W == 16 && kk > 4); }
+// waves per workgroup of an SpMM launch on tiles of S items per lane: the
+// SpMV's W, except that 16-item tiles with more than 8 columns (64-item walks
+// per lane group) launch 8 waves, whose lanes hold twice the registers
+int merbit_lp_mm_waves(int S, int W, int kk) { return (S == 16 && kk > 8) ? 8 : W; }
+bool merbit_lp_mm_fits(int S, int W, int kk)
+{
+    W = merbit_lp_mm_waves(S, W, kk);
+    return (S == 8 || S == 16) && (W == 8 || W == 16) && kk >= 1 && kk <= 16;
+}
 
 hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t n_tiles, const int32_t* pcm,
                           const void* pv, const int32_t* ty, const void* X, int64_t ldx, int kk, int xr, void* Y,
@@ -520,6 +552,7 @@ hipError_t merbit_lp_spmm(int dt, int S, int W, int nt, int64_t n_rows, int64_t 
     if (n_tiles <= 0 || n_rows <= 0 || kk <= 0) return hipSuccess;
     const int K = merbit_lp_mm_chunk(kk);
     if (!merbit_lp_mm_fits(S, W, kk)) return hipErrorInvalidValue;
+    W = merbit_lp_mm_waves(S, W, kk);
     if (dt == DT_F64 || grid <= 0 || kk > 16 || ldx < (kk + 3) / 4 * 4 || (ldx & 3) || ldy < kk ||
         (reinterpret_cast<uintptr_t>(X) & 15))
         return hipErrorInvalidValue;

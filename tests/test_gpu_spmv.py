@@ -277,6 +277,23 @@ def test_spmm_lane_padded(ctx, mr, oracle, case, k, ldx, store):
     first X rows: 16 columns per pass, ldx a multiple of 4 (ldx = 5 takes the
     workgroup-tile SpMM instead); 1e-5 of the FP64 oracle on the stored
     values, padding columns of Y untouched, bit-reproducible."""
+    check_lp_spmm(ctx, mr, oracle, case, k, ldx, store, None)
+
+
+SHAPE_CASES = [c for c in CASES if c[0] in ("rmat_s14", "long_rows", "stencil27")]
+
+
+@pytest.mark.parametrize("case", SHAPE_CASES, ids=[c[0] for c in SHAPE_CASES])
+@pytest.mark.parametrize("shape", [(8, 16), (16, 16), (8, 8), (16, 8)], ids=lambda s: f"S{s[0]}W{s[1]}")
+@pytest.mark.parametrize("k,ldx", [(4, 4), (6, 8), (16, 16), (33, 36)])
+@pytest.mark.parametrize("store", ["f32", "bf16"])
+def test_spmm_lane_padded_shapes(ctx, mr, oracle, case, shape, k, ldx, store):
+    """The same on every tile shape (items per lane, waves): 16-item tiles
+    with more than 8 columns launch 8 waves whatever the plan's count."""
+    check_lp_spmm(ctx, mr, oracle, case, k, ldx, store, shape)
+
+
+def check_lp_spmm(ctx, mr, oracle, case, k, ldx, store, shape):
     name, rp, ci, nc = case
     rng = np.random.default_rng(200 + k)
     val = rng.uniform(-1, 1, len(ci))
@@ -284,6 +301,8 @@ def test_spmm_lane_padded(ctx, mr, oracle, case, k, ldx, store):
     st = mr.F32 if store == "f32" else mr.BF16
     mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=st)
     mat.merbit_options(4)
+    if shape is not None:
+        mat.merbit_xc_configure(shape[0], shape[1], -1)
     _, _, vs = mat.export()
     v64 = vs.astype(np.float64) if st == mr.F32 else (vs.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
     n = len(rp) - 1
