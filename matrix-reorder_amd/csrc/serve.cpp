@@ -795,6 +795,10 @@ int mr_serve_add(mr_server s, mr_mat m, int* id)
         MR_HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     }
     std::lock_guard<std::mutex> g(s->mu);
+    // checked again: another add may have fixed the dtype while the metadata
+    // was built without the lock (the fused kernels read every matrix as it)
+    if (s->dtype >= 0 && m->dtype != s->dtype)
+        return set_error(MR_ERR_UNSUPPORTED, "serve: all matrices share one storage dtype (%d)", s->dtype);
     s->dtype = m->dtype;
     SMat M;
     M.m = m;
