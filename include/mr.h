@@ -302,6 +302,17 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s);
 /* MERBIT with a caller workspace of mr_merbit_ws_bytes() bytes */
 int mr_merbit_ws_bytes(mr_mat m, size_t* bytes);
 int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s);
+/* MERBIT SpMV that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * mr_spmv_ws's kernels with the carries in the library's static per-device
+ * scratch array of 64 MiB instead of a workspace argument.  The plan must
+ * already be built (a prior mr_spmv / mr_spmv_ws / mr_merbit_autotune on the
+ * handle; else MR_ERR_INVALID_ARG), and plans whose mr_merbit_ws_bytes exceeds
+ * 64 MiB are refused with MR_ERR_UNSUPPORTED.  Every call on one GPU shares
+ * that GPU's scratch array: calls that overlap in time on one device (other
+ * streams, other threads, other handles) race. */
+int mr_spmv_noalloc(mr_mat m, const void* x, void* y, mr_stream s);
 /* Y = A X with k right-hand sides, row-major X[n_cols][ldx], Y[n_rows][ldy] */
 int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, int64_t ldy,
             mr_stream s);

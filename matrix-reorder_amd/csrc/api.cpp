@@ -10,6 +10,7 @@
 #include "common.hpp"
 #include "kernels/launch.hpp"
 #include "kernels/launch_perm.hpp"
+#include "kernels/launch_scratch.hpp"
 
 namespace mr {
 
@@ -254,6 +255,7 @@ int mr_ctx_create(int device, mr_ctx* out)
     MR_HIP_TRY(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
+    MR_HIP_TRY(k::scratch_init(device, &c->scratch));
     *out = c.release();
     return MR_OK;
 }
@@ -646,15 +648,17 @@ int mr_merbit_ws_bytes(mr_mat m, size_t* bytes)
     return MR_OK;
 }
 
-int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s)
+// MERBIT SpMV with caller-provided carries; build = false refuses to build a
+// missing plan (the no-allocation entry point)
+static int spmv_ws_impl(mr_mat m, const void* x, void* y, void* ws, hipStream_t st, bool build)
 {
-    MR_TRY(check_mat(m));
-    if (!ws) return set_error(MR_ERR_INVALID_ARG, "null workspace");
-    if ((!x || !y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null x or y");
-    hipStream_t st = stream_of(m->ctx, s);
     MerbitPlan& p = m->merbit;
+    const bool lp = (p.policy & 4) && m->dtype != MR_F64;
+    if (!build && (!p.built || (lp && !p.xc_built)))
+        return set_error(MR_ERR_INVALID_ARG,
+                         "MERBIT plan not built: run mr_spmv, mr_spmv_ws or mr_merbit_autotune on the matrix first");
     if (!p.built) MR_TRY(merbit_build(m, st));
-    if ((p.policy & 4) && m->dtype != MR_F64) {
+    if (lp) {
         MR_TRY(merbit_xc_build(m, st));
         const size_t nt = (size_t)std::max<int64_t>(p.xtiles, 1);
         char* base = static_cast<char*>(ws);
@@ -674,6 +678,29 @@ int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s)
                               p.flags.as<uint32_t>(), p.blk_x.as<int32_t>(), p.blk_y.as<int32_t>(), x, y,
                               carry, crow, st));
     return MR_OK;
+}
+
+int mr_spmv_ws(mr_mat m, const void* x, void* y, void* ws, mr_stream s)
+{
+    MR_TRY(check_mat(m));
+    if (!ws) return set_error(MR_ERR_INVALID_ARG, "null workspace");
+    if ((!x || !y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null x or y");
+    return spmv_ws_impl(m, x, y, ws, stream_of(m->ctx, s), true);
+}
+
+int mr_spmv_noalloc(mr_mat m, const void* x, void* y, mr_stream s)
+{
+    MR_TRY(check_mat(m));
+    if ((!x || !y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null x or y");
+    if (!m->ctx->scratch) return set_error(MR_ERR_INVALID_ARG, "context has no static scratch address");
+    size_t need = 0;
+    MR_TRY(mr_merbit_ws_bytes(m, &need));
+    if (need > k::kScratchBytes)
+        return set_error(MR_ERR_UNSUPPORTED,
+                         "mr_spmv_noalloc: this plan needs %zu bytes of carries, above the %zu-byte static scratch "
+                         "buffer; use mr_spmv_ws",
+                         need, k::kScratchBytes);
+    return spmv_ws_impl(m, x, y, m->ctx->scratch, stream_of(m->ctx, s), false);
 }
 
 int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, int64_t ldy, mr_stream s)

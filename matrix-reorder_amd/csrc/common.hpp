@@ -91,6 +91,9 @@ struct mr_ctx_s {
     int cu_count = 0;
     std::string arch;
     std::mutex mu;
+    // device address of the library's static scratch array on this device
+    // (kernels/scratch.hip), looked up when the context is created
+    void* scratch = nullptr;
 };
 
 // MERBIT plan: merge-path tiles of 256 lanes x sigma items

@@ -155,6 +155,7 @@ SIGNATURES = {
     "mr_spmv": [_P, C.c_int, _P, _P, _P],
     "mr_merbit_ws_bytes": [_P, C.POINTER(C.c_size_t)],
     "mr_spmv_ws": [_P, _P, _P, _P, _P],
+    "mr_spmv_noalloc": [_P, _P, _P, _P],
     "mr_spmm": [_P, C.c_int, C.c_int, _P, C.c_int64, _P, C.c_int64, _P],
     "mr_merbit_configure": [_P, C.c_int],
     "mr_merbit_options": [_P, C.c_int],
@@ -682,6 +683,10 @@ class Mat:
 
     def spmv_ws(self, x: DeviceArray, y: DeviceArray, ws: DeviceArray, stream=None):
         check(lib().mr_spmv_ws(self.h, x.ptr, y.ptr, ws.ptr, stream))
+
+    def spmv_noalloc(self, x: DeviceArray, y: DeviceArray, stream=None):
+        """MERBIT SpMV with the carries in the library's static scratch array."""
+        check(lib().mr_spmv_noalloc(self.h, x.ptr, y.ptr, stream))
 
     def spmm(self, k: int, X: DeviceArray, Y: DeviceArray, kernel=MERBIT, ldx=None, ldy=None, stream=None):
         check(lib().mr_spmm(self.h, kernel, k, X.ptr, ldx or k, Y.ptr, ldy or k, stream))
