@@ -324,10 +324,11 @@ int mr_merbit_configure(mr_mat m, int sigma);
 /* load policy of the MERBIT kernel: bit 0 = non-temporal col/val loads and
  * y stores, bit 1 = XCD-contiguous block order, bit 2 = the wave-tile variant
  * with an LDS cache of the hottest columns of x (F32/F16/BF16 only; its tiles
- * are set by mr_merbit_xc_configure).  Default 4 for F32/F16/BF16 matrices
- * with at least 4,194,304 nonzeros, 1 otherwise.  Bits 0 and 1 do not change
- * results; bit 2 changes the tile partition and hence the rounding order
- * (results stay deterministic) */
+ * are set by mr_merbit_xc_configure), bit 3 = the column-panel variant (F32/
+ * F16/BF16 only; mr_merbit_panel_configure); bits 2 and 3 exclude each other.
+ * Default 4 for F32/F16/BF16 matrices with at least 4,194,304 nonzeros, 1
+ * otherwise.  Bits 0 and 1 do not change results; bits 2 and 3 change the
+ * partition and hence the rounding order (results stay deterministic) */
 int mr_merbit_options(mr_mat m, int policy);
 int mr_merbit_get_options(mr_mat m, int* policy);
 /* autotuner: times every (sigma, policy) on this matrix (reps launches each,
@@ -342,6 +343,18 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
  * variant's metadata (about (n_rows + nnz) * (4 + |val|) bytes). */
 int mr_merbit_xc_configure(mr_mat m, int items_per_lane, int waves, int cached_cols);
 int mr_merbit_xc_info(mr_mat m, int* items_per_lane, int* waves, int* cached_cols, int64_t* tiles);
+/* column-panel variant (no reference counterpart; replaces merbit_kernel's
+ * x gathers, include/merbit.cuh:408-493, by LDS reads): the columns are cut
+ * into panels of 32768; the nonzeros are stored panel-major as (16-bit panel
+ * column, value) items on tiles of 64 lanes x items_per_lane (8 or 16), and a
+ * persistent workgroup of `waves` (8 or 16) waves per CU loads one panel of x
+ * into LDS at a time.  Every row's run of nonzeros in a panel gives one partial
+ * sum; a second kernel adds each row's partials in panel order.  Rebuilds the
+ * variant's plan (nnz * (2 + |val|) bytes plus three int32 per partial). */
+int mr_merbit_panel_configure(mr_mat m, int items_per_lane, int waves);
+/* tiles, partials, segments and workgroups are -1 before the plan is built */
+int mr_merbit_panel_info(mr_mat m, int* items_per_lane, int* waves, int64_t* tiles, int64_t* partials,
+                         int64_t* segments, int* workgroups, double* build_ms);
 int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms);
 /* row-end bitmap over the merge path (bit p of word p/32 set when item p ends a
  * row) and the merge coordinates at every block start (+ sentinel) */

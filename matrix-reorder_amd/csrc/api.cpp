@@ -9,6 +9,7 @@
 
 #include "common.hpp"
 #include "kernels/launch.hpp"
+#include "kernels/launch_panel.hpp"
 #include "kernels/launch_perm.hpp"
 #include "kernels/launch_scratch.hpp"
 
@@ -535,6 +536,8 @@ int mr_mat_convert(mr_mat m, int dtype, mr_mat* out)
     c->merbit.xs = (dtype != MR_F32 && m->merbit.xs == 4) ? 8 : m->merbit.xs;
     c->merbit.xw = m->merbit.xw;
     c->merbit.xc_req = m->merbit.xc_req;
+    c->merbit.ps = m->merbit.ps;
+    c->merbit.pw = m->merbit.pw;
     MR_TRY(c->rp.alloc(m->rp.bytes));
     MR_TRY(c->ci.alloc(m->ci.bytes));
     MR_TRY(c->val.alloc(dtype_size(dtype) * (size_t)(m->nnz + kPad)));
@@ -598,6 +601,14 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
     case MR_KERNEL_MERBIT: {
         MerbitPlan& p = m->merbit;
         if (!p.built) MR_TRY(merbit_build(m, st));
+        if ((p.policy & 8) && m->dtype != MR_F64) {
+            MR_TRY(merbit_panel_build(m, st));
+            MR_HIP_TRY(k::panel_spmv(m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
+                                     p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg, x,
+                                     m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(), p.ppart.p, y,
+                                     st));
+            return MR_OK;
+        }
         if ((p.policy & 4) && m->dtype != MR_F64) {
             MR_TRY(merbit_xc_build(m, st));
             MR_HIP_TRY(k::merbit_lp_spmv(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles,
@@ -639,7 +650,16 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
 int mr_merbit_ws_bytes(mr_mat m, size_t* bytes)
 {
     if (!m || !bytes) return set_error(MR_ERR_INVALID_ARG, "null argument");
-    const MerbitPlan& p = m->merbit;
+    MerbitPlan& p = m->merbit;
+    if ((p.policy & 8) && m->dtype != MR_F64) {
+        // one float per partial sum of the column-panel plan
+        if (!p.pn_built) {
+            MR_TRY(check_mat(m));
+            MR_TRY(merbit_panel_build(m, m->ctx->stream));
+        }
+        *bytes = sizeof(float) * (size_t)std::max<int64_t>(p.pparts, 1);
+        return MR_OK;
+    }
     size_t nb = (size_t)(p.n_blocks > 0 ? p.n_blocks : 1);
     // the x-cache variant needs one carry per tile of 64 x xs items
     const int64_t xt = (p.items + 64LL * p.xs - 1) / (64LL * p.xs);
@@ -653,11 +673,19 @@ int mr_merbit_ws_bytes(mr_mat m, size_t* bytes)
 static int spmv_ws_impl(mr_mat m, const void* x, void* y, void* ws, hipStream_t st, bool build)
 {
     MerbitPlan& p = m->merbit;
-    const bool lp = (p.policy & 4) && m->dtype != MR_F64;
-    if (!build && (!p.built || (lp && !p.xc_built)))
+    const bool pn = (p.policy & 8) && m->dtype != MR_F64;
+    const bool lp = !pn && (p.policy & 4) && m->dtype != MR_F64;
+    if (!build && (!p.built || (lp && !p.xc_built) || (pn && !p.pn_built)))
         return set_error(MR_ERR_INVALID_ARG,
                          "MERBIT plan not built: run mr_spmv, mr_spmv_ws or mr_merbit_autotune on the matrix first");
     if (!p.built) MR_TRY(merbit_build(m, st));
+    if (pn) {
+        MR_TRY(merbit_panel_build(m, st));
+        MR_HIP_TRY(k::panel_spmv(m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
+                                 p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg, x,
+                                 m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(), ws, y, st));
+        return MR_OK;
+    }
     if (lp) {
         MR_TRY(merbit_xc_build(m, st));
         const size_t nt = (size_t)std::max<int64_t>(p.xtiles, 1);
@@ -693,6 +721,10 @@ int mr_spmv_noalloc(mr_mat m, const void* x, void* y, mr_stream s)
     MR_TRY(check_mat(m));
     if ((!x || !y) && m->n_rows > 0) return set_error(MR_ERR_INVALID_ARG, "null x or y");
     if (!m->ctx->scratch) return set_error(MR_ERR_INVALID_ARG, "context has no static scratch address");
+    const MerbitPlan& p = m->merbit;
+    if ((p.policy & 8) && m->dtype != MR_F64 && !p.pn_built)
+        return set_error(MR_ERR_INVALID_ARG,
+                         "MERBIT plan not built: run mr_spmv, mr_spmv_ws or mr_merbit_autotune on the matrix first");
     size_t need = 0;
     MR_TRY(mr_merbit_ws_bytes(m, &need));
     if (need > k::kScratchBytes)
@@ -717,7 +749,7 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
     }
     if (m->n_rows == 0) return MR_OK;
     MerbitPlan& p = m->merbit;
-    if ((p.policy & 4) && m->dtype != MR_F64 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+    if ((p.policy & 12) && m->dtype != MR_F64 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
         k::merbit_lp_mm_fits(p.xs, p.xw, std::min(k, 16))) {
         // lane-padded tiles: 16 columns per pass, X rows cached in LDS
         MR_TRY(merbit_lp_mm_build(m, st));
@@ -760,9 +792,11 @@ int mr_merbit_configure(mr_mat m, int sigma)
 int mr_merbit_options(mr_mat m, int policy)
 {
     if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
-    if (policy < 0 || policy > 7) return set_error(MR_ERR_INVALID_ARG, "policy must be in [0, 7]");
-    if ((policy & 4) && m->dtype == MR_F64)
-        return set_error(MR_ERR_UNSUPPORTED, "the x-cache MERBIT variant needs F32/F16/BF16 values");
+    if (policy < 0 || policy > 15) return set_error(MR_ERR_INVALID_ARG, "policy must be in [0, 15]");
+    if ((policy & 4) && (policy & 8))
+        return set_error(MR_ERR_INVALID_ARG, "policy bits 2 (x-cache tiles) and 3 (column panels) exclude each other");
+    if ((policy & 12) && m->dtype == MR_F64)
+        return set_error(MR_ERR_UNSUPPORTED, "the x-cache and column-panel MERBIT variants need F32/F16/BF16 values");
     m->merbit.policy = policy;
     return MR_OK;
 }
@@ -790,7 +824,7 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
     MR_HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     int bs = m->merbit.sigma, bp = m->merbit.policy, bxs = m->merbit.xs, bxw = m->merbit.xw,
-        bxc = m->merbit.xc_req;
+        bxc = m->merbit.xc_req, bps = m->merbit.ps, bpw = m->merbit.pw;
     auto time_it = [&](float* us) -> int {
         MR_TRY(mr_spmv(m, MR_KERNEL_MERBIT, x.p, y.p, s));
         MR_HIP_TRY(hipEventRecord(e0, s));
@@ -824,9 +858,13 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
     // wave tiles with the full LDS x-cache: (items per lane, waves) x streaming
     // hint.  Smaller caches (more resident workgroups per CU) were slower on
     // every matrix measured, so they are not searched.
+    // the best wave-tile shape is kept even when another variant wins: the
+    // SpMM of the column-panel variant runs on the wave tiles
+    int lxs = bxs, lxw = bxw;
     if (m->dtype != MR_F64) {
         const int shapes[5][2] = {{8, 16}, {16, 16}, {8, 8}, {16, 8}, {4, 16}};
         const int xc0 = m->merbit.xc_req;
+        float lbest = 1e30f;
         for (int q = 0; q < 5 && rc == MR_OK; ++q) {
             if (shapes[q][0] == 4 && m->dtype != MR_F32) continue;
             m->merbit.xs = shapes[q][0];
@@ -837,17 +875,44 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
                 m->merbit.policy = 4 | nt;
                 float us = 0;
                 rc = time_it(&us);
+                if (rc == MR_OK && us < lbest) {
+                    lbest = us; lxs = shapes[q][0]; lxw = shapes[q][1];
+                }
                 if (rc == MR_OK && us < best) {
                     best = us; bp = 4 | nt; bxs = shapes[q][0]; bxw = shapes[q][1]; bxc = -1;
                 }
             }
         }
         if (!(bp & 4)) m->merbit.xc_req = xc0;
+        // column panels: (items per lane, waves) x streaming hint; the plan
+        // depends on the items per lane only
+        const int pshapes[4][2] = {{8, 16}, {16, 16}, {8, 8}, {16, 8}};
+        for (int q = 0; q < 4 && rc == MR_OK; ++q) {
+            if (m->merbit.ps != pshapes[q][0]) m->merbit.pn_built = false;
+            m->merbit.ps = pshapes[q][0];
+            m->merbit.pw = pshapes[q][1];
+            for (int nt = 0; nt < 2 && rc == MR_OK; ++nt) {
+                m->merbit.policy = 8 | nt;
+                float us = 0;
+                rc = time_it(&us);
+                if (rc == MR_OK && us < best) {
+                    best = us; bp = 8 | nt; bps = pshapes[q][0]; bpw = pshapes[q][1];
+                }
+            }
+        }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc != MR_OK) return rc;
     m->merbit.policy = bp;
+    if (bp & 8) {
+        // panels won: wave tiles keep their own best shape for SpMM
+        bxs = lxs;
+        bxw = lxw;
+        if (m->merbit.ps != bps) m->merbit.pn_built = false;
+        m->merbit.ps = bps;
+        m->merbit.pw = bpw;
+    }
     if (m->merbit.xs != bxs || m->merbit.xw != bxw || ((bp & 4) && m->merbit.xc_req != bxc)) {
         m->merbit.xs = bxs;
         m->merbit.xw = bxw;
@@ -855,6 +920,7 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
         m->merbit.xc_built = false;
     }
     if (bp & 4) MR_TRY(merbit_xc_build(m, s));
+    if (bp & 8) MR_TRY(merbit_panel_build(m, s));
     if (best_sigma) *best_sigma = bs;
     if (best_policy) *best_policy = bp;
     if (best_us) *best_us = best;
@@ -878,6 +944,35 @@ int mr_merbit_xc_configure(mr_mat m, int items_per_lane, int waves, int cached_c
     m->merbit.xc_req = cached_cols;
     m->merbit.xc_built = false;
     return merbit_xc_build(m, m->ctx->stream);
+}
+
+int mr_merbit_panel_configure(mr_mat m, int items_per_lane, int waves)
+{
+    MR_TRY(check_mat(m));
+    if (m->dtype == MR_F64)
+        return set_error(MR_ERR_UNSUPPORTED, "the column-panel MERBIT variant needs F32/F16/BF16 values");
+    const int S = items_per_lane, W = waves;
+    if (!((S == 8 || S == 16) && (W == 8 || W == 16)))
+        return set_error(MR_ERR_INVALID_ARG, "column panels: items_per_lane must be 8 or 16 and waves 8 or 16");
+    if (m->merbit.ps != S) m->merbit.pn_built = false;
+    m->merbit.ps = S;
+    m->merbit.pw = W;
+    return merbit_panel_build(m, m->ctx->stream);
+}
+
+int mr_merbit_panel_info(mr_mat m, int* items_per_lane, int* waves, int64_t* tiles, int64_t* partials,
+                         int64_t* segments, int* workgroups, double* build_ms)
+{
+    if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");
+    const MerbitPlan& p = m->merbit;
+    if (items_per_lane) *items_per_lane = p.ps;
+    if (waves) *waves = p.pw;
+    if (tiles) *tiles = p.pn_built ? p.ptiles : -1;
+    if (partials) *partials = p.pn_built ? p.pparts : -1;
+    if (segments) *segments = p.pn_built ? p.psegs : -1;
+    if (workgroups) *workgroups = p.pn_built ? p.pn_wg : -1;
+    if (build_ms) *build_ms = p.pn_built ? p.pn_build_ms : 0.0;
+    return MR_OK;
 }
 
 int mr_merbit_xc_info(mr_mat m, int* items_per_lane, int* waves, int* cached_cols, int64_t* tiles)

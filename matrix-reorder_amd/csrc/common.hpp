@@ -129,6 +129,17 @@ struct MerbitPlan {
     // SpMM on the same tiles: plain-column stream and 16-wide carries
     mr::DevBuf pcm, xcarry_mm;
     bool pcm_built = false;
+    // column-panel variant (policy bit 3): x in LDS one panel of 32768 columns
+    // at a time; tiles of 64 x ps items, pw waves per workgroup; pcol / pval =
+    // panel-major streams, ptseg = first partial of every tile, pjobs / pwg =
+    // the workgroups' (panel, tile range) jobs, prs / pgidx = every row's
+    // partials in order, ppart = the partial sums (pparts floats)
+    int ps = 8, pw = 16;
+    int64_t ptiles = 0, pparts = 0, psegs = 0;
+    int pn_wg = 0, pn_jobs = 0;
+    mr::DevBuf pcol, pval, ptseg, pjobs, pwg, prs, pgidx, ppart;
+    bool pn_built = false;
+    double pn_build_ms = 0.0;
     int max_lds = 0;
     bool built = false;
     double build_ms = 0.0;
@@ -170,4 +181,6 @@ int merbit_xc_build(mr_mat m, hipStream_t s);
 int merbit_lp_mm_build(mr_mat m, hipStream_t s);
 // rows of X a LP SpMM launch of kk columns caches in LDS
 int merbit_lp_mm_rows(mr_mat m, int kk);
+// plan of the column-panel variant, built on first use (panel_plan.cpp)
+int merbit_panel_build(mr_mat m, hipStream_t s);
 }  // namespace mr

@@ -1,0 +1,483 @@
+// merbit_panel.hip -- MERBIT SpMV over column panels held in LDS (F32, F16 and
+// BF16 storage, FP32 accumulation).
+//
+// The merge-path SpMV (include/merbit.cuh:7-155, 408-493 in the reference)
+// gathers x[col] for every nonzero.  After pGOrder on a power-law matrix most
+// of those gathers are random 4-byte reads spread over all of x, and on
+// MI355X they are bound by outstanding L1 -> L2 requests, not by HBM bytes.
+// This variant removes every global gather:
+//
+//  * the columns are cut into panels of kPanelCols = 32768 ids.  A workgroup
+//    loads one panel of x (128 KB, coalesced) into LDS and multiplies only the
+//    nonzeros whose columns fall in it, so every x read is an LDS read;
+//  * a panel's nonzeros are stored panel-major: row by row, each row's run of
+//    nonzeros in the panel (a "segment") in column order.  An item is a 16-bit
+//    panel-local column with bit 15 marking the last item of a segment, and
+//    the value: 2 + |val| bytes, against 4 + |val| for CSR;
+//  * the stream is cut into wave tiles of 64 lanes x S items (lane l owns S
+//    consecutive items, 16-byte vector loads); every panel starts on a tile
+//    and a segment still open at a tile end is closed there, so no sum ever
+//    crosses a tile: there are no carries and no fix-up pass;
+//  * each closed segment writes one partial sum; tseg[t] = index of tile t's
+//    first partial.  A second kernel adds each row's partials, listed row by
+//    row in panel order (rs / gidx), in a fixed order: y is bit-reproducible.
+//
+// The plan (streams, partial lists and the workgroups' job table) is built on
+// the device by the kernels at the end of this file; the host only lays the
+// panels out on tiles and splits them into jobs (panel_plan.cpp).
+#include "dtypes.hpp"
+#include "launch_panel.hpp"
+#include "ldst.hpp"
+
+namespace mr {
+namespace k {
+
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool NT> __device__ __forceinline__ u32x4 ld_vec(const u32x4* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <typename VT> __device__ __forceinline__ float value_at(const uint32_t* w, int j);
+template <> __device__ __forceinline__ float value_at<float>(const uint32_t* w, int j) { return __uint_as_float(w[j]); }
+template <> __device__ __forceinline__ float value_at<__hip_bfloat16>(const uint32_t* w, int j)
+{
+    const uint32_t h = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
+    return __uint_as_float(h << 16);
+}
+template <> __device__ __forceinline__ float value_at<__half>(const uint32_t* w, int j)
+{
+    const uint32_t h = (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xffffu);
+    return __half2float(__ushort_as_half((unsigned short)h));
+}
+
+// ---- SpMV ------------------------------------------------------------------
+// jobs[3j .. 3j+2] = (panel, first tile, end tile); workgroup b runs jobs
+// [wg[b], wg[b+1]) in order.  part[tseg[t] + i] = i-th segment closed in tile t.
+template <typename VT, int S, int W, bool NT>
+__global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __restrict__ pcol,
+                                                            const VT* __restrict__ pval,
+                                                            const int32_t* __restrict__ tseg,
+                                                            const int32_t* __restrict__ jobs,
+                                                            const int32_t* __restrict__ wg,
+                                                            const float* __restrict__ x, int64_t n_cols,
+                                                            float* __restrict__ part)
+{
+    constexpr int T = 64 * S;
+    constexpr int CW = S / 2;                       // column words (two 16-bit items each)
+    constexpr int VW = S * (int)sizeof(VT) / 4;     // value words
+    static_assert(CW % 4 == 0 && VW % 4 == 0, "lanes load whole 16-byte vectors");
+    extern __shared__ float xs[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool x16 = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+
+    const int j1 = wg[blockIdx.x + 1];
+    for (int j = wg[blockIdx.x]; j < j1; ++j) {
+        const int32_t pn = jobs[3 * j], t0 = jobs[3 * j + 1], t1 = jobs[3 * j + 2];
+        const int64_t c0 = (int64_t)pn * kPanelCols;
+        const int nc = (int)((n_cols - c0) < kPanelCols ? (n_cols - c0) : kPanelCols);
+        __syncthreads();   // the previous job's tiles are done with xs
+        if (x16) {
+            const float4* xv = reinterpret_cast<const float4*>(x + c0);
+            for (int i = threadIdx.x; 4 * i < nc; i += 64 * W) {
+                if (4 * i + 3 < nc) {
+                    reinterpret_cast<float4*>(xs)[i] = xv[i];
+                } else {
+                    for (int e = 4 * i; e < nc; ++e) xs[e] = x[c0 + e];
+                }
+            }
+        } else {
+            for (int i = threadIdx.x; i < nc; i += 64 * W) xs[i] = x[c0 + i];
+        }
+        __syncthreads();
+
+        for (int32_t t = t0 + wid; t < t1; t += W) {
+            const int64_t base = (int64_t)t * T + (int64_t)lane * S;
+            uint32_t cw[CW], vw[VW];
+            const u32x4* cp = reinterpret_cast<const u32x4*>(pcol + base);
+            const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pval) +
+                                                             base * (int64_t)sizeof(VT));
+#pragma unroll
+            for (int q = 0; q < CW / 4; ++q) {
+                const u32x4 a = ld_vec<NT>(cp + q);
+                cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
+            }
+#pragma unroll
+            for (int q = 0; q < VW / 4; ++q) {
+                const u32x4 b = ld_vec<NT>(vp + q);
+                vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
+            }
+            const int32_t pbase = tseg[t];
+            float p[S];
+            uint32_t ends = 0;
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj) {
+                const uint32_t c16 = (jj & 1) ? (cw[jj >> 1] >> 16) : (cw[jj >> 1] & 0xffffu);
+                p[jj] = xs[c16 & 0x7fffu];
+                ends |= (c16 >> 15) << jj;
+            }
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj) p[jj] *= value_at<VT>(vw, jj);
+
+            // partials closed by earlier lanes of the wave
+            const int cnt = __popc(ends);
+            int inc = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += o;
+            }
+            int idx = pbase + inc - cnt;
+            float acc = 0.0f, head = 0.0f;
+            bool has_end = false;
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj) {
+                acc += p[jj];
+                if ((ends >> jj) & 1u) {
+                    if (!has_end) {
+                        head = acc;
+                        has_end = true;
+                    } else {
+                        part[idx] = acc;
+                    }
+                    ++idx;
+                    acc = 0.0f;
+                }
+            }
+            // segmented scan of (tail, has_end): the open sum each lane's first
+            // closed segment continues from the lanes before it
+            float sv = acc;
+            int fl = has_end ? 1 : 0;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const float vu = __shfl_up(sv, d, 64);
+                const int fu = __shfl_up(fl, d, 64);
+                if (lane >= d) {
+                    if (!fl) sv += vu;
+                    fl |= fu;
+                }
+            }
+            const float up = __shfl_up(sv, 1, 64);
+            if (has_end) part[pbase + inc - cnt] = (lane == 0 ? 0.0f : up) + head;
+        }
+    }
+}
+
+// y[r] = sum of row r's partials, in the order listed
+__global__ __launch_bounds__(256) void panel_reduce_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
+                                                           const int32_t* __restrict__ gidx,
+                                                           const float* __restrict__ part, float* __restrict__ y)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = rs[r], e = rs[r + 1];
+        float s = 0.0f;
+        for (int32_t j = b; j < e; ++j) s += part[gidx[j]];
+        y[r] = s;
+    }
+}
+
+template <typename VT, int S, int W, bool NT>
+hipError_t launch_panel(const uint16_t* pcol, const void* pval, const int32_t* tseg, const int32_t* jobs,
+                        const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
+                        const int32_t* rs, const int32_t* gidx, void* part, void* y, hipStream_t s)
+{
+    const size_t lds = sizeof(float) * (size_t)kPanelCols;
+    auto kern = panel_spmv_kernel<VT, S, W, NT>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    if (n_wg > 0) {
+        kern<<<(unsigned)n_wg, 64 * W, lds, s>>>(pcol, static_cast<const VT*>(pval), tseg, jobs, wg,
+                                                 static_cast<const float*>(x), n_cols, static_cast<float*>(part));
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    int64_t g = (n_rows + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    panel_reduce_kernel<<<(unsigned)g, 256, 0, s>>>(n_rows, rs, gidx, static_cast<const float*>(part),
+                                                    static_cast<float*>(y));
+    return hipGetLastError();
+}
+
+// ---- plan build --------------------------------------------------------------
+unsigned grid_of(int64_t work, int64_t cap = 65536)
+{
+    int64_t g = (work + 255) / 256;
+    if (g > cap) g = cap;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// F[k] = 1 where a segment starts: a row's first nonzero, or a nonzero whose
+// panel differs from its predecessor's in the row (one wave per row)
+__global__ __launch_bounds__(256) void pn_seg_flags_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci, int32_t* __restrict__ F)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rows; r += waves) {
+        const int32_t b = rp[r], e = rp[r + 1];
+        for (int32_t k = b + lane; k < e; k += 64)
+            F[k] = (k == b || (ci[k] >> kPanelBits) != (ci[k - 1] >> kPanelBits)) ? 1 : 0;
+    }
+}
+
+// per segment s (row-major order): first nonzero, panel key, identity payload
+__global__ void pn_seg_info_kernel(int64_t nnz, const int32_t* __restrict__ ci, const int32_t* __restrict__ F,
+                                   const int32_t* __restrict__ sid, int32_t* __restrict__ k0,
+                                   uint32_t* __restrict__ key, uint32_t* __restrict__ ident)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        if (!F[k]) continue;
+        const int32_t s = sid[k];
+        k0[s] = (int32_t)k;
+        key[s] = (uint32_t)(ci[k] >> kPanelBits);
+        ident[s] = (uint32_t)s;
+    }
+}
+
+__global__ void pn_set_tail_kernel(int32_t* a, int64_t at, const int64_t* total, int64_t fixed)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) a[at] = (int32_t)(total ? *total : fixed);
+}
+
+// len[i] = length of the i-th segment in panel order
+__global__ void pn_sorted_len_kernel(int64_t n_seg, const uint32_t* __restrict__ order, const int32_t* __restrict__ k0,
+                                     int32_t* __restrict__ len)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_seg; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = order[i];
+        len[i] = k0[s + 1] - k0[s];
+    }
+}
+
+// first segment (in panel order) and first unpadded item of every panel
+__global__ void pn_panel_starts_kernel(int64_t n_seg, const uint32_t* __restrict__ skey, const int32_t* __restrict__ X,
+                                       int32_t* __restrict__ pfirst, int32_t* __restrict__ pstart)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_seg; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i == 0 || skey[i] != skey[i - 1]) {
+            pfirst[skey[i]] = (int32_t)i;
+            pstart[skey[i]] = X[i];
+        }
+    }
+}
+
+// position of every segment's first item on the padded tile layout
+__global__ void pn_seg_pos_kernel(int64_t n_seg, const uint32_t* __restrict__ order, const uint32_t* __restrict__ skey,
+                                  const int32_t* __restrict__ X, const int32_t* __restrict__ padd,
+                                  int32_t* __restrict__ pos)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_seg; i += (int64_t)gridDim.x * blockDim.x)
+        pos[order[i]] = X[i] + padd[skey[i]];
+}
+
+__global__ void pn_zero_kernel(int64_t n, uint16_t* __restrict__ pcol, uint16_t* __restrict__ pv16,
+                               uint32_t* __restrict__ pv32, int32_t* __restrict__ flag)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        pcol[i] = 0;
+        if (pv16) pv16[i] = 0;
+        if (pv32) pv32[i] = 0u;
+        flag[i] = 0;
+    }
+}
+
+// items: 16-bit panel-local column (bit 15 = last item of a segment or of a
+// tile), value, and the end flag for the partial numbering
+template <typename VT>
+__global__ void pn_scatter_kernel(int64_t nnz, int tile, const int32_t* __restrict__ ci, const VT* __restrict__ val,
+                                  const int32_t* __restrict__ F, const int32_t* __restrict__ sid,
+                                  const int32_t* __restrict__ k0, const int32_t* __restrict__ pos,
+                                  uint16_t* __restrict__ pcol, VT* __restrict__ pval, int32_t* __restrict__ flag)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        // sid is the exclusive count of segment starts: the segment holding k
+        // is one less than the inclusive count
+        const int32_t s = sid[k] + F[k] - 1;
+        const int64_t q = (int64_t)pos[s] + (k - k0[s]);
+        const bool end = (k == (int64_t)k0[s + 1] - 1) || ((q % tile) == tile - 1);
+        pcol[q] = (uint16_t)((ci[k] & (kPanelCols - 1)) | (end ? 0x8000 : 0));
+        pval[q] = val[k];
+        flag[q] = end ? 1 : 0;
+    }
+}
+
+__global__ void pn_tile_base_kernel(int64_t n_tiles, int tile, const int32_t* __restrict__ pidx,
+                                    int32_t* __restrict__ tseg)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_tiles; t += (int64_t)gridDim.x * blockDim.x)
+        tseg[t] = pidx[t * tile];
+}
+
+__global__ void pn_pieces_kernel(int64_t n_seg, int tile, const int32_t* __restrict__ pos,
+                                 const int32_t* __restrict__ k0, int32_t* __restrict__ npieces)
+{
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n_seg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q0 = pos[s], len = k0[s + 1] - k0[s];
+        npieces[s] = (int32_t)((q0 + len - 1) / tile - q0 / tile + 1);
+    }
+}
+
+__global__ void pn_row_start_kernel(int64_t n_rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ sid,
+                                    const int32_t* __restrict__ goff, int32_t* __restrict__ rs)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= n_rows; r += (int64_t)gridDim.x * blockDim.x)
+        rs[r] = goff[sid[rp[r]]];
+}
+
+__global__ void pn_gidx_kernel(int64_t n_seg, int tile, const int32_t* __restrict__ pos, const int32_t* __restrict__ k0,
+                               const int32_t* __restrict__ goff, const int32_t* __restrict__ pidx,
+                               int32_t* __restrict__ gidx)
+{
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n_seg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q0 = pos[s], last = q0 + (k0[s + 1] - k0[s]) - 1;
+        int32_t o = goff[s];
+        for (int64_t tb = q0 / tile; tb <= last / tile; ++tb) {
+            const int64_t e = ((tb + 1) * tile - 1) < last ? ((tb + 1) * tile - 1) : last;
+            gidx[o++] = pidx[e];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t panel_seg_flags(int64_t n_rows, const int32_t* rp, const int32_t* ci, int32_t* F, hipStream_t s)
+{
+    if (n_rows <= 0) return hipSuccess;
+    int64_t g = (n_rows + 3) / 4;
+    if (g > 65536) g = 65536;
+    pn_seg_flags_kernel<<<(unsigned)g, 256, 0, s>>>(n_rows, rp, ci, F);
+    return hipGetLastError();
+}
+
+hipError_t panel_seg_info(int64_t nnz, const int32_t* ci, const int32_t* F, const int32_t* sid, int32_t* k0,
+                          uint32_t* key, uint32_t* ident, hipStream_t s)
+{
+    if (nnz <= 0) return hipSuccess;
+    pn_seg_info_kernel<<<grid_of(nnz), 256, 0, s>>>(nnz, ci, F, sid, k0, key, ident);
+    return hipGetLastError();
+}
+
+hipError_t panel_set_tail(int32_t* a, int64_t at, const int64_t* total, int64_t fixed, hipStream_t s)
+{
+    pn_set_tail_kernel<<<1, 64, 0, s>>>(a, at, total, fixed);
+    return hipGetLastError();
+}
+
+hipError_t panel_sorted_len(int64_t n_seg, const uint32_t* order, const int32_t* k0, int32_t* len, hipStream_t s)
+{
+    if (n_seg <= 0) return hipSuccess;
+    pn_sorted_len_kernel<<<grid_of(n_seg), 256, 0, s>>>(n_seg, order, k0, len);
+    return hipGetLastError();
+}
+
+hipError_t panel_starts(int64_t n_seg, const uint32_t* skey, const int32_t* X, int32_t* pfirst, int32_t* pstart,
+                        hipStream_t s)
+{
+    if (n_seg <= 0) return hipSuccess;
+    pn_panel_starts_kernel<<<grid_of(n_seg), 256, 0, s>>>(n_seg, skey, X, pfirst, pstart);
+    return hipGetLastError();
+}
+
+hipError_t panel_seg_pos(int64_t n_seg, const uint32_t* order, const uint32_t* skey, const int32_t* X,
+                         const int32_t* padd, int32_t* pos, hipStream_t s)
+{
+    if (n_seg <= 0) return hipSuccess;
+    pn_seg_pos_kernel<<<grid_of(n_seg), 256, 0, s>>>(n_seg, order, skey, X, padd, pos);
+    return hipGetLastError();
+}
+
+hipError_t panel_scatter(int dt, int64_t n_items, int64_t nnz, int tile, const int32_t* ci, const void* val,
+                         const int32_t* F, const int32_t* sid, const int32_t* k0, const int32_t* pos, uint16_t* pcol,
+                         void* pval, int32_t* flag, hipStream_t s)
+{
+    if (dt == DT_F64) return hipErrorInvalidValue;
+    if (n_items > 0) {
+        pn_zero_kernel<<<grid_of(n_items), 256, 0, s>>>(n_items, pcol, dt == DT_F32 ? nullptr : static_cast<uint16_t*>(pval),
+                                                         dt == DT_F32 ? static_cast<uint32_t*>(pval) : nullptr, flag);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nnz <= 0) return hipSuccess;
+    return dispatch_dt(dt, [&](auto v) -> hipError_t {
+        using VT = decltype(v);
+        pn_scatter_kernel<VT><<<grid_of(nnz), 256, 0, s>>>(nnz, tile, ci, static_cast<const VT*>(val), F, sid, k0, pos,
+                                                           pcol, static_cast<VT*>(pval), flag);
+        return hipGetLastError();
+    });
+}
+
+hipError_t panel_tile_base(int64_t n_tiles, int tile, const int32_t* pidx, int32_t* tseg, hipStream_t s)
+{
+    if (n_tiles <= 0) return hipSuccess;
+    pn_tile_base_kernel<<<grid_of(n_tiles), 256, 0, s>>>(n_tiles, tile, pidx, tseg);
+    return hipGetLastError();
+}
+
+hipError_t panel_pieces(int64_t n_seg, int tile, const int32_t* pos, const int32_t* k0, int32_t* npieces,
+                        hipStream_t s)
+{
+    if (n_seg <= 0) return hipSuccess;
+    pn_pieces_kernel<<<grid_of(n_seg), 256, 0, s>>>(n_seg, tile, pos, k0, npieces);
+    return hipGetLastError();
+}
+
+hipError_t panel_row_starts(int64_t n_rows, const int32_t* rp, const int32_t* sid, const int32_t* goff, int32_t* rs,
+                            hipStream_t s)
+{
+    pn_row_start_kernel<<<grid_of(n_rows + 1), 256, 0, s>>>(n_rows, rp, sid, goff, rs);
+    return hipGetLastError();
+}
+
+hipError_t panel_gidx(int64_t n_seg, int tile, const int32_t* pos, const int32_t* k0, const int32_t* goff,
+                      const int32_t* pidx, int32_t* gidx, hipStream_t s)
+{
+    if (n_seg <= 0) return hipSuccess;
+    pn_gidx_kernel<<<grid_of(n_seg), 256, 0, s>>>(n_seg, tile, pos, k0, goff, pidx, gidx);
+    return hipGetLastError();
+}
+
+size_t panel_lds_bytes() { return sizeof(float) * (size_t)kPanelCols; }
+
+hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const void* pval, const int32_t* tseg,
+                      const int32_t* jobs, const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
+                      const int32_t* rs, const int32_t* gidx, void* part, void* y, hipStream_t s)
+{
+    if (n_rows <= 0) return hipSuccess;
+    if (dt == DT_F64) return hipErrorInvalidValue;
+    return dispatch_dt(dt, [&](auto v) -> hipError_t {
+        using VT = decltype(v);
+        if constexpr (sizeof(VT) == 8) {
+            return hipErrorInvalidValue;
+        } else {
+            auto go = [&](auto stag, auto wtag) -> hipError_t {
+                constexpr int SS = decltype(stag)::value;
+                constexpr int WW = decltype(wtag)::value;
+                if constexpr ((SS / 2) % 4 != 0 || (SS * sizeof(VT)) % 16 != 0) {
+                    return hipErrorInvalidValue;
+                } else {
+                    return nt ? launch_panel<VT, SS, WW, true>(pcol, pval, tseg, jobs, wg, n_wg, x, n_cols, n_rows, rs,
+                                                               gidx, part, y, s)
+                              : launch_panel<VT, SS, WW, false>(pcol, pval, tseg, jobs, wg, n_wg, x, n_cols, n_rows,
+                                                                rs, gidx, part, y, s);
+                }
+            };
+            using I8 = std::integral_constant<int, 8>;
+            using I16 = std::integral_constant<int, 16>;
+            if (S == 8 && W == 16) return go(I8{}, I16{});
+            if (S == 16 && W == 16) return go(I16{}, I16{});
+            if (S == 8 && W == 8) return go(I8{}, I8{});
+            if (S == 16 && W == 8) return go(I16{}, I8{});
+            return hipErrorInvalidValue;
+        }
+    });
+}
+
+}  // namespace k
+}  // namespace mr

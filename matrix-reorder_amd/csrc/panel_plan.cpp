@@ -1,0 +1,267 @@
+// panel_plan.cpp -- plan of the column-panel MERBIT variant (policy bit 3).
+//
+// The device builds the panel-major streams in these steps (kernels in
+// kernels/merbit_panel.hip):
+//   1. segment starts: a row's first nonzero or a change of panel in the row;
+//      their prefix sum numbers the segments in row-major order (sid);
+//   2. a stable radix sort of the segments by panel gives panel-major order
+//      (rows ascending inside a panel);
+//   3. the host lays every panel on whole tiles and splits the tiles into the
+//      workgroups' jobs (below);
+//   4. every nonzero is scattered to its item; items closing a segment or a
+//      tile are flagged, and the prefix sum of the flags numbers the partials;
+//   5. every row lists its partials in row-major segment order (rs / gidx).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels/launch_panel.hpp"
+#include "kernels/launch_perm.hpp"
+#include "kernels/launch_radix.hpp"
+
+namespace mr {
+
+namespace {
+
+struct JobTable {
+    std::vector<int32_t> jobs;   // (panel, first tile, end tile) triples
+    std::vector<int32_t> wg;     // first job of every workgroup, plus the end
+};
+
+// Greedy split of the panels' tiles into at most `groups` contiguous job
+// lists of cost <= budget: a tile costs 1, loading a panel into LDS costs
+// `load` tiles.  Returns the number of workgroups used.
+int split_jobs(const std::vector<int32_t>& tile0, const std::vector<int32_t>& ntile, double budget, double load,
+               JobTable* out)
+{
+    out->jobs.clear();
+    out->wg.assign(1, 0);
+    double rem = budget;
+    bool open = false;   // the current workgroup has a job
+    for (size_t p = 0; p < ntile.size(); ++p) {
+        int32_t t = tile0[p], left = ntile[p];
+        while (left > 0) {
+            if (open && rem < load + 1.0) {
+                out->wg.push_back((int32_t)(out->jobs.size() / 3));
+                rem = budget;
+                open = false;
+            }
+            const int32_t take = (int32_t)std::max<double>(1.0, std::min<double>((double)left, std::floor(rem - load)));
+            out->jobs.push_back((int32_t)p);
+            out->jobs.push_back(t);
+            out->jobs.push_back(t + take);
+            rem -= take + load;
+            open = true;
+            t += take;
+            left -= take;
+        }
+    }
+    if (open) out->wg.push_back((int32_t)(out->jobs.size() / 3));
+    return (int)out->wg.size() - 1;
+}
+
+JobTable plan_jobs(const std::vector<int32_t>& tile0, const std::vector<int32_t>& ntile, int groups, double load)
+{
+    double total = 0.0;
+    for (size_t p = 0; p < ntile.size(); ++p)
+        if (ntile[p] > 0) total += ntile[p] + load;
+    JobTable best;
+    if (total <= 0.0) {
+        best.wg.assign(1, 0);
+        return best;
+    }
+    // smallest budget whose greedy split fits the workgroups
+    double lo = std::max(1.0, total / groups), hi = total + load + 1.0;
+    JobTable t;
+    if (split_jobs(tile0, ntile, lo, load, &t) <= groups) return t;
+    split_jobs(tile0, ntile, hi, load, &best);
+    for (int it = 0; it < 40 && hi - lo > 0.5; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (split_jobs(tile0, ntile, mid, load, &t) <= groups) {
+            hi = mid;
+            best = t;
+        } else {
+            lo = mid;
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+int merbit_panel_build(mr_mat m, hipStream_t s)
+{
+    MerbitPlan& p = m->merbit;
+    if (m->dtype == MR_F64) return set_error(MR_ERR_UNSUPPORTED, "the column-panel MERBIT variant needs F32/F16/BF16 values");
+    if (p.pn_built) return MR_OK;
+    const auto t_start = std::chrono::steady_clock::now();
+    const int64_t n = m->n_rows, nnz = m->nnz, nc = m->n_cols;
+    const int tile = 64 * p.ps;
+    const int64_t P = std::max<int64_t>(1, (nc + k::kPanelCols - 1) / k::kPanelCols);
+    if (nnz + P * (int64_t)tile >= INT32_MAX)
+        return set_error(MR_ERR_UNSUPPORTED, "column panels: %lld nonzeros exceed the int32 item index",
+                         (long long)nnz);
+    int key_bits = 1;
+    while (key_bits < 31 && ((int64_t)1 << key_bits) < P) ++key_bits;
+
+    DevBuf F, sid, total, scan_ws;
+    MR_TRY(F.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    MR_TRY(sid.alloc(sizeof(int32_t) * (size_t)(nnz + 1)));
+    MR_TRY(total.alloc(sizeof(int64_t) * 4));
+    // 1. segments in row-major order
+    MR_HIP_TRY(k::panel_seg_flags(n, m->rp.as<int32_t>(), m->ci.as<int32_t>(), F.as<int32_t>(), s));
+    int64_t n_seg = 0;
+    if (nnz > 0) {
+        MR_TRY(scan_ws.alloc(k::scan_workspace_bytes(nnz)));
+        MR_HIP_TRY(k::exclusive_scan_i32(F.as<int32_t>(), nnz, sid.as<int32_t>(), total.as<int64_t>(), scan_ws.p, s));
+        MR_HIP_TRY(k::panel_set_tail(sid.as<int32_t>(), nnz, total.as<int64_t>(), 0, s));
+        MR_HIP_TRY(hipMemcpyAsync(&n_seg, total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        MR_HIP_TRY(k::panel_set_tail(sid.as<int32_t>(), 0, nullptr, 0, s));
+    }
+    const int64_t ns1 = std::max<int64_t>(n_seg, 1);
+    DevBuf k0, key, ident, skey, order;
+    MR_TRY(k0.alloc(sizeof(int32_t) * (size_t)(n_seg + 1)));
+    MR_TRY(key.alloc(sizeof(uint32_t) * (size_t)ns1));
+    MR_TRY(ident.alloc(sizeof(uint32_t) * (size_t)ns1));
+    MR_TRY(skey.alloc(sizeof(uint32_t) * (size_t)ns1));
+    MR_TRY(order.alloc(sizeof(uint32_t) * (size_t)ns1));
+    MR_HIP_TRY(k::panel_seg_info(nnz, m->ci.as<int32_t>(), F.as<int32_t>(), sid.as<int32_t>(), k0.as<int32_t>(),
+                                 key.as<uint32_t>(), ident.as<uint32_t>(), s));
+    MR_HIP_TRY(k::panel_set_tail(k0.as<int32_t>(), n_seg, nullptr, nnz, s));
+    // 2. panel-major order of the segments (stable: rows stay ascending)
+    if (n_seg > 0) {
+        DevBuf sort_ws;
+        const size_t wsb = k::radix_sort_ws_bytes(n_seg);
+        MR_TRY(sort_ws.alloc(wsb));
+        MR_HIP_TRY(k::radix_sort_pairs(key.as<uint32_t>(), ident.as<uint32_t>(), skey.as<uint32_t>(),
+                                       order.as<uint32_t>(), n_seg, key_bits, sort_ws.p, wsb, s));
+    }
+    key.release();
+    ident.release();
+    DevBuf len, X, pfirst, pstart;
+    MR_TRY(len.alloc(sizeof(int32_t) * (size_t)ns1));
+    MR_TRY(X.alloc(sizeof(int32_t) * (size_t)(n_seg + 1)));
+    MR_TRY(pfirst.alloc(sizeof(int32_t) * (size_t)P));
+    MR_TRY(pstart.alloc(sizeof(int32_t) * (size_t)P));
+    MR_HIP_TRY(hipMemsetAsync(pfirst.p, 0xff, pfirst.bytes, s));
+    if (n_seg > 0) {
+        DevBuf ws2;
+        MR_TRY(ws2.alloc(k::scan_workspace_bytes(n_seg)));
+        MR_HIP_TRY(k::panel_sorted_len(n_seg, order.as<uint32_t>(), k0.as<int32_t>(), len.as<int32_t>(), s));
+        MR_HIP_TRY(k::exclusive_scan_i32(len.as<int32_t>(), n_seg, X.as<int32_t>(), nullptr, ws2.p, s));
+        MR_HIP_TRY(k::panel_starts(n_seg, skey.as<uint32_t>(), X.as<int32_t>(), pfirst.as<int32_t>(),
+                                   pstart.as<int32_t>(), s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+    }
+    len.release();
+    // 3. panels on whole tiles, tiles into jobs
+    std::vector<int32_t> h_first((size_t)P), h_start((size_t)P), tile0((size_t)P, 0), ntile((size_t)P, 0),
+        padd((size_t)P, 0);
+    MR_HIP_TRY(hipMemcpyAsync(h_first.data(), pfirst.p, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipMemcpyAsync(h_start.data(), pstart.p, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    int64_t tiles = 0;
+    for (int64_t q = 0; q < P; ++q) {
+        if (h_first[(size_t)q] < 0) continue;
+        int64_t next = nnz;
+        for (int64_t r = q + 1; r < P; ++r)
+            if (h_first[(size_t)r] >= 0) {
+                next = h_start[(size_t)r];
+                break;
+            }
+        const int64_t items = next - h_start[(size_t)q];
+        const int64_t nt = (items + tile - 1) / tile;
+        tile0[(size_t)q] = (int32_t)tiles;
+        ntile[(size_t)q] = (int32_t)nt;
+        padd[(size_t)q] = (int32_t)(tiles * tile - h_start[(size_t)q]);
+        tiles += nt;
+    }
+    const int64_t n_items = tiles * tile;
+    const double tile_bytes = (double)tile * (2.0 + (double)dtype_size(m->dtype));
+    const double load = (double)k::panel_lds_bytes() / tile_bytes;
+    const JobTable jt = plan_jobs(tile0, ntile, std::max(1, m->ctx->cu_count), load);
+    DevBuf d_padd, pos;
+    MR_TRY(d_padd.alloc(sizeof(int32_t) * (size_t)P));
+    MR_TRY(pos.alloc(sizeof(int32_t) * (size_t)ns1));
+    MR_HIP_TRY(hipMemcpyAsync(d_padd.p, padd.data(), sizeof(int32_t) * (size_t)P, hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(k::panel_seg_pos(n_seg, order.as<uint32_t>(), skey.as<uint32_t>(), X.as<int32_t>(),
+                                d_padd.as<int32_t>(), pos.as<int32_t>(), s));
+    skey.release();
+    order.release();
+    X.release();
+    // 4. items and the partial numbering
+    DevBuf flag, pidx;
+    MR_TRY(p.pcol.alloc(sizeof(uint16_t) * (size_t)std::max<int64_t>(n_items, 1)));
+    MR_TRY(p.pval.alloc(dtype_size(m->dtype) * (size_t)std::max<int64_t>(n_items, 1)));
+    MR_TRY(flag.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(n_items, 1)));
+    MR_TRY(pidx.alloc(sizeof(int32_t) * (size_t)(n_items + 1)));
+    MR_HIP_TRY(k::panel_scatter(m->dtype, n_items, nnz, tile, m->ci.as<int32_t>(), m->val.p, F.as<int32_t>(),
+                                sid.as<int32_t>(), k0.as<int32_t>(), pos.as<int32_t>(), p.pcol.as<uint16_t>(), p.pval.p,
+                                flag.as<int32_t>(), s));
+    int64_t n_parts = 0;
+    if (n_items > 0) {
+        DevBuf ws3;
+        MR_TRY(ws3.alloc(k::scan_workspace_bytes(n_items)));
+        MR_HIP_TRY(k::exclusive_scan_i32(flag.as<int32_t>(), n_items, pidx.as<int32_t>(), total.as<int64_t>(), ws3.p,
+                                         s));
+        MR_HIP_TRY(k::panel_set_tail(pidx.as<int32_t>(), n_items, total.as<int64_t>(), 0, s));
+        MR_HIP_TRY(hipMemcpyAsync(&n_parts, total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        MR_HIP_TRY(k::panel_set_tail(pidx.as<int32_t>(), 0, nullptr, 0, s));
+    }
+    flag.release();
+    MR_TRY(p.ptseg.alloc(sizeof(int32_t) * (size_t)(tiles + 1)));
+    MR_HIP_TRY(k::panel_tile_base(tiles, tile, pidx.as<int32_t>(), p.ptseg.as<int32_t>(), s));
+    MR_HIP_TRY(k::panel_set_tail(p.ptseg.as<int32_t>(), tiles, nullptr, n_parts, s));
+    // 5. every row's partials, in row-major segment order
+    DevBuf npieces, goff;
+    MR_TRY(npieces.alloc(sizeof(int32_t) * (size_t)ns1));
+    MR_TRY(goff.alloc(sizeof(int32_t) * (size_t)(n_seg + 1)));
+    int64_t n_pieces = 0;
+    if (n_seg > 0) {
+        DevBuf ws4;
+        MR_TRY(ws4.alloc(k::scan_workspace_bytes(n_seg)));
+        MR_HIP_TRY(k::panel_pieces(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), npieces.as<int32_t>(), s));
+        MR_HIP_TRY(k::exclusive_scan_i32(npieces.as<int32_t>(), n_seg, goff.as<int32_t>(), total.as<int64_t>() + 1,
+                                         ws4.p, s));
+        MR_HIP_TRY(k::panel_set_tail(goff.as<int32_t>(), n_seg, total.as<int64_t>() + 1, 0, s));
+        MR_HIP_TRY(hipMemcpyAsync(&n_pieces, total.as<int64_t>() + 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        MR_HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        MR_HIP_TRY(k::panel_set_tail(goff.as<int32_t>(), 0, nullptr, 0, s));
+    }
+    if (n_pieces != n_parts)
+        return set_error(MR_ERR_HIP, "column panels: %lld row pieces but %lld partials", (long long)n_pieces,
+                         (long long)n_parts);
+    MR_TRY(p.prs.alloc(sizeof(int32_t) * (size_t)(n + 1)));
+    MR_TRY(p.pgidx.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(n_parts, 1)));
+    MR_TRY(p.ppart.alloc(sizeof(float) * (size_t)std::max<int64_t>(n_parts, 1)));
+    MR_HIP_TRY(k::panel_row_starts(n, m->rp.as<int32_t>(), sid.as<int32_t>(), goff.as<int32_t>(), p.prs.as<int32_t>(),
+                                   s));
+    MR_HIP_TRY(k::panel_gidx(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), goff.as<int32_t>(), pidx.as<int32_t>(),
+                             p.pgidx.as<int32_t>(), s));
+    MR_TRY(p.pjobs.alloc(sizeof(int32_t) * std::max<size_t>(jt.jobs.size(), 3)));
+    MR_TRY(p.pwg.alloc(sizeof(int32_t) * jt.wg.size()));
+    if (!jt.jobs.empty())
+        MR_HIP_TRY(hipMemcpyAsync(p.pjobs.p, jt.jobs.data(), sizeof(int32_t) * jt.jobs.size(), hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(hipMemcpyAsync(p.pwg.p, jt.wg.data(), sizeof(int32_t) * jt.wg.size(), hipMemcpyHostToDevice, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));   // temporaries are freed on return
+    p.ptiles = tiles;
+    p.pparts = n_parts;
+    p.psegs = n_seg;
+    p.pn_wg = (int)jt.wg.size() - 1;
+    p.pn_jobs = (int)(jt.jobs.size() / 3);
+    p.pn_built = true;
+    p.pn_build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return MR_OK;
+}
+
+}  // namespace mr

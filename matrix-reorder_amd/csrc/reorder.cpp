@@ -146,6 +146,7 @@ int relabel(mr_mat m, const std::vector<int32_t>& perm, double* ms_out)
     for (auto& p : m->perm) p = perm[(size_t)p];
     m->n_long = -1;   // row lengths moved: the long-row list is rebuilt on next use
     m->hot_valid = false;   // column ids moved: the hot columns are recounted on next use
+    m->merbit.pn_built = false;   // and the column panels are rebuilt
     if (ms_out) *ms_out = ms_since(t0);
     return merbit_build(m, s);
 }

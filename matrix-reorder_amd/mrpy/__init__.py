@@ -164,6 +164,9 @@ SIGNATURES = {
     "mr_merbit_info": [_P, C.POINTER(C.c_int), _I64P, _I64P, _DP],
     "mr_merbit_export": [_P, _U32P, _I32P, _I32P],
     "mr_merbit_xc_configure": [_P, C.c_int, C.c_int, C.c_int],
+    "mr_merbit_panel_configure": [_P, C.c_int, C.c_int],
+    "mr_merbit_panel_info": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64P, _I64P, _I64P, C.POINTER(C.c_int),
+                             _DP],
     "mr_merbit_xc_info": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I64P],
     "mr_batch_create": [_P, C.c_int, C.POINTER(_P), C.c_int, C.POINTER(_P)],
     "mr_batch_destroy": [_P],
@@ -714,6 +717,17 @@ class Mat:
         s, w, c, t = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
         check(lib().mr_merbit_xc_info(self.h, C.byref(s), C.byref(w), C.byref(c), C.byref(t)))
         return {"items_per_lane": s.value, "waves": w.value, "cached_cols": c.value, "tiles": t.value}
+
+    def merbit_panel_configure(self, items_per_lane: int = 8, waves: int = 16):
+        check(lib().mr_merbit_panel_configure(self.h, items_per_lane, waves))
+
+    def merbit_panel_info(self):
+        s, w, wg = C.c_int(), C.c_int(), C.c_int()
+        t, pa, sg, ms = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        check(lib().mr_merbit_panel_info(self.h, C.byref(s), C.byref(w), C.byref(t), C.byref(pa), C.byref(sg),
+                                         C.byref(wg), C.byref(ms)))
+        return {"items_per_lane": s.value, "waves": w.value, "tiles": t.value, "partials": pa.value,
+                "segments": sg.value, "workgroups": wg.value, "build_ms": ms.value}
 
     def merbit_info(self):
         s, it, nb, ms = C.c_int(), C.c_int64(), C.c_int64(), C.c_double()
