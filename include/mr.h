@@ -346,9 +346,9 @@ int mr_merbit_xc_info(mr_mat m, int* items_per_lane, int* waves, int* cached_col
 /* column-panel variant (no reference counterpart; replaces merbit_kernel's
  * x gathers, include/merbit.cuh:408-493, by LDS reads): the columns are cut
  * into panels of 32768; the nonzeros are stored panel-major as (16-bit panel
- * column, value) items on tiles of 64 lanes x items_per_lane (8 or 16), and a
- * persistent workgroup of `waves` (8 or 16) waves per CU loads one panel of x
- * into LDS at a time.  Every row's run of nonzeros in a panel gives one partial
+ * column, value) items on tiles of 64 lanes x items_per_lane, and a persistent
+ * workgroup of `waves` waves per CU loads one panel of x into LDS at a time;
+ * (items_per_lane, waves) is (8, 16), (8, 8) or (16, 8).  Every row's run of nonzeros in a panel gives one partial
  * sum; a second kernel adds each row's partials in panel order.  Rebuilds the
  * variant's plan (nnz * (2 + |val|) bytes plus three int32 per partial). */
 int mr_merbit_panel_configure(mr_mat m, int items_per_lane, int waves);

@@ -605,8 +605,8 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
             MR_TRY(merbit_panel_build(m, st));
             MR_HIP_TRY(k::panel_spmv(m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
                                      p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg, x,
-                                     m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(), p.ppart.p, y,
-                                     st));
+                                     m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
+                                     p.pwrow.as<int32_t>(), p.pn_chunks, p.pchunk_max, p.ppart.p, y, st));
             return MR_OK;
         }
         if ((p.policy & 4) && m->dtype != MR_F64) {
@@ -683,7 +683,8 @@ static int spmv_ws_impl(mr_mat m, const void* x, void* y, void* ws, hipStream_t 
         MR_TRY(merbit_panel_build(m, st));
         MR_HIP_TRY(k::panel_spmv(m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
                                  p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg, x,
-                                 m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(), ws, y, st));
+                                 m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
+                                 p.pwrow.as<int32_t>(), p.pn_chunks, p.pchunk_max, ws, y, st));
         return MR_OK;
     }
     if (lp) {
@@ -886,8 +887,8 @@ int mr_merbit_autotune(mr_mat m, int reps, int* best_sigma, int* best_policy, fl
         if (!(bp & 4)) m->merbit.xc_req = xc0;
         // column panels: (items per lane, waves) x streaming hint; the plan
         // depends on the items per lane only
-        const int pshapes[4][2] = {{8, 16}, {16, 16}, {8, 8}, {16, 8}};
-        for (int q = 0; q < 4 && rc == MR_OK; ++q) {
+        const int pshapes[3][2] = {{8, 16}, {8, 8}, {16, 8}};
+        for (int q = 0; q < 3 && rc == MR_OK; ++q) {
             if (m->merbit.ps != pshapes[q][0]) m->merbit.pn_built = false;
             m->merbit.ps = pshapes[q][0];
             m->merbit.pw = pshapes[q][1];
@@ -952,8 +953,9 @@ int mr_merbit_panel_configure(mr_mat m, int items_per_lane, int waves)
     if (m->dtype == MR_F64)
         return set_error(MR_ERR_UNSUPPORTED, "the column-panel MERBIT variant needs F32/F16/BF16 values");
     const int S = items_per_lane, W = waves;
-    if (!((S == 8 || S == 16) && (W == 8 || W == 16)))
-        return set_error(MR_ERR_INVALID_ARG, "column panels: items_per_lane must be 8 or 16 and waves 8 or 16");
+    if (!((S == 8 && (W == 8 || W == 16)) || (S == 16 && W == 8)))
+        return set_error(MR_ERR_INVALID_ARG,
+                         "column panels: (items_per_lane, waves) must be (8, 16), (8, 8) or (16, 8)");
     if (m->merbit.ps != S) m->merbit.pn_built = false;
     m->merbit.ps = S;
     m->merbit.pw = W;

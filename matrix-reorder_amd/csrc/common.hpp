@@ -137,7 +137,10 @@ struct MerbitPlan {
     int ps = 8, pw = 16;
     int64_t ptiles = 0, pparts = 0, psegs = 0;
     int pn_wg = 0, pn_jobs = 0;
-    mr::DevBuf pcol, pval, ptseg, pjobs, pwg, prs, pgidx, ppart;
+    // pwrow = pn_chunks + 1 row bounds of the partial-sum chunks (at most
+    // pchunk_max partials each)
+    int pn_chunks = 0, pchunk_max = 0;
+    mr::DevBuf pcol, pval, ptseg, pjobs, pwg, prs, pgidx, ppart, pwrow;
     bool pn_built = false;
     double pn_build_ms = 0.0;
     int max_lds = 0;

@@ -11,12 +11,21 @@ namespace k {
 constexpr int kPanelBits = 15;
 constexpr int kPanelCols = 1 << kPanelBits;   // columns per panel: 128 KB of F32 x in LDS
 
-// LDS bytes of one panel of x
-size_t panel_lds_bytes();
-// y = A x over the panel plan; part holds one float per partial
+// LDS bytes of a workgroup: one panel of x plus every wave's partial staging
+size_t panel_lds_bytes(int items_per_lane, int waves);
+// y = A x over the panel plan; part holds one float per partial.  wrow
+// (n_chunks + 1 rows) cuts the rows for the partial sums; chunks of at most
+// chunk_max partials are added through LDS (chunk_max * 4 bytes <=
+// panel_reduce_lds_cap()), else straight from global memory
 hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const void* pval, const int32_t* tseg,
                       const int32_t* jobs, const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
-                      const int32_t* rs, const int32_t* gidx, void* part, void* y, hipStream_t s);
+                      const int32_t* rs, const int32_t* gidx, const int32_t* wrow, int n_chunks, int chunk_max,
+                      void* part, void* y, hipStream_t s);
+// LDS bytes the partial-sum kernel may use for one chunk
+size_t panel_reduce_lds_cap();
+// partial-balanced row chunks: wrow[0..n_chunks], *maxc = max chunk (atomicMax
+// into a caller-zeroed int)
+hipError_t panel_chunks(int64_t n_rows, const int32_t* rs, int n_chunks, int32_t* wrow, int32_t* maxc, hipStream_t s);
 
 // plan build steps (see panel_plan.cpp for the order)
 hipError_t panel_seg_flags(int64_t n_rows, const int32_t* rp, const int32_t* ci, int32_t* F, hipStream_t s);

@@ -15,7 +15,9 @@
 //    panel-local column with bit 15 marking the last item of a segment, and
 //    the value: 2 + |val| bytes, against 4 + |val| for CSR;
 //  * the stream is cut into wave tiles of 64 lanes x S items (lane l owns S
-//    consecutive items, 16-byte vector loads); every panel starts on a tile
+//    consecutive items).  Inside a tile the 16-byte chunks are stored chunk
+//    row by chunk row (lane l's chunk c at c * 64 + l), so every vector load
+//    of a wave reads 1 KB of consecutive bytes; every panel starts on a tile
 //    and a segment still open at a tile end is closed there, so no sum ever
 //    crosses a tile: there are no carries and no fix-up pass;
 //  * each closed segment writes one partial sum; tseg[t] = index of tile t's
@@ -56,6 +58,101 @@ template <> __device__ __forceinline__ float value_at<__half>(const uint32_t* w,
 }
 
 // ---- SpMV ------------------------------------------------------------------
+// the 16-byte vectors of lane `lane` in tile t, and the tile's first partial
+template <typename VT, int S, bool NT>
+__device__ __forceinline__ void load_tile(const uint16_t* __restrict__ pcol, const VT* __restrict__ pval,
+                                          const int32_t* __restrict__ tseg, int32_t t, int lane, uint32_t* cw,
+                                          uint32_t* vw, int32_t* pb)
+{
+    constexpr int T = 64 * S;
+    constexpr int CW = S / 2;
+    constexpr int VW = S * (int)sizeof(VT) / 4;
+    // chunk q of every lane is one contiguous 1 KB row of the tile: each load
+    // instruction reads 64 consecutive 16-byte words
+    const int64_t base = (int64_t)t * T;
+    const u32x4* cp = reinterpret_cast<const u32x4*>(pcol + base) + lane;
+    const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pval) + base * (int64_t)sizeof(VT)) +
+                      lane;
+#pragma unroll
+    for (int q = 0; q < CW / 4; ++q) {
+        const u32x4 a = ld_vec<NT>(cp + 64 * q);
+        cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
+    }
+#pragma unroll
+    for (int q = 0; q < VW / 4; ++q) {
+        const u32x4 b = ld_vec<NT>(vp + 64 * q);
+        vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
+    }
+    *pb = tseg[t];
+}
+
+// one tile: products from the LDS panel, a wave scan ranks the closed
+// segments, a segmented scan joins each lane's open head to the lanes before
+// it.  The tile's partials (a contiguous range part[pbase, pbase + total))
+// are ranked into the wave's LDS staging area and written out with full-wave
+// stores: one store instruction per 64 partials instead of one per item slot.
+template <typename VT, int S>
+__device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* vw, int32_t pbase, int lane,
+                                           const float* xs, float* stg, float* __restrict__ part)
+{
+    float p[S];
+    uint32_t ends = 0;
+#pragma unroll
+    for (int jj = 0; jj < S; ++jj) {
+        const uint32_t c16 = (jj & 1) ? (cw[jj >> 1] >> 16) : (cw[jj >> 1] & 0xffffu);
+        p[jj] = xs[c16 & 0x7fffu];
+        ends |= (c16 >> 15) << jj;
+    }
+#pragma unroll
+    for (int jj = 0; jj < S; ++jj) p[jj] *= value_at<VT>(vw, jj);
+
+    // rank of this lane's first partial in the tile
+    const int cnt = __popc(ends);
+    int inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    const int r0 = inc - cnt;
+    int r = r0;
+    float acc = 0.0f, head = 0.0f;
+    bool has_end = false;
+#pragma unroll
+    for (int jj = 0; jj < S; ++jj) {
+        acc += p[jj];
+        if ((ends >> jj) & 1u) {
+            if (!has_end) {
+                head = acc;
+                has_end = true;
+            } else {
+                stg[r] = acc;
+            }
+            ++r;
+            acc = 0.0f;
+        }
+    }
+    // segmented scan of (tail, has_end): the open sum each lane's first
+    // closed segment continues from the lanes before it
+    float sv = acc;
+    int fl = has_end ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float vu = __shfl_up(sv, d, 64);
+        const int fu = __shfl_up(fl, d, 64);
+        if (lane >= d) {
+            if (!fl) sv += vu;
+            fl |= fu;
+        }
+    }
+    const float up = __shfl_up(sv, 1, 64);
+    if (has_end) stg[r0] = (lane == 0 ? 0.0f : up) + head;
+    // LDS operations of one wave complete in order: the reads below see the
+    // writes above
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
+    for (int q = lane; q < total; q += 64) part[pbase + q] = stg[q];
+}
+
 // jobs[3j .. 3j+2] = (panel, first tile, end tile); workgroup b runs jobs
 // [wg[b], wg[b+1]) in order.  part[tseg[t] + i] = i-th segment closed in tile t.
 template <typename VT, int S, int W, bool NT>
@@ -67,19 +164,36 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
                                                             const float* __restrict__ x, int64_t n_cols,
                                                             float* __restrict__ part)
 {
-    constexpr int T = 64 * S;
     constexpr int CW = S / 2;                       // column words (two 16-bit items each)
     constexpr int VW = S * (int)sizeof(VT) / 4;     // value words
     static_assert(CW % 4 == 0 && VW % 4 == 0, "lanes load whole 16-byte vectors");
     extern __shared__ float xs[];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // the wave index is uniform across the wave; saying so keeps the tile
+    // loop's control flow and the tile indices scalar
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool x16 = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    float* stg = xs + kPanelCols + wid * (64 * S);   // this wave's partial staging
 
     const int j1 = wg[blockIdx.x + 1];
     for (int j = wg[blockIdx.x]; j < j1; ++j) {
         const int32_t pn = jobs[3 * j], t0 = jobs[3 * j + 1], t1 = jobs[3 * j + 2];
         const int64_t c0 = (int64_t)pn * kPanelCols;
         const int nc = (int)((n_cols - c0) < kPanelCols ? (n_cols - c0) : kPanelCols);
+        int32_t t = t0 + wid;
+        // three register sets in turn: tile t is multiplied while tiles t + W
+        // and t + 2W are in flight.  Loads and stores share one in-order
+        // counter, so waiting for a tile also waits for the partial stores
+        // issued before its load; two tiles ahead, those stores have had a
+        // whole tile to complete.  Every load is unconditional (clamped to the
+        // job's last tile): a load under a branch makes the compiler wait for
+        // all outstanding loads.
+        const int32_t tl = t1 - 1;
+        uint32_t ca[CW], va[VW], cb[CW], vb[VW], cc[CW], vc[VW];
+        int32_t pa = 0, pb = 0, pc = 0;
+        if (t < t1) {
+            load_tile<VT, S, NT>(pcol, pval, tseg, t, lane, ca, va, &pa);
+            load_tile<VT, S, NT>(pcol, pval, tseg, t + W < t1 ? t + W : tl, lane, cb, vb, &pb);
+        }
         __syncthreads();   // the previous job's tiles are done with xs
         if (x16) {
             const float4* xv = reinterpret_cast<const float4*>(x + c0);
@@ -95,98 +209,113 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
         }
         __syncthreads();
 
-        for (int32_t t = t0 + wid; t < t1; t += W) {
-            const int64_t base = (int64_t)t * T + (int64_t)lane * S;
-            uint32_t cw[CW], vw[VW];
-            const u32x4* cp = reinterpret_cast<const u32x4*>(pcol + base);
-            const u32x4* vp = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(pval) +
-                                                             base * (int64_t)sizeof(VT));
-#pragma unroll
-            for (int q = 0; q < CW / 4; ++q) {
-                const u32x4 a = ld_vec<NT>(cp + q);
-                cw[4 * q] = a.x; cw[4 * q + 1] = a.y; cw[4 * q + 2] = a.z; cw[4 * q + 3] = a.w;
-            }
-#pragma unroll
-            for (int q = 0; q < VW / 4; ++q) {
-                const u32x4 b = ld_vec<NT>(vp + q);
-                vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
-            }
-            const int32_t pbase = tseg[t];
-            float p[S];
-            uint32_t ends = 0;
-#pragma unroll
-            for (int jj = 0; jj < S; ++jj) {
-                const uint32_t c16 = (jj & 1) ? (cw[jj >> 1] >> 16) : (cw[jj >> 1] & 0xffffu);
-                p[jj] = xs[c16 & 0x7fffu];
-                ends |= (c16 >> 15) << jj;
-            }
-#pragma unroll
-            for (int jj = 0; jj < S; ++jj) p[jj] *= value_at<VT>(vw, jj);
-
-            // partials closed by earlier lanes of the wave
-            const int cnt = __popc(ends);
-            int inc = cnt;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int o = __shfl_up(inc, d, 64);
-                if (lane >= d) inc += o;
-            }
-            int idx = pbase + inc - cnt;
-            float acc = 0.0f, head = 0.0f;
-            bool has_end = false;
-#pragma unroll
-            for (int jj = 0; jj < S; ++jj) {
-                acc += p[jj];
-                if ((ends >> jj) & 1u) {
-                    if (!has_end) {
-                        head = acc;
-                        has_end = true;
-                    } else {
-                        part[idx] = acc;
-                    }
-                    ++idx;
-                    acc = 0.0f;
-                }
-            }
-            // segmented scan of (tail, has_end): the open sum each lane's first
-            // closed segment continues from the lanes before it
-            float sv = acc;
-            int fl = has_end ? 1 : 0;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const float vu = __shfl_up(sv, d, 64);
-                const int fu = __shfl_up(fl, d, 64);
-                if (lane >= d) {
-                    if (!fl) sv += vu;
-                    fl |= fu;
-                }
-            }
-            const float up = __shfl_up(sv, 1, 64);
-            if (has_end) part[pbase + inc - cnt] = (lane == 0 ? 0.0f : up) + head;
+        // this wave's tiles t + k W, k < n; the main loop runs whole rounds of
+        // three with no exit inside, so each register set stays in place
+        const int n = t < t1 ? (t1 - t + W - 1) / W : 0;
+        auto tile_at = [&](int k) { return t + k * W < t1 ? t + k * W : tl; };
+        int k = 0;
+        for (; k + 3 <= n; k += 3) {
+            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 2), lane, cc, vc, &pc);
+            panel_tile<VT, S>(ca, va, pa, lane, xs, stg, part);
+            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 3), lane, ca, va, &pa);
+            panel_tile<VT, S>(cb, vb, pb, lane, xs, stg, part);
+            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 4), lane, cb, vb, &pb);
+            panel_tile<VT, S>(cc, vc, pc, lane, xs, stg, part);
         }
+        if (k < n) panel_tile<VT, S>(ca, va, pa, lane, xs, stg, part);
+        if (k + 1 < n) panel_tile<VT, S>(cb, vb, pb, lane, xs, stg, part);
     }
 }
 
-// y[r] = sum of row r's partials, in the order listed
-__global__ __launch_bounds__(256) void panel_reduce_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
+// y[r] = sum of row r's partials, in the order listed.  The rows are cut into
+// gridDim.x contiguous chunks of about equal partial counts (wrow); the
+// workgroups are dispatched round-robin over the 8 XCDs and chunk
+// (b % 8) * (gridDim.x / 8) + b / 8 keeps each XCD on one contiguous eighth of
+// the rows, so the partial lines those rows share are fetched into one L2.  A
+// workgroup gathers its chunk's partials into LDS (coalesced index reads,
+// four gathers in flight per thread), then one thread per row adds its run.
+__global__ __launch_bounds__(256) void panel_reduce_kernel(const int32_t* __restrict__ rs,
                                                            const int32_t* __restrict__ gidx,
-                                                           const float* __restrict__ part, float* __restrict__ y)
+                                                           const float* __restrict__ part,
+                                                           const int32_t* __restrict__ wrow, float* __restrict__ y)
 {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t b = rs[r], e = rs[r + 1];
-        float s = 0.0f;
-        for (int32_t j = b; j < e; ++j) s += part[gidx[j]];
-        y[r] = s;
+    extern __shared__ float sp[];
+    const int per_xcd = gridDim.x / 8;
+    const int c = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    const int32_t r0 = wrow[c], r1 = wrow[c + 1];
+    const int32_t j0 = rs[r0], j1 = rs[r1];
+    int32_t j = j0 + (int32_t)threadIdx.x;
+    for (; j + 3 * 256 < j1; j += 4 * 256) {
+        const int32_t g0 = gidx[j], g1 = gidx[j + 256], g2 = gidx[j + 512], g3 = gidx[j + 768];
+        const float a0 = part[g0], a1 = part[g1], a2 = part[g2], a3 = part[g3];
+        sp[j - j0] = a0;
+        sp[j + 256 - j0] = a1;
+        sp[j + 512 - j0] = a2;
+        sp[j + 768 - j0] = a3;
     }
+    for (; j < j1; j += 256) sp[j - j0] = part[gidx[j]];
+    __syncthreads();
+    for (int32_t r = r0 + (int32_t)threadIdx.x; r < r1; r += 256) {
+        const int32_t b = rs[r] - j0, e = rs[r + 1] - j0;
+        float acc = 0.0f;
+        for (int32_t q = b; q < e; ++q) acc += sp[q];
+        y[r] = acc;
+    }
+}
+
+// the same sum straight from global memory, G lanes per row (plans whose
+// largest chunk would not fit the LDS)
+template <int G>
+__global__ __launch_bounds__(256) void panel_reduce_direct_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
+                                                                  const int32_t* __restrict__ gidx,
+                                                                  const float* __restrict__ part, float* __restrict__ y)
+{
+    const int q = threadIdx.x & (G - 1);
+    const int64_t groups = (int64_t)gridDim.x * (blockDim.x / G);
+    // the G lanes of a group share r, so they leave the loop together
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < n_rows; r += groups) {
+        const int32_t b = rs[r], e = rs[r + 1];
+        float acc = 0.0f;
+        for (int32_t j = b + q; j < e; j += G) acc += part[gidx[j]];
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+        if (q == 0) y[r] = acc;
+    }
+}
+
+// chunks of about equal cost = partials + rows (empty rows still cost a
+// store): wrow[b] = first row r with rs[r] + r >= b * (total + n_rows) / n_chunks
+__global__ void panel_chunks_kernel(int64_t n_rows, const int32_t* __restrict__ rs, int n_chunks,
+                                    int32_t* __restrict__ wrow)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > n_chunks) return;
+    const int64_t total = (int64_t)rs[n_rows] + n_rows;
+    const int64_t target = (int64_t)b * total / n_chunks;
+    int64_t lo = 0, hi = n_rows;   // rs[r] + r is strictly increasing in r
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)rs[mid] + mid >= target) hi = mid;
+        else lo = mid + 1;
+    }
+    wrow[b] = b == n_chunks ? (int32_t)n_rows : (int32_t)lo;
+}
+
+// *maxc = the largest chunk's partial count
+__global__ void panel_chunk_max_kernel(int n_chunks, const int32_t* __restrict__ rs, const int32_t* __restrict__ wrow,
+                                       int32_t* __restrict__ maxc)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n_chunks) atomicMax(maxc, rs[wrow[b + 1]] - rs[wrow[b]]);
 }
 
 template <typename VT, int S, int W, bool NT>
 hipError_t launch_panel(const uint16_t* pcol, const void* pval, const int32_t* tseg, const int32_t* jobs,
                         const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
-                        const int32_t* rs, const int32_t* gidx, void* part, void* y, hipStream_t s)
+                        const int32_t* rs, const int32_t* gidx, const int32_t* wrow, int n_chunks, int chunk_max,
+                        void* part, void* y, hipStream_t s)
 {
-    const size_t lds = sizeof(float) * (size_t)kPanelCols;
+    const size_t lds = panel_lds_bytes(S, W);
     auto kern = panel_spmv_kernel<VT, S, W, NT>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -197,11 +326,21 @@ hipError_t launch_panel(const uint16_t* pcol, const void* pval, const int32_t* t
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    int64_t g = (n_rows + 255) / 256;
-    if (g > 65536) g = 65536;
+    if (n_chunks > 0 && (size_t)chunk_max * sizeof(float) <= panel_reduce_lds_cap()) {
+        const size_t rl = sizeof(float) * (size_t)(chunk_max > 0 ? chunk_max : 1);
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(panel_reduce_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)rl);
+        if (e != hipSuccess) return e;
+        panel_reduce_kernel<<<(unsigned)n_chunks, 256, rl, s>>>(rs, gidx, static_cast<const float*>(part), wrow,
+                                                               static_cast<float*>(y));
+        return hipGetLastError();
+    }
+    constexpr int G = 8;
+    int64_t g = (n_rows * G + 255) / 256;
+    if (g > 16384) g = 16384;
     if (g < 1) g = 1;
-    panel_reduce_kernel<<<(unsigned)g, 256, 0, s>>>(n_rows, rs, gidx, static_cast<const float*>(part),
-                                                    static_cast<float*>(y));
+    panel_reduce_direct_kernel<G><<<(unsigned)g, 256, 0, s>>>(n_rows, rs, gidx, static_cast<const float*>(part),
+                                                              static_cast<float*>(y));
     return hipGetLastError();
 }
 
@@ -302,9 +441,17 @@ __global__ void pn_scatter_kernel(int64_t nnz, int tile, const int32_t* __restri
         const int32_t s = sid[k] + F[k] - 1;
         const int64_t q = (int64_t)pos[s] + (k - k0[s]);
         const bool end = (k == (int64_t)k0[s + 1] - 1) || ((q % tile) == tile - 1);
-        pcol[q] = (uint16_t)((ci[k] & (kPanelCols - 1)) | (end ? 0x8000 : 0));
-        pval[q] = val[k];
         flag[q] = end ? 1 : 0;
+        // logical item i of a tile belongs to lane i / S at slot jj = i % S;
+        // lane l's 16-byte chunk c is stored at chunk row c: c * 64 + l
+        const int S = tile / 64;
+        const int64_t tb = q - q % tile;
+        const int i = (int)(q - tb), l = i / S, jj = i % S;
+        constexpr int EC = 8;                       // 16-bit columns per chunk
+        constexpr int EV = 16 / (int)sizeof(VT);    // values per chunk
+        pcol[tb + (int64_t)((jj / EC) * 64 + l) * EC + jj % EC] =
+            (uint16_t)((ci[k] & (kPanelCols - 1)) | (end ? 0x8000 : 0));
+        pval[tb + (int64_t)((jj / EV) * 64 + l) * EV + jj % EV] = val[k];
     }
 }
 
@@ -443,11 +590,25 @@ hipError_t panel_gidx(int64_t n_seg, int tile, const int32_t* pos, const int32_t
     return hipGetLastError();
 }
 
-size_t panel_lds_bytes() { return sizeof(float) * (size_t)kPanelCols; }
+size_t panel_reduce_lds_cap() { return 64 * 1024; }
+
+hipError_t panel_chunks(int64_t n_rows, const int32_t* rs, int n_chunks, int32_t* wrow, int32_t* maxc, hipStream_t s)
+{
+    if (n_chunks <= 0) return hipSuccess;
+    const unsigned g = (unsigned)((n_chunks + 1 + 255) / 256);
+    panel_chunks_kernel<<<g, 256, 0, s>>>(n_rows, rs, n_chunks, wrow);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    panel_chunk_max_kernel<<<(unsigned)((n_chunks + 255) / 256), 256, 0, s>>>(n_chunks, rs, wrow, maxc);
+    return hipGetLastError();
+}
+
+size_t panel_lds_bytes(int S, int W) { return sizeof(float) * ((size_t)kPanelCols + (size_t)W * 64 * S); }
 
 hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const void* pval, const int32_t* tseg,
                       const int32_t* jobs, const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
-                      const int32_t* rs, const int32_t* gidx, void* part, void* y, hipStream_t s)
+                      const int32_t* rs, const int32_t* gidx, const int32_t* wrow, int n_chunks, int chunk_max,
+                      void* part, void* y, hipStream_t s)
 {
     if (n_rows <= 0) return hipSuccess;
     if (dt == DT_F64) return hipErrorInvalidValue;
@@ -463,15 +624,14 @@ hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const 
                     return hipErrorInvalidValue;
                 } else {
                     return nt ? launch_panel<VT, SS, WW, true>(pcol, pval, tseg, jobs, wg, n_wg, x, n_cols, n_rows, rs,
-                                                               gidx, part, y, s)
+                                                               gidx, wrow, n_chunks, chunk_max, part, y, s)
                               : launch_panel<VT, SS, WW, false>(pcol, pval, tseg, jobs, wg, n_wg, x, n_cols, n_rows,
-                                                                rs, gidx, part, y, s);
+                                                                rs, gidx, wrow, n_chunks, chunk_max, part, y, s);
                 }
             };
             using I8 = std::integral_constant<int, 8>;
             using I16 = std::integral_constant<int, 16>;
             if (S == 8 && W == 16) return go(I8{}, I16{});
-            if (S == 16 && W == 16) return go(I16{}, I16{});
             if (S == 8 && W == 8) return go(I8{}, I8{});
             if (S == 16 && W == 8) return go(I16{}, I8{});
             return hipErrorInvalidValue;

@@ -184,7 +184,7 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     }
     const int64_t n_items = tiles * tile;
     const double tile_bytes = (double)tile * (2.0 + (double)dtype_size(m->dtype));
-    const double load = (double)k::panel_lds_bytes() / tile_bytes;
+    const double load = (double)(sizeof(float) * k::kPanelCols) / tile_bytes;
     const JobTable jt = plan_jobs(tile0, ntile, std::max(1, m->ctx->cu_count), load);
     DevBuf d_padd, pos;
     MR_TRY(d_padd.alloc(sizeof(int32_t) * (size_t)P));
@@ -247,6 +247,19 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
                                    s));
     MR_HIP_TRY(k::panel_gidx(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), goff.as<int32_t>(), pidx.as<int32_t>(),
                              p.pgidx.as<int32_t>(), s));
+    // row chunks of about 4096 partials + rows for the partial sums, a
+    // multiple of 8 (one contiguous eighth of the rows per XCD)
+    int64_t chunks = (n_parts + n + 4095) / 4096;
+    chunks = std::min<int64_t>(std::max<int64_t>(chunks, 8), 1 << 20);
+    chunks = (chunks + 7) / 8 * 8;
+    if (n <= 0) chunks = 0;
+    int32_t chunk_max = 0;
+    MR_TRY(p.pwrow.alloc(sizeof(int32_t) * (size_t)(chunks + 1)));
+    if (chunks > 0) {
+        MR_HIP_TRY(hipMemsetAsync(total.p, 0, sizeof(int32_t), s));
+        MR_HIP_TRY(k::panel_chunks(n, p.prs.as<int32_t>(), (int)chunks, p.pwrow.as<int32_t>(), total.as<int32_t>(), s));
+        MR_HIP_TRY(hipMemcpyAsync(&chunk_max, total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    }
     MR_TRY(p.pjobs.alloc(sizeof(int32_t) * std::max<size_t>(jt.jobs.size(), 3)));
     MR_TRY(p.pwg.alloc(sizeof(int32_t) * jt.wg.size()));
     if (!jt.jobs.empty())
@@ -258,6 +271,8 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     p.psegs = n_seg;
     p.pn_wg = (int)jt.wg.size() - 1;
     p.pn_jobs = (int)(jt.jobs.size() / 3);
+    p.pn_chunks = (int)chunks;
+    p.pchunk_max = chunk_max;
     p.pn_built = true;
     p.pn_build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
