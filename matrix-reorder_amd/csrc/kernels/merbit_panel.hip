@@ -91,7 +91,7 @@ __device__ __forceinline__ void load_tile(const uint16_t* __restrict__ pcol, con
 // it.  The tile's partials (a contiguous range part[pbase, pbase + total))
 // are ranked into the wave's LDS staging area and written out with full-wave
 // stores: one store instruction per 64 partials instead of one per item slot.
-template <typename VT, int S>
+template <typename VT, int S, bool NT>
 __device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* vw, int32_t pbase, int lane,
                                            const float* xs, float* stg, float* __restrict__ part)
 {
@@ -150,6 +150,7 @@ __device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* v
     // LDS operations of one wave complete in order: the reads below see the
     // writes above
     const int total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
+    // plain stores: the partial sums stay cached for the kernel that adds them
     for (int q = lane; q < total; q += 64) part[pbase + q] = stg[q];
 }
 
@@ -216,14 +217,14 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
         int k = 0;
         for (; k + 3 <= n; k += 3) {
             load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 2), lane, cc, vc, &pc);
-            panel_tile<VT, S>(ca, va, pa, lane, xs, stg, part);
+            panel_tile<VT, S, NT>(ca, va, pa, lane, xs, stg, part);
             load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 3), lane, ca, va, &pa);
-            panel_tile<VT, S>(cb, vb, pb, lane, xs, stg, part);
+            panel_tile<VT, S, NT>(cb, vb, pb, lane, xs, stg, part);
             load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 4), lane, cb, vb, &pb);
-            panel_tile<VT, S>(cc, vc, pc, lane, xs, stg, part);
+            panel_tile<VT, S, NT>(cc, vc, pc, lane, xs, stg, part);
         }
-        if (k < n) panel_tile<VT, S>(ca, va, pa, lane, xs, stg, part);
-        if (k + 1 < n) panel_tile<VT, S>(cb, vb, pb, lane, xs, stg, part);
+        if (k < n) panel_tile<VT, S, NT>(ca, va, pa, lane, xs, stg, part);
+        if (k + 1 < n) panel_tile<VT, S, NT>(cb, vb, pb, lane, xs, stg, part);
     }
 }
 

@@ -247,9 +247,9 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
                                    s));
     MR_HIP_TRY(k::panel_gidx(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), goff.as<int32_t>(), pidx.as<int32_t>(),
                              p.pgidx.as<int32_t>(), s));
-    // row chunks of about 4096 partials + rows for the partial sums, a
+    // row chunks of about 2048 partials + rows for the partial sums, a
     // multiple of 8 (one contiguous eighth of the rows per XCD)
-    int64_t chunks = (n_parts + n + 4095) / 4096;
+    int64_t chunks = (n_parts + n + 2047) / 2048;
     chunks = std::min<int64_t>(std::max<int64_t>(chunks, 8), 1 << 20);
     chunks = (chunks + 7) / 8 * 8;
     if (n <= 0) chunks = 0;
