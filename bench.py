@@ -48,11 +48,23 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "matrix-reorder_amd"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = ROOT / "profiles" / "merbit_lp_s22_fp32_pmc.md"
+# rocprofv3 counter summaries of the headline kernels, by MERBIT variant
+TRAFFIC_PROFILES = {"panel": ROOT / "profiles" / "merbit_panel_s22_fp32_pmc.md",
+                    "lane_padded": ROOT / "profiles" / "merbit_lp_s22_fp32_pmc.md"}
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def host_cores():
@@ -85,13 +97,20 @@ def csr_bytes(n_rows, n_cols, nnz, val_bytes, vec_bytes=4, k=1):
     return nnz * (val_bytes + 4) + (n_rows + 1) * 4 + k * (n_cols + n_rows) * vec_bytes
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the MERBIT kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE scaled by the copy kernel)."""
-    if not TRAFFIC_PROFILE.exists():
+def traffic_from_profile(policy):
+    """HBM bytes per SpMV of the headline kernel(s) from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE scaled by the copy kernel).
+    The summary names the MERBIT policy it was measured with; a run that picked
+    another policy gets None rather than a figure of other code."""
+    path = TRAFFIC_PROFILES["panel" if policy & 8 else "lane_padded"]
+    if not path.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_PROFILE.read_text())
-    return float(m.group(1)) if m else None
+    text = path.read_text()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    pm = re.search(r"^policy:\s*(\d+)", text, re.M)
+    if not m or (pm and int(pm.group(1)) != policy):
+        return None
+    return float(m.group(1))
 
 
 def err_stats(y, yref):
@@ -154,6 +173,8 @@ def section_main(mr, ctx, a, extra):
     if a.policy >= 0:
         mat.merbit_options(a.policy)
     cfg = {"policy": mat.merbit_policy(), **mat.merbit_info(), **mat.merbit_xc_info()}
+    if cfg["policy"] & 8:
+        cfg.update({f"panel_{k}": v for k, v in mat.merbit_panel_info().items()})
     log(f"MERBIT FP32 after pGOrder: {a.warmup} warmup + {a.steps} timed steps")
     ms, wall = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, a.steps, a.warmup)
     gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
@@ -435,7 +456,9 @@ def section_cpu(mr, ctx, a, data):
     gflops = 2.0 * len(ci) / (per * 1e-3) / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
             "sample": f"oracle threaded CSR SpMV (FP32, nnz-balanced row ranges) on the unreordered R-MAT s22 "
-                      f"matrix, {iters} iterations, {per:.2f} ms/iter"}
+                      f"matrix, {iters} iterations, {per:.2f} ms/iter",
+            "cpu_model": cpu_model(), "host_threads": host_cores(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def section_cpu_reorders(mr, ctx, extra):
@@ -538,7 +561,7 @@ def main():
                    "n_rows": n, "nnz": nnz, "kernel": "merbit", **cfg},
         "roofline": {"bound": "hbm", "achieved": round(main_res["achieved"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_res["achieved"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile()},
+                     "traffic": traffic_from_profile(cfg["policy"])},
         "cpu_baseline": cpu,
         "extra": extra,
     }
