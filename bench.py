@@ -274,28 +274,9 @@ def section_precision(mr, ctx, a, extra):
     extra["precision_sweep"] = out
 
 
-def batched_set(mr, seed=3, count=64):
-    """64 matrices: rows log-uniform in [2^10, 2^18], nnz/row log-uniform in
-    [4, 64]; even entries R-MAT (scale = round(log2 rows), edge factor = the
-    drawn nnz/row), odd entries random banded (bandwidth uniform in [8, 512])."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        rows = int(round(2.0 ** rng.uniform(10, 18)))
-        d = float(2.0 ** rng.uniform(2, 6))
-        if i % 2 == 0:
-            scale = int(round(math.log2(rows)))
-            rp, ci = mr.gen_rmat(scale, max(1, int(round(d))), (0.57, 0.19, 0.19), seed=1000 + i)
-        else:
-            bw = int(rng.integers(8, 513))
-            rp, ci = mr.gen_banded(rows, bw, max(1, min(int(round(d)), 2 * bw + 1)), seed=2000 + i)
-        out.append((rp, ci))
-    return out
-
-
 def section_batched(mr, ctx, a, extra):
     t = time.perf_counter()
-    shapes = batched_set(mr)
+    shapes = mr.gen_batched_set(3, 64)
     mats = []
     t_re = 0.0
     for i, (rp, ci) in enumerate(shapes):

@@ -425,15 +425,20 @@ void retire_slot(mr_server s, Slot& sl, std::unique_lock<std::mutex>& lk)
         std::vector<std::pair<int64_t, std::vector<float>>> outs;
         lk.unlock();
         for (const Job& J : sl.jobs) {
+            // the job's whole n_rows x K block in one contiguous copy, split
+            // into its requests' columns on the host (a strided 2-D copy of
+            // millions of short rows per request is far slower)
+            const size_t nr = (size_t)J.m->n_rows, K = (size_t)J.K;
+            std::vector<float> blk(nr * K);
+            if (!blk.empty())
+                hipMemcpy(blk.data(), static_cast<char*>(sl.arena.p) + J.y_off, sizeof(float) * nr * K,
+                          hipMemcpyDeviceToHost);
             for (size_t q = 0; q < J.reqs.size(); ++q) {
                 const int64_t r = J.reqs[q];
-                const int k = J.ks[q];
-                std::vector<float> y((size_t)J.m->n_rows * (size_t)k);
-                if (!y.empty())
-                    hipMemcpy2D(y.data(), sizeof(float) * (size_t)k,
-                                static_cast<char*>(sl.arena.p) + J.y_off + sizeof(float) * (size_t)J.off[q],
-                                sizeof(float) * (size_t)J.K, sizeof(float) * (size_t)k, (size_t)J.m->n_rows,
-                                hipMemcpyDeviceToHost);
+                const size_t k = (size_t)J.ks[q], off = (size_t)J.off[q];
+                std::vector<float> y(nr * k);
+                for (size_t i = 0; i < nr; ++i)
+                    for (size_t c = 0; c < k; ++c) y[i * k + c] = blk[i * K + off + c];
                 outs.emplace_back(r, std::move(y));
             }
         }

@@ -292,6 +292,27 @@ def gen_banded(n: int, bandwidth: int, nnz_per_row: int, seed: int):
     return rp, ci[: nnz.value]
 
 
+def gen_batched_set(seed: int = 3, count: int = 64):
+    """The batched-launch workload: `count` matrices, rows log-uniform in
+    [2^10, 2^18], nnz/row log-uniform in [4, 64]; even entries R-MAT (scale =
+    round(log2 rows), edge factor = the drawn nnz/row), odd entries random
+    banded (bandwidth uniform in [8, 512]).  Returns [(row_ptr, col_idx)]."""
+    import math
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        rows = int(round(2.0 ** rng.uniform(10, 18)))
+        d = float(2.0 ** rng.uniform(2, 6))
+        if i % 2 == 0:
+            scale = int(round(math.log2(rows)))
+            rp, ci = gen_rmat(scale, max(1, int(round(d))), (0.57, 0.19, 0.19), seed=1000 + i)
+        else:
+            bw = int(rng.integers(8, 513))
+            rp, ci = gen_banded(rows, bw, max(1, min(int(round(d)), 2 * bw + 1)), seed=2000 + i)
+        out.append((rp, ci))
+    return out
+
+
 def csr_hash(rp: np.ndarray, ci: np.ndarray, n_cols: int | None = None) -> int:
     """Fingerprint of a host CSR pattern (the plan-cache key); equals
     ``Mat.hash()`` of the same matrix."""

@@ -4,6 +4,7 @@
  *   oracle_get_width         src/class.cu:733-750  (float arithmetic kept)
  *   oracle_pgorder_schedule  src/class.cu:2995-3016 (Pstart/Pleng/Ostart/Oleng)
  *   oracle_pgorder_gpu_loop  src/class.cu:3041-3059 with gorder_gpu_cpu_kernel1
+ *   (_ex: hub cap and a batch cap as parameters)
  *                            (:2357-2369), gorder6_kernel2 (:2897-2913),
  *                            gorder6_kernel3 (:2917-2953) and the stable
  *                            descending CUB radix sort (:3046,3052), restated
@@ -88,8 +89,14 @@ int32_t oracle_pgorder_schedule(int32_t n, int32_t W, int32_t cap, int32_t* Ps,
 }
 
 /* ------------------------------------------------------------------------ */
+/* The reference orders all n vertices each batch with a stable descending
+ * radix sort of the scores over identity keys (class.cu:3046,3052), i.e. by
+ * (score descending, vertex id ascending), and takes the first |P|.  That
+ * order is total, so the first |P| are the |P| largest of the composite key
+ * (score << 32) | ~id, found here by a selection and a sort of the selected. */
 typedef struct { uint32_t score; uint32_t id; } skey;
 
+/* the stable descending order written as a comparison (gorder5 sorts with it) */
 static int cmp_desc_stable(const void* a, const void* b)
 {
     const skey* x = (const skey*)a;
@@ -98,46 +105,109 @@ static int cmp_desc_stable(const void* a, const void* b)
     return (x->id < y->id) ? -1 : (x->id > y->id);
 }
 
-int oracle_pgorder_gpu_loop(int32_t n, const int32_t* rp, const int32_t* ci,
-                            const int32_t* tp, const int32_t* ti, int32_t W,
-                            uint32_t* placement)
+static void swap64(uint64_t* a, uint64_t* b) { uint64_t t = *a; *a = *b; *b = t; }
+
+/* move the k largest of a[0..m) to a[0..k) (any order) */
+static void select_top(uint64_t* a, int64_t m, int64_t k)
+{
+    int64_t lo = 0, hi = m - 1;
+    while (lo < hi) {
+        /* median of three as pivot; partition descending */
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (a[mid] > a[lo]) swap64(&a[mid], &a[lo]);
+        if (a[hi] > a[lo]) swap64(&a[hi], &a[lo]);
+        if (a[hi] > a[mid]) swap64(&a[hi], &a[mid]);
+        const uint64_t piv = a[mid];
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (a[i] > piv) ++i;
+            while (a[j] < piv) --j;
+            if (i <= j) { swap64(&a[i], &a[j]); ++i; --j; }
+        }
+        if (k - 1 <= j) hi = j;
+        else if (k - 1 >= i) lo = i;
+        else break;
+    }
+}
+
+static int cmp_u64_desc(const void* a, const void* b)
+{
+    const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x > y ? -1 : (x < y);
+}
+
+int oracle_pgorder_gpu_loop_ex(int32_t n, const int32_t* rp, const int32_t* ci,
+                               const int32_t* tp, const int32_t* ti, int32_t W, int32_t hub_cap,
+                               int32_t max_batches, uint32_t* placement)
 {
     const int32_t cap = (n + W - 1) / W + 64;
     int32_t *Ps = malloc(sizeof(int32_t) * cap), *Pl = malloc(sizeof(int32_t) * cap);
     int32_t *Os = malloc(sizeof(int32_t) * cap), *Ol = malloc(sizeof(int32_t) * cap);
-    const int32_t B = oracle_pgorder_schedule(n, W, cap, Ps, Pl, Os, Ol);
-    uint32_t* score = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
-    skey* keys = (skey*)malloc(sizeof(skey) * (size_t)n);
+    int32_t B = oracle_pgorder_schedule(n, W, cap, Ps, Pl, Os, Ol);
+    if (max_batches >= 0 && max_batches < B) B = max_batches;
+    uint32_t* score = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+    int32_t* hcnt = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
+    int32_t* touched = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
     /* kernel1: score = rowlen | 1<<31 (offset2 = SpMV row pointer) */
     for (int32_t v = 0; v < n; ++v) score[v] = (uint32_t)(rp[v + 1] - rp[v]) | 0x80000000u;
     for (int32_t b = 0; b < B; ++b) {
-        /* stable descending radix sort of (score, id) with ids in identity order */
-        for (int32_t v = 0; v < n; ++v) { keys[v].score = score[v]; keys[v].id = (uint32_t)v; }
-        qsort(keys, (size_t)n, sizeof(skey), cmp_desc_stable);
+        /* the first Pleng vertices by (score desc, id asc) */
+        const int64_t take = Pl[b] < n ? Pl[b] : n;
+        int64_t m = 0;
+        for (int32_t v = 0; v < n; ++v)
+            if (score[v]) keys[m++] = ((uint64_t)score[v] << 32) | (uint32_t)~(uint32_t)v;
+        if (m < take)   /* placed vertices (score 0) follow in id order */
+            for (int32_t v = 0; v < n && m < take; ++v)
+                if (!score[v]) keys[m++] = (uint32_t)~(uint32_t)v;
+        if (take > 0) {
+            select_top(keys, m, take);
+            qsort(keys, (size_t)take, sizeof(uint64_t), cmp_u64_desc);
+        }
         /* kernel2: keep only the status bit; place the first Pleng */
         for (int32_t v = 0; v < n; ++v) score[v] &= 0x80000000u;
-        for (int32_t i = 0; i < Pl[b] && i < n; ++i) {
-            score[keys[i].id] = 0u;
-            placement[Ps[b] + i] = keys[i].id;
+        for (int64_t i = 0; i < take; ++i) {
+            const uint32_t v = ~(uint32_t)(keys[i] & 0xffffffffu);
+            score[v] = 0u;
+            placement[Ps[b] + i] = v;
         }
-        /* kernel3: window 2-hop sibling counts */
+        /* kernel3: window 2-hop sibling counts, hubs (>= hub_cap) skipped.
+         * The reference adds 1 to score[z] once per (x, y, z) path; the same
+         * sum is taken here as h[y] per (y, z), h[y] = number of window x
+         * reaching y (integer addition, so the order does not matter).  It
+         * keeps hub-heavy first windows from walking a column list once per
+         * window vertex. */
         int32_t olen = Ol[b];
         if (Os[b] + olen > n) olen = n - Os[b];
+        int64_t nt = 0;
         for (int32_t w = 0; w < olen; ++w) {
             const int32_t x = (int32_t)placement[Os[b] + w];
-            if (rp[x + 1] - rp[x] >= 10000) continue;
+            if (rp[x + 1] - rp[x] >= hub_cap) continue;
             for (int32_t a = rp[x]; a < rp[x + 1]; ++a) {
                 const int32_t y = ci[a];
-                if (tp[y + 1] - tp[y] >= 10000) continue;
-                for (int32_t c = tp[y]; c < tp[y + 1]; ++c) {
-                    const int32_t z = ti[c];
-                    if (score[z] >> 31) score[z] += 1u;
-                }
+                if (tp[y + 1] - tp[y] >= hub_cap) continue;
+                if (hcnt[y]++ == 0) touched[nt++] = y;
             }
         }
+        for (int64_t q = 0; q < nt; ++q) {
+            const int32_t y = touched[q];
+            for (int32_t c = tp[y]; c < tp[y + 1]; ++c) {
+                const int32_t z = ti[c];
+                if (score[z] >> 31) score[z] += (uint32_t)hcnt[y];
+            }
+            hcnt[y] = 0;
+        }
     }
-    free(Ps); free(Pl); free(Os); free(Ol); free(score); free(keys);
+    free(Ps); free(Pl); free(Os); free(Ol); free(score); free(keys); free(hcnt); free(touched);
     return B;
+}
+
+int oracle_pgorder_gpu_loop(int32_t n, const int32_t* rp, const int32_t* ci,
+                            const int32_t* tp, const int32_t* ti, int32_t W,
+                            uint32_t* placement)
+{
+    /* the 10,000 hub caps of gorder6_kernel3 (class.cu:2930,2938) */
+    return oracle_pgorder_gpu_loop_ex(n, rp, ci, tp, ti, W, 10000, -1, placement);
 }
 
 /* ------------------------------------------------------------------------ */

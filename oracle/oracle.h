@@ -78,6 +78,12 @@ int32_t oracle_pgorder_schedule(int32_t n, int32_t width, int32_t cap, int32_t* 
 int  oracle_pgorder_gpu_loop(int32_t n, const int32_t* rp, const int32_t* ci,
                              const int32_t* tp, const int32_t* ti, int32_t width,
                              uint32_t* placement);
+/* the same loop with the hub cap of kernel3 (10,000 in the reference) as a
+ * parameter, stopped after max_batches batches (-1 = all); returns the number
+ * of batches run */
+int  oracle_pgorder_gpu_loop_ex(int32_t n, const int32_t* rp, const int32_t* ci,
+                                const int32_t* tp, const int32_t* ti, int32_t width, int32_t hub_cap,
+                                int32_t max_batches, uint32_t* placement);
 /* order_on_cpu (class.cu:2220-2314) on one slice; batch_of[v] plays Node::batch_id. */
 void oracle_order_on_cpu(int32_t n, const int32_t* rp, const int32_t* ci,
                          const int32_t* tp, const int32_t* ti, int32_t* batch_of,

@@ -35,6 +35,8 @@ _SIG = {
     "oracle_order_dbg": (None, [C.c_int32, _I32P, _I32P, _I32P]),
     "oracle_pgorder_schedule": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "oracle_pgorder_gpu_loop": (C.c_int, [C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32, _U32P]),
+    "oracle_pgorder_gpu_loop_ex": (C.c_int, [C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32, C.c_int32, C.c_int32,
+                                             _U32P]),
     "oracle_order_on_cpu": (None, [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, _U32P, C.c_int32, C.c_int32,
                                    C.c_int32, C.c_int32]),
     "oracle_pgorder": (C.c_int, [C.c_int32, _I32P, _I32P, C.c_int32, C.c_int32, _U32P, _U32P, _I32P]),
@@ -168,12 +170,18 @@ def pgorder_schedule(n, width):
     return [x[:b] for x in a]
 
 
-def pgorder_gpu_loop(n, rp, ci, width):
+def pgorder_gpu_loop(n, rp, ci, width, hub_cap=10000, max_batches=-1, tp=None, ti=None):
+    """Placement (position -> vertex) after the GPU loop; with max_batches >= 0
+    only the positions of the first max_batches batches are filled (the rest
+    stay 0xFFFFFFFF)."""
     rp, ci = _i32(rp), _i32(ci)
-    tp, ti = transpose(n, n, rp, ci)
-    pl = np.empty(n, np.uint32)
-    lib().oracle_pgorder_gpu_loop(n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(tp, C.c_int32),
-                                  _p(ti if len(ti) else np.zeros(1, np.int32), C.c_int32), width, _p(pl, C.c_uint32))
+    if tp is None:
+        tp, ti = transpose(n, n, rp, ci)
+    pl = np.full(n, 0xFFFFFFFF, np.uint32)
+    lib().oracle_pgorder_gpu_loop_ex(n, _p(rp, C.c_int32), _p(ci if len(ci) else np.zeros(1, np.int32), C.c_int32),
+                                     _p(_i32(tp), C.c_int32),
+                                     _p(_i32(ti) if len(ti) else np.zeros(1, np.int32), C.c_int32), width, hub_cap,
+                                     max_batches, _p(pl, C.c_uint32))
     return pl
 
 
@@ -187,10 +195,13 @@ def pgorder(n, rp, ci, width=1024, window=5):
     return g, f, perm
 
 
-def order_on_cpu(n, rp, ci, slice_, batch_id, window, degree):
+def order_on_cpu(n, rp, ci, slice_, batch_id, window, degree, tp=None, ti=None, batch_of=None):
     rp, ci = _i32(rp), _i32(ci)
-    tp, ti = transpose(n, n, rp, ci)
-    batch_of = np.full(n, -1, np.int32)
+    if tp is None:
+        tp, ti = transpose(n, n, rp, ci)
+    tp, ti = _i32(tp), _i32(ti)
+    if batch_of is None:
+        batch_of = np.full(n, -1, np.int32)
     s = np.ascontiguousarray(slice_, np.uint32).copy()
     lib().oracle_order_on_cpu(n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(tp, C.c_int32), _p(ti, C.c_int32),
                               _p(batch_of, C.c_int32), _p(s, C.c_uint32), batch_id, len(s), window, degree)

@@ -246,3 +246,37 @@ def test_lane_padded_route_matches_library(ctx, mr):
         assert mr.serve_hash(yd.to_host()) == h, (k, sd)
     srv.close()
     big.close()
+
+
+def test_concurrent_adds_of_two_dtypes(ctx, mr):
+    """Two threads add matrices of different storage dtypes to one server at
+    the same time: exactly one add succeeds and the other is refused
+    (MR_ERR_UNSUPPORTED), so the fused kernels never read a matrix as the
+    wrong dtype."""
+    import threading
+    rp, ci = rmat_csr(12, 16, seed=61)
+    n = len(rp) - 1
+    for rep in range(8):
+        a = mr.Mat.from_csr(ctx, rp, ci, None, n_cols=n, store=mr.F32)
+        b = mr.Mat.from_csr(ctx, rp, ci, None, n_cols=n, store=mr.BF16)
+        srv = mr.Server(ctx)
+        codes = {}
+        gate = threading.Barrier(2)
+
+        def add(name, m):
+            gate.wait()
+            try:
+                srv.add(m)
+                codes[name] = 0
+            except mr.MRError as e:
+                codes[name] = e.code
+
+        ts = [threading.Thread(target=add, args=("a", a)), threading.Thread(target=add, args=("b", b))]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert sorted(codes.values()) == [0, 5], (rep, codes)
+        srv.close()
+        a.close()
+        b.close()
