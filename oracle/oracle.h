@@ -114,6 +114,8 @@ int  oracle_rd_exact(const int32_t* stream, int32_t n, int64_t nnz, int32_t sect
 int  oracle_rd_bruteforce(const int32_t* stream, int64_t nnz, int32_t sector,
                           int64_t* rd_out);   /* -1 = cold */
 uint32_t oracle_hll_hash(uint32_t x);
+/* avalanche bias of oracle_hll_hash over 2^log2_n sampled inputs (checker) */
+double oracle_hll_hash_bias(int32_t log2_n, uint64_t seed);
 int32_t  oracle_hll_rank(uint32_t h, int32_t p);
 float    oracle_hll_estimate(const int32_t* regs, int32_t m);
 /* one sample of sampled_hll_rd_block_kernel (class.cu:1014-1131) */

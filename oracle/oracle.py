@@ -35,6 +35,7 @@ _SIG = {
     "oracle_order_dbg": (None, [C.c_int32, _I32P, _I32P, _I32P]),
     "oracle_pgorder_schedule": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "oracle_pgorder_gpu_loop": (C.c_int, [C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32, _U32P]),
+    "oracle_hll_hash_bias": (C.c_double, [C.c_int32, C.c_uint64]),
     "oracle_pgorder_gpu_loop_ex": (C.c_int, [C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32, C.c_int32, C.c_int32,
                                              _U32P]),
     "oracle_order_on_cpu": (None, [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, _U32P, C.c_int32, C.c_int32,
@@ -257,6 +258,10 @@ def rd_bruteforce(stream, sector):
 
 def hll_hash(x):
     return lib().oracle_hll_hash(x)
+
+
+def hll_hash_bias(log2_n=23, seed=1):
+    return lib().oracle_hll_hash_bias(log2_n, seed)
 
 
 def hll_rank(h, p):

@@ -108,6 +108,41 @@ uint32_t oracle_hll_hash(uint32_t x)
     return x;
 }
 
+/* Avalanche bias of oracle_hll_hash, as the hash's published source
+ * (C. Wellons, hash-prospector, where lowbias32 is listed with bias
+ * 0.17353355999581582 over all 2^32 inputs) measures it: for 2^log2_n inputs x
+ * and every input bit j, count how often output bit k flips when bit j of x
+ * flips; bias = 1000 * sqrt(mean over (j, k) of ((count - n/2) / (n/2))^2).
+ * The sampling noise of that mean, 1/n for independent fair bits, is
+ * subtracted.  A checker of the restated constants, not part of any path. */
+double oracle_hll_hash_bias(int32_t log2_n, uint64_t seed)
+{
+    const int64_t n = (int64_t)1 << log2_n;
+    static int64_t bins[32][32];
+    memset(bins, 0, sizeof(bins));
+    uint64_t s = seed;
+    for (int64_t i = 0; i < n; ++i) {
+        s += 0x9E3779B97F4A7C15ull;            /* SplitMix64 inputs */
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        const uint32_t x = (uint32_t)(z ^ (z >> 31));
+        const uint32_t h0 = oracle_hll_hash(x);
+        for (int j = 0; j < 32; ++j) {
+            const uint32_t flip = h0 ^ oracle_hll_hash(x ^ (1u << j));
+            for (int k = 0; k < 32; ++k) bins[j][k] += (flip >> k) & 1u;
+        }
+    }
+    double mean = 0.0;
+    for (int j = 0; j < 32; ++j)
+        for (int k = 0; k < 32; ++k) {
+            const double d = ((double)bins[j][k] - (double)n / 2.0) / ((double)n / 2.0);
+            mean += d * d / 1024.0;
+        }
+    mean -= 1.0 / (double)n;
+    return 1000.0 * sqrt(mean > 0.0 ? mean : 0.0);
+}
+
 int32_t oracle_hll_rank(uint32_t h, int32_t p)
 {
     const uint32_t w = h >> p;

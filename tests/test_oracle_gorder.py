@@ -252,3 +252,74 @@ def test_tile_order_pops_carry_max_score(oracle, seed, n, w1, w2):
         s = tile_scores(A, order, pid, w1, w2)
         assert s[order[pid]] == s[~placed].max(), f"pop {pid}"
         placed[order[pid]] = True
+
+
+@pytest.mark.parametrize("cap", [2, 3, 4])
+@pytest.mark.parametrize("seed,n,W", [(11, 120, 32), (12, 300, 64), (13, 257, 64)])
+def test_gpu_loop_hub_cap_equals_dense_brute_force(oracle, cap, seed, n, W):
+    """The hub cap of the window score (10,000 in the reference) as a
+    parameter: with caps of 2-4 the small graphs have hubs on both sides of
+    it, and the loop must equal the dense brute force with the same cap."""
+    rng = np.random.default_rng(seed)
+    rp, ci = random_csr(rng, n, max_row=7, empty_frac=0.2)
+    got = oracle.pgorder_gpu_loop(n, rp, ci, W, hub_cap=cap)
+    want = brute_gpu_loop(n, dense(n, rp, ci), W, cap=cap)
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+def _csr(n, edges):
+    rows = [[] for _ in range(n)]
+    for r, c in edges:
+        rows[r].append(c)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.array([c for r in rows for c in sorted(r)], np.int32)
+    return rp, ci
+
+
+def test_gorder1_hand_worked_update_order(oracle):
+    """gorder1 (class.cu:1881-1971) on n = 64 (L = floor(8/4) = 2), window 5,
+    multiplied matrix M with entries (5,9), (5,10), (7,5); every other vertex
+    isolated.  Worked by hand from the reference:
+      push 0..63 into bucket 0: the list reads 0, 63, 62, ..., 1 (each push
+      goes right after the head); start = first argmax row length = 5 -> bucket 1.
+      pop 5.  Rows-of-5 update first (collen(5) = 1 < L): 7 -> bucket 1 [7];
+      then cols(5) = {9, 10} ascending: 9 -> [7, 9], 10 -> [7, 10, 9]
+      (after the head).  pop 7, then 10, then 9; bucket 1 is empty, so
+      bucket 0 follows from its head: 0, 63, 62, ..., 1 without 5, 7, 9, 10.
+    Running the column loop before the row loop would pop 9 second."""
+    rp, ci = _csr(64, [(5, 9), (5, 10), (7, 5)])
+    order, perm = oracle.gorder1(64, rp, ci, 5)
+    want = [5, 7, 10, 9, 0] + [v for v in range(63, 0, -1) if v not in (5, 7, 9, 10)]
+    assert order.tolist() == want
+
+
+def test_gorder1_hand_worked_hub_above_cap(oracle):
+    """Symmetric star on n = 64 (L = 2): centre 10 with leaves 20, 21, 22 in
+    both directions.  collen(10) = 3 >= L, so popping a leaf reaches no
+    sibling through the centre.  By hand: start 10 (row length 3); pop 10,
+    its columns 20, 21, 22 enter bucket 1 as [20, 22, 21]; the leaves pop in
+    that order and add nothing; bucket 0 follows.  Ignoring the cap would give
+    popping 20 a +1 for 21 and 22 (both in rows(10)) and pop 21 before 22."""
+    edges = [(10, 20), (10, 21), (10, 22), (20, 10), (21, 10), (22, 10)]
+    rp, ci = _csr(64, edges)
+    order, _ = oracle.gorder1(64, rp, ci, 5)
+    want = [10, 20, 22, 21, 0] + [v for v in range(63, 0, -1) if v not in (10, 20, 21, 22)]
+    assert order.tolist() == want
+
+
+def test_order_on_cpu_hand_worked(oracle):
+    """order_on_cpu (class.cu:2220-2314) on the slice [3, 1, 4, 0, 2, 5] of
+    an 8-vertex M with rows 0:[4] 1:[4,6] 2:[4] 3:[6] 4:[6] 5:[6], window 5,
+    degree cap 100.  By hand:
+      the slice is pushed in order, so bucket 0 reads 3, 5, 2, 0, 4, 1;
+      start = first vertex with the largest column count = 4 (rows 0, 1, 2).
+      pop 4: y = 6, z in rows(6) = 1, 3, 5 (4 placed) -> bucket 1 [1, 5, 3].
+      pop 1: y = 4 gives 0, 2 -> bucket 1 [5, 2, 0, 3]; y = 6 moves 3 and 5 up
+      -> bucket 2 [3, 5], bucket 1 [2, 0].
+      pop 3: y = 6 moves 5 up -> bucket 3 [5].   pop 5: nothing left.
+      pop 2 (bucket 1 head): y = 4 moves 0 up.    pop 0.
+    Traversing rows(6) in descending order would pop 5 second."""
+    rp, ci = _csr(8, [(0, 4), (1, 4), (1, 6), (2, 4), (3, 6), (4, 6), (5, 6)])
+    got = oracle.order_on_cpu(8, rp, ci, np.array([3, 1, 4, 0, 2, 5]), 7, 5, 100)
+    assert got.tolist() == [4, 1, 3, 5, 2, 0]
