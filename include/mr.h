@@ -176,8 +176,9 @@ int mr_get_width(int64_t n, int64_t nnz, int* width);
 /* pGOrder = GRAPH::gorder6 (class.cu:2955-3140): GPU greedy placement in
  * batches (32 doubling up to width, then width), then Gorder with `window`
  * (5 in the reference) on host threads for every batch from log2(width).
- * width 0 = get_width.  Relabels the matrix in place; perm_out (host, n,
- * nullable) receives perm[old] = new. */
+ * width 0 = get_width; any power of two from 32 to 2^22 (1024, 16384 and
+ * 262144 are the ones get_width gives).  Relabels the matrix in place;
+ * perm_out (host, n, nullable) receives perm[old] = new. */
 int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_reorder_stats* stats);
 /* the two placement stages of gorder6 without relabelling (position -> vertex):
  * gpu_order after the GPU loop (class.cu:3062), final_order after the CPU
@@ -185,8 +186,8 @@ int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_re
 int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
                      mr_reorder_stats* stats);
 /* GRAPH::gorder5 (class.cu:2727-2873), the earlier pGOrder schedule: batches
- * of vsize (0 = get_width; at most 1024), start scores = column counts, the
- * GPU window = the last width1 batches (width1 * vsize <= 1024) with no hub
+ * of vsize (0 = get_width; at most 2^22), start scores = column counts, the
+ * GPU window = the last width1 batches (width1 * vsize <= 2^24) with no hub
  * caps, then order_on_cpu with window width2 on every batch.  The reference
  * calls it with (get_width, 1, 5) (main.cu:112-118).  Relabels. */
 int mr_reorder_gorder5(mr_mat m, int vsize, int width1, int width2, int32_t* perm_out, mr_reorder_stats* stats);

@@ -35,8 +35,12 @@
 // gorder5) with T = all vertices whose start length is non-zero.  The placement is identical to
 // the reference's for every input (the oracle restates the full-sort loop and
 // the tests compare the two bit for bit).
+#include <algorithm>
+
 #include "dtypes.hpp"
+#include "launch_perm.hpp"
 #include "launch_pg.hpp"
+#include "launch_radix.hpp"
 
 namespace mr {
 namespace k {
@@ -214,10 +218,10 @@ __global__ void init_kernel(int64_t n, const int32_t* __restrict__ rp, uint32_t*
 // reads the coarse bins first); later lists get theirs from touched_kernel
 __global__ __launch_bounds__(256) void hist_kernel(const int32_t* __restrict__ T, const PgState* __restrict__ st,
                                                    int cur, const uint32_t* __restrict__ score,
-                                                   int32_t* __restrict__ hist)
+                                                   int32_t* __restrict__ hist, int32_t small_t)
 {
     const int32_t tn = st->tcount[cur];
-    if (tn <= kSmallT) return;
+    if (tn <= small_t) return;
     __shared__ int32_t h[4096];
     __shared__ int32_t hc[kCoarse];
     for (int i = threadIdx.x; i < 4096; i += blockDim.x) h[i] = 0;
@@ -300,22 +304,28 @@ __device__ Thresh find_threshold(const int32_t* __restrict__ hist, int32_t tn, i
     return s_th;
 }
 
-// candidates of a large touched list: every block finds the threshold bin
-// itself from the histogram (a few KB read per block, no launch of its own),
-// block 0 records it for final_kernel, and the vertices above it (sure) and
-// in it (tie) are listed
+// candidates of a touched list longer than small_t: every block finds the
+// threshold bin itself from the histogram (a few KB read per block, no launch
+// of its own), block 0 records it for the selection, and the vertices above
+// it (sure, at most sure_cap) and in it (tie) are listed.  A list shorter than
+// |P| has no threshold (tb = -1): all of it is sure.
 __global__ __launch_bounds__(256) void collect_kernel(const int32_t* __restrict__ T, PgState* __restrict__ st,
                                                       int cur, const uint32_t* __restrict__ score,
                                                       int32_t* __restrict__ sure, int32_t* __restrict__ tie,
-                                                      int32_t pl, const int32_t* __restrict__ hist)
+                                                      int32_t pl, const int32_t* __restrict__ hist, int32_t small_t,
+                                                      int32_t sure_cap)
 {
     const int32_t tn = st->tcount[cur];
-    if (tn <= kSmallT || pl <= 0) return;
+    if (tn <= small_t || pl <= 0) return;
     const Thresh th = find_threshold(hist, tn, pl);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->tb = th.tb;
         st->above = th.above;
         st->need = th.need;
+        // the wide tie selection starts from the count when the bin is exact
+        st->kneed = th.need;
+        st->khi = (th.tb >= 0 && th.tb < kExactBins) ? ~(uint32_t)th.tb : 0u;
+        st->klo = 0u;
     }
     const int tb = th.tb;
     const int64_t lim = ((int64_t)tn + 63) / 64 * 64;
@@ -326,7 +336,7 @@ __global__ __launch_bounds__(256) void collect_kernel(const int32_t* __restrict_
             v = T[i];
             b = count_bin(score[v] & ~kUnplaced);
         }
-        wave_append(i < tn && b > tb, v, sure, &st->nsure, kSureCap);
+        wave_append(i < tn && b > tb, v, sure, &st->nsure, sure_cap);
         wave_append(i < tn && b == tb, v, tie, &st->ntie);
     }
 }
@@ -785,7 +795,9 @@ __global__ __launch_bounds__(256) void final_kernel(int64_t n, const int32_t* __
 // the longest lists.
 //
 // A work item is (chunk << 32) | y: entries [chunk * kItem, (chunk + 1) * kItem)
-// of the live part of rows(y).
+// of the live part of rows(y).  kLds: the window's prefix sums (ol <= 1024) are
+// staged in LDS; a wider window searches them in global memory.
+template <bool kLds>
 __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t* __restrict__ window, int32_t ol,
                                                        const int32_t* __restrict__ wprefix,
                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -803,9 +815,13 @@ __global__ __launch_bounds__(256) void window_y_kernel(int64_t n, const uint32_t
         if (score[v] != 0u) score[v] = kUnplaced;
     }
     for (int64_t i = gtid; i < kBins + kCoarse; i += threads) hist[i] = 0;
-    __shared__ int32_t s_pre[1025];
-    for (int i = threadIdx.x; i <= ol; i += blockDim.x) s_pre[i] = wprefix[i];
-    __syncthreads();
+    __shared__ int32_t s_lds[kLds ? 1025 : 1];
+    const int32_t* s_pre = wprefix;
+    if constexpr (kLds) {
+        for (int i = threadIdx.x; i <= ol; i += blockDim.x) s_lds[i] = wprefix[i];
+        __syncthreads();
+        s_pre = s_lds;
+    }
     const int32_t total = s_pre[ol];
     const int64_t lim = ((int64_t)total + 63) / 64 * 64;
     for (int64_t p = gtid; p < lim; p += threads) {
@@ -999,16 +1015,373 @@ __global__ __launch_bounds__(256) void compact_kernel(int64_t n, const int32_t* 
     }
 }
 
+// ---- wide batches: |P| or the window above 1024 ----
+//
+// The selection of the |P| first (count descending, id ascending) unplaced
+// vertices, spread over the grid:
+//   1. collect_kernel lists the touched vertices above the threshold bin
+//      (sure, fewer than |P|) and in it (tie);
+//   2. when only some ties are taken, the key of the need-th smallest tie is
+//      found digit by digit (11-bit digits of the 64-bit (~count, id) key,
+//      high to low: a histogram pass over the ties whose key matches the
+//      digits found so far, then one workgroup picks the digit).  An exact
+//      threshold bin fixes the count, so only the id digits are searched;
+//   3. the ties up to that key join the sure list: exactly min(|T|, |P|)
+//      vertices;
+//   4. two stable radix sorts (by id, then by count descending) order them;
+//   5. they are placed, then the untouched rest (every unplaced vertex left,
+//      all at count 0) fills the batch in ascending id order: one workgroup
+//      scans the per-1024-id unplaced counters for each chunk's first slot,
+//      one workgroup per chunk ranks and places its ids.
+// The window's prefix sums come from a device scan of its row lengths.
+
+constexpr int kDigitBins = 2048;   // 11-bit digits
+
+__device__ __forceinline__ bool tie_search_runs(const PgState* st, int shift)
+{
+    const int32_t need = st->need, nt = st->ntie;
+    if (need <= 0 || need >= nt) return false;   // no tie, or every tie, is taken
+    return !(shift >= 32 && st->tb < kExactBins);  // the count of an exact bin is known
+}
+
+__global__ __launch_bounds__(256) void tie_hist_kernel(const PgState* __restrict__ st, const int32_t* __restrict__ tie,
+                                                       const uint32_t* __restrict__ score, int shift, int bits,
+                                                       int32_t* __restrict__ hist2)
+{
+    if (!tie_search_runs(st, shift)) return;
+    const int32_t nt = st->ntie;
+    if ((int64_t)blockIdx.x * blockDim.x >= nt) return;   // uniform over the block
+    const uint64_t prefix = ((uint64_t)st->khi << 32) | st->klo;
+    const int top = shift + bits;
+    const int nb = 1 << bits;
+    __shared__ int32_t h[kDigitBins];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nt; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = tie[i];
+        const uint64_t kk = sel_key(score[v] & ~kUnplaced, (uint32_t)v);
+        if (top >= 64 || (kk >> top) == (prefix >> top)) atomicAdd(&h[(int)((kk >> shift) & (uint64_t)(nb - 1))], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist2[i], h[i]);
+}
+
+// one workgroup: the digit holding the kneed-th key; clears the bins
+__global__ __launch_bounds__(256) void tie_pick_kernel(PgState* __restrict__ st, int shift, int bits,
+                                                       int32_t* __restrict__ hist2)
+{
+    if (!tie_search_runs(st, shift)) return;
+    __shared__ int s_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int nb = 1 << bits;
+    const int per = (nb + 255) / 256;   // <= 8
+    const int32_t kneed = st->kneed;
+    int own[8];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int b = tid * per + q;
+        own[q] = (q < per && b < nb) ? hist2[b] : 0;
+        sum += own[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int b = tid * per + q;
+        if (q < per && b < nb) hist2[b] = 0;
+    }
+    int inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();   // also: every thread has read kneed
+    int before = inc - sum;
+    for (int w = 0; w < wid; ++w) before += s_w[w];
+    if (before < kneed && before + sum >= kneed) {
+        // the first bin of this thread where the count reaches kneed
+        int run = before, d = tid * per;
+        bool found = false;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (!found && q < per) {
+                if (run + own[q] >= kneed) {
+                    found = true;
+                    d = tid * per + q;
+                } else {
+                    run += own[q];
+                }
+            }
+        }
+        // run = keys under the prefix with a smaller digit than d
+        st->kneed = kneed - run;
+        if (shift >= 32) st->khi |= (uint32_t)d << (shift - 32);
+        else st->klo |= (uint32_t)d << shift;
+    }
+}
+
+// the ties taken join the sure list (every tie, or those up to the key found)
+__global__ __launch_bounds__(256) void tie_take_kernel(PgState* __restrict__ st, const int32_t* __restrict__ tie,
+                                                       const uint32_t* __restrict__ score, int32_t* __restrict__ sure,
+                                                       int32_t cap)
+{
+    const int32_t need = st->need, nt = st->ntie;
+    if (need <= 0) return;
+    const uint64_t kt = need >= nt ? ~0ull : (((uint64_t)st->khi << 32) | st->klo);
+    const int64_t lim = ((int64_t)nt + 63) / 64 * 64;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t v = 0;
+        bool take = false;
+        if (i < nt) {
+            v = tie[i];
+            take = sel_key(score[v] & ~kUnplaced, (uint32_t)v) <= kt;
+        }
+        wave_append(take, v, sure, &st->nsure, cap);
+    }
+}
+
+// sort 1 input: (id, id) for the m selected, (pad, pad) up to pl (pad = n
+// sorts after every id)
+__global__ void wide_keys1_kernel(const PgState* __restrict__ st, const int32_t* __restrict__ sel, int32_t pl,
+                                  uint32_t pad, uint32_t* __restrict__ k, uint32_t* __restrict__ v)
+{
+    const int32_t m = st->nsure;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < pl; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t id = i < m ? (uint32_t)sel[i] : pad;
+        k[i] = id;
+        v[i] = id;
+    }
+}
+
+// sort 2 keys over the id-sorted ids: 0x7fffffff - count (31 bits, count >= 1
+// for a touched vertex), the pads 0x7fffffff
+__global__ void wide_keys2_kernel(const PgState* __restrict__ st, const uint32_t* __restrict__ ids, int32_t pl,
+                                  const uint32_t* __restrict__ score, uint32_t* __restrict__ k)
+{
+    const int32_t m = st->nsure;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < pl; i += (int64_t)gridDim.x * blockDim.x)
+        k[i] = i < m ? (~(score[ids[i]] & ~kUnplaced)) & 0x7fffffffu : 0x7fffffffu;
+}
+
+__global__ void wide_place_kernel(const PgState* __restrict__ st, const uint32_t* __restrict__ ids, int32_t ps,
+                                  int32_t pl, uint32_t* __restrict__ score, uint32_t* __restrict__ placement,
+                                  int32_t* __restrict__ chunk_left)
+{
+    const int32_t m = st->nsure;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m && i < pl;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = ids[i];
+        score[v] = 0u;
+        placement[ps + i] = v;
+        atomicSub(&chunk_left[v / kChunk], 1);
+    }
+}
+
+// One workgroup: records how many touched vertices were placed, clears the
+// per-batch counters, and when the batch is not full, scans the unplaced
+// counters from lo_chunk: coff[c - lo] = fill slot of chunk c's first unplaced
+// id, up to the chunk holding the last one needed.
+__global__ __launch_bounds__(256) void wide_fill_scan_kernel(int64_t n, PgState* __restrict__ st, int nxt,
+                                                             int32_t pl, const int32_t* __restrict__ chunk_left,
+                                                             int32_t* __restrict__ coff)
+{
+    __shared__ int s_w[4];
+    __shared__ int64_t s_run;
+    __shared__ int64_t s_last;
+    __shared__ int64_t s_first;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int32_t m = st->nsure;
+    const int64_t need = (int64_t)pl - m;
+    const int64_t nchunks = (n + kChunk - 1) / kChunk;
+    const int64_t lo = st->lo_chunk;
+    if (tid == 0) {
+        s_run = 0;
+        s_last = -1;
+        s_first = nchunks;
+    }
+    __syncthreads();   // every thread has read the state
+    if (tid == 0) {
+        if (m > pl) st->overflow = 1;   // never expected: the host reports it
+        st->msel = m;
+        st->nsure = 0;
+        st->ntie = 0;
+        st->ycount = 0;
+        st->tcount[nxt] = 0;
+        st->fill_lo = (int32_t)lo;
+        st->fill_n = 0;
+    }
+    if (need <= 0) return;
+    for (int64_t c0 = lo; c0 < nchunks; c0 += 256 * 16) {
+        int v[16];
+        int sum = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int64_t c = c0 + 16 * tid + j;
+            v[j] = c < nchunks ? chunk_left[c] : 0;
+            sum += v[j];
+        }
+        int inc = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) s_w[wid] = inc;
+        __syncthreads();
+        int64_t run = s_run + inc - sum;
+        for (int w = 0; w < wid; ++w) run += s_w[w];
+        int64_t first = nchunks;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int64_t c = c0 + 16 * tid + j;
+            if (c < nchunks && run < need) {
+                coff[c - lo] = (int32_t)run;
+                if (run + v[j] >= need) s_last = c;
+            }
+            if (c < nchunks && v[j] > 0 && first == nchunks) first = c;
+            run += v[j];
+        }
+        if (first < nchunks) atomicMin(reinterpret_cast<unsigned long long*>(&s_first), (unsigned long long)first);
+        __syncthreads();
+        if (tid == 0) s_run += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        __syncthreads();
+        if (s_last >= 0) break;   // uniform
+    }
+    if (tid == 0) {
+        const int64_t last = s_last >= 0 ? s_last : nchunks - 1;
+        st->fill_n = (int32_t)(last - lo + 1);
+        st->lo_chunk = (int32_t)s_first;
+    }
+}
+
+// one workgroup per chunk of the fill: its unplaced ids in ascending order to
+// the slots from coff
+__global__ __launch_bounds__(256) void wide_fill_place_kernel(int64_t n, const PgState* __restrict__ st, int32_t ps,
+                                                              int32_t pl, const int32_t* __restrict__ coff,
+                                                              uint32_t* __restrict__ score,
+                                                              uint32_t* __restrict__ placement,
+                                                              int32_t* __restrict__ chunk_left)
+{
+    __shared__ int s_w[4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int32_t m = st->msel;
+    const int64_t need = (int64_t)pl - m;
+    if (need <= 0) return;
+    const int64_t lo = st->fill_lo;
+    const int32_t nc = st->fill_n;
+    for (int32_t cc = blockIdx.x; cc < nc; cc += gridDim.x) {
+        const int64_t base = coff[cc];   // uniform over the block
+        if (base >= need) continue;
+        const int64_t id0 = (lo + cc) * kChunk + 4 * tid;
+        uint32_t f = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (id0 + j < n && score[id0 + j] == kUnplaced) f |= 1u << j;
+        const int cnt = __popc(f);
+        int inc = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) s_w[wid] = inc;
+        __syncthreads();
+        int64_t r = base + inc - cnt;
+        for (int w = 0; w < wid; ++w) r += s_w[w];
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (f & (1u << j)) {
+                if (r < need) {
+                    score[id0 + j] = 0u;
+                    placement[ps + m + r] = (uint32_t)(id0 + j);
+                }
+                ++r;
+            }
+        }
+        if (tid == 0) {
+            const int64_t placed = tot < need - base ? tot : need - base;
+            atomicSub(&chunk_left[lo + cc], (int32_t)placed);
+        }
+        __syncthreads();   // s_w is rewritten by the next chunk
+    }
+}
+
+// row lengths of the window's vertices (0 at or above the hub cap, 0 for an
+// unplaced position), lens[ol] = 0 so an exclusive scan of ol + 1 entries ends
+// with the total
+__global__ void window_lens_kernel(int64_t n, const uint32_t* __restrict__ window, int32_t ol,
+                                   const int32_t* __restrict__ rp, int32_t hub_cap, int32_t* __restrict__ lens)
+{
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w <= ol; w += (int64_t)gridDim.x * blockDim.x) {
+        int32_t len = 0;
+        if (w < ol) {
+            const uint32_t x = window[w];
+            if (x < (uint32_t)n) {
+                len = rp[x + 1] - rp[x];
+                if (len >= hub_cap) len = 0;
+            }
+        }
+        lens[w] = len;
+    }
+}
+
 }  // namespace
 
 hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, PgState* st, int32_t* chunk_left,
-                   int32_t* hist, hipStream_t s)
+                   int32_t* hist, bool always_hist, hipStream_t s)
 {
     int64_t g = (n + 255) / 256;
     if (g > 4096) g = 4096;
     if (g < 1) g = 1;
     init_kernel<<<(unsigned)g, 256, 0, s>>>(n, rp, score, t0, st, chunk_left);
-    hist_kernel<<<kGridStride, 256, 0, s>>>(t0, st, 0, score, hist);
+    hist_kernel<<<kGridStride, 256, 0, s>>>(t0, st, 0, score, hist, always_hist ? -1 : kSmallT);
+    return hipGetLastError();
+}
+
+// the selection and placement of a wide batch (steps 1-5 above)
+static hipError_t pg_select_wide(int64_t n, int cur, uint32_t* score, int32_t* const Tl[2], PgState* st,
+                                 int32_t* sure, int32_t* tie, int32_t* hist, int32_t* chunk_left,
+                                 uint32_t* placement, int32_t ps, int32_t pl, const PgWide& w, hipStream_t s)
+{
+    const int nxt = cur ^ 1;
+    if (pl > w.cap) return hipErrorInvalidValue;
+    collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl, hist, -1, w.cap);
+    // digits of the tie key, high to low: the count word (searched only for an
+    // inexact bin), then the id bits below n
+    int id_bits = 1;
+    while (id_bits < 31 && ((n - 1) >> id_bits) != 0) ++id_bits;
+    int digits[8][2], nd = 0;
+    digits[nd][0] = 53, digits[nd][1] = 11, ++nd;
+    digits[nd][0] = 42, digits[nd][1] = 11, ++nd;
+    digits[nd][0] = 32, digits[nd][1] = 10, ++nd;
+    for (int left = id_bits; left > 0;) {
+        const int b = left < 11 ? left : 11;
+        left -= b;
+        digits[nd][0] = left, digits[nd][1] = b, ++nd;
+    }
+    for (int d = 0; d < nd; ++d) {
+        tie_hist_kernel<<<kGridStride, 256, 0, s>>>(st, tie, score, digits[d][0], digits[d][1], w.hist2);
+        tie_pick_kernel<<<1, 256, 0, s>>>(st, digits[d][0], digits[d][1], w.hist2);
+    }
+    tie_take_kernel<<<kGridStride, 256, 0, s>>>(st, tie, score, sure, w.cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // order: stable by id, then stable by count descending
+    const unsigned gp = (unsigned)std::min<int64_t>(((int64_t)pl + 255) / 256, 4096);
+    int key_bits = 1;
+    while (key_bits < 32 && ((uint64_t)n >> key_bits) != 0) ++key_bits;   // ids and the pad n
+    wide_keys1_kernel<<<gp, 256, 0, s>>>(st, sure, pl, (uint32_t)n, w.ka, w.va);
+    e = radix_sort_pairs(w.ka, w.va, w.kb, w.vb, pl, key_bits, w.rws, w.rws_bytes, s);
+    if (e != hipSuccess) return e;
+    wide_keys2_kernel<<<gp, 256, 0, s>>>(st, w.vb, pl, score, w.ka);
+    e = radix_sort_pairs(w.ka, w.vb, w.kb, w.va, pl, 31, w.rws, w.rws_bytes, s);
+    if (e != hipSuccess) return e;
+    wide_place_kernel<<<gp, 256, 0, s>>>(st, w.va, ps, pl, score, placement, chunk_left);
+    wide_fill_scan_kernel<<<1, 256, 0, s>>>(n, st, nxt, pl, chunk_left, w.coff);
+    wide_fill_place_kernel<<<kGridStride, 256, 0, s>>>(n, st, ps, pl, w.coff, score, placement, chunk_left);
     return hipGetLastError();
 }
 
@@ -1019,14 +1392,27 @@ hipError_t pg_init(int64_t n, const int32_t* rp, uint32_t* score, int32_t* t0, P
 hipError_t pg_batch(int64_t n, int cur, const int32_t* rp, const int32_t* ci, const int32_t* tp, const int32_t* ti,
                     uint32_t* score, int32_t* const Tl[2], PgState* st, int32_t* sure, int32_t* tie, int32_t* hist,
                     int32_t* chunk_left, uint32_t* placement, int32_t* wprefix, int32_t* h, uint64_t* items,
-                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap, hipStream_t s)
+                    int32_t* live, int32_t ps, int32_t pl, int32_t os, int32_t ol, int32_t hub_cap,
+                    const PgWide* wide, hipStream_t s)
 {
     const int nxt = cur ^ 1;
-    collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl, hist);
-    final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, chunk_left, placement, ps, pl, rp, os, ol,
-                                   wprefix, hub_cap);
-    window_y_kernel<<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, live, items, st,
-                                                hub_cap, Tl[cur], cur, score, hist);
+    if (pl <= kPgNarrow && ol <= kPgNarrow) {
+        collect_kernel<<<kGridStride, 256, 0, s>>>(Tl[cur], st, cur, score, sure, tie, pl, hist, kSmallT, kSureCap);
+        final_kernel<<<1, 256, 0, s>>>(n, Tl[cur], cur, st, score, sure, tie, chunk_left, placement, ps, pl, rp, os,
+                                       ol, wprefix, hub_cap);
+        window_y_kernel<true><<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, live, items,
+                                                          st, hub_cap, Tl[cur], cur, score, hist);
+    } else {
+        if (!wide) return hipErrorInvalidValue;
+        hipError_t e = pg_select_wide(n, cur, score, Tl, st, sure, tie, hist, chunk_left, placement, ps, pl, *wide, s);
+        if (e != hipSuccess) return e;
+        const unsigned gl = (unsigned)std::min<int64_t>(((int64_t)ol + 256) / 256, 4096);
+        window_lens_kernel<<<gl, 256, 0, s>>>(n, placement + os, ol, rp, hub_cap, wide->lens);
+        e = exclusive_scan_i32(wide->lens, (int64_t)ol + 1, wprefix, nullptr, wide->sws, s);
+        if (e != hipSuccess) return e;
+        window_y_kernel<false><<<kWindowGrid, 256, 0, s>>>(n, placement + os, ol, wprefix, rp, ci, tp, h, live, items,
+                                                           st, hub_cap, Tl[cur], cur, score, hist);
+    }
     if (ol > 0) {
         window_z_kernel<<<kWindowGrid, 256, 0, s>>>(tp, ti, live, h, items, score, st);
         const int64_t per_block = 256 * (int64_t)kTouchedPer;

@@ -19,6 +19,7 @@
 #include "kernels/launch_degree.hpp"
 #include "kernels/launch_perm.hpp"
 #include "kernels/launch_pg.hpp"
+#include "kernels/launch_radix.hpp"
 #include "kernels/launch_transpose.hpp"
 #include "parallel.hpp"
 #include "reorder.hpp"
@@ -320,13 +321,43 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     MR_TRY(score.alloc(sizeof(uint32_t) * (size_t)n));
     MR_TRY(T0.alloc(sizeof(int32_t) * (size_t)n));
     MR_TRY(T1.alloc(sizeof(int32_t) * (size_t)n));
-    MR_TRY(sure.alloc(sizeof(int32_t) * 4096));
+    // batches with |P| or a window above 1024 take the wide selection
+    const int32_t max_pl = B > 0 ? *std::max_element(P.Pl.begin(), P.Pl.end()) : 0;
+    const int32_t max_ol = B > 0 ? *std::max_element(P.Ol.begin(), P.Ol.end()) : 0;
+    const bool any_wide = max_pl > k::kPgNarrow || max_ol > k::kPgNarrow;
+    MR_TRY(sure.alloc(sizeof(int32_t) * (size_t)std::max<int32_t>(4096, max_pl)));
     MR_TRY(tie.alloc(sizeof(int32_t) * (size_t)n));
     MR_TRY(hist.alloc(sizeof(int32_t) * (size_t)k::pg_hist_bins()));
     MR_TRY(chunk.alloc(sizeof(int32_t) * (size_t)((n + 1023) / 1024 + 1)));
     MR_TRY(placement.alloc(sizeof(uint32_t) * (size_t)n));
     MR_TRY(state.alloc(sizeof(k::PgState)));
-    MR_TRY(wprefix.alloc(sizeof(int32_t) * 1025));
+    MR_TRY(wprefix.alloc(sizeof(int32_t) * (size_t)(std::max<int32_t>(1024, max_ol) + 1)));
+    DevBuf wka, wva, wkb, wvb, wrws, wcoff, wlens, wsws, whist2;
+    k::PgWide wide{};
+    if (any_wide) {
+        const int32_t cap = std::max<int32_t>(1, max_pl);
+        MR_TRY(wka.alloc(sizeof(uint32_t) * (size_t)cap));
+        MR_TRY(wva.alloc(sizeof(uint32_t) * (size_t)cap));
+        MR_TRY(wkb.alloc(sizeof(uint32_t) * (size_t)cap));
+        MR_TRY(wvb.alloc(sizeof(uint32_t) * (size_t)cap));
+        MR_TRY(wrws.alloc(k::radix_sort_ws_bytes(cap)));
+        MR_TRY(wcoff.alloc(sizeof(int32_t) * (size_t)((n + 1023) / 1024 + 1)));
+        MR_TRY(wlens.alloc(sizeof(int32_t) * (size_t)(max_ol + 1)));
+        MR_TRY(wsws.alloc(k::scan_workspace_bytes((int64_t)max_ol + 1)));
+        MR_TRY(whist2.alloc(sizeof(int32_t) * 2048));
+        MR_HIP_TRY(hipMemsetAsync(whist2.p, 0, whist2.bytes, s));
+        wide.cap = cap;
+        wide.ka = wka.as<uint32_t>();
+        wide.va = wva.as<uint32_t>();
+        wide.kb = wkb.as<uint32_t>();
+        wide.vb = wvb.as<uint32_t>();
+        wide.rws = wrws.p;
+        wide.rws_bytes = wrws.bytes;
+        wide.coff = wcoff.as<int32_t>();
+        wide.lens = wlens.as<int32_t>();
+        wide.sws = wsws.p;
+        wide.hist2 = whist2.as<int32_t>();
+    }
     MR_TRY(hcol.alloc(sizeof(int32_t) * (size_t)n));
     MR_TRY(items.alloc(sizeof(uint64_t) * (size_t)k::pg_item_capacity(n, nnz)));
     MR_HIP_TRY(hipMemsetAsync(hcol.p, 0, hcol.bytes, s));
@@ -344,7 +375,8 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
     MR_HIP_TRY(k::pg_live_init(n, tp.as<int32_t>(), live.as<int32_t>(), s));
     const int32_t compact_every = std::max<int32_t>(32, B / 32);
     MR_HIP_TRY(k::pg_init(n, P.start_columns ? tp.as<int32_t>() : m->rp.as<int32_t>(), score.as<uint32_t>(),
-                          T0.as<int32_t>(), st, chunk.as<int32_t>(), hist.as<int32_t>(), s));
+                          T0.as<int32_t>(), st, chunk.as<int32_t>(), hist.as<int32_t>(),
+                          B > 0 && (P.Pl[0] > k::kPgNarrow || P.Ol[0] > k::kPgNarrow), s));
     int32_t* Tl[2] = {T0.as<int32_t>(), T1.as<int32_t>()};
 
     // ---- host copies of both adjacency lists for the CPU phase ----
@@ -495,12 +527,22 @@ static int gorder_family_order(mr_mat m, const GorderPlan& P, uint32_t* gpu_orde
                             score.as<uint32_t>(), Tl, st, sure.as<int32_t>(), tie.as<int32_t>(), hist.as<int32_t>(),
                             chunk.as<int32_t>(), placement.as<uint32_t>(), wprefix.as<int32_t>(), hcol.as<int32_t>(),
                             items.as<uint64_t>(), live.as<int32_t>(), P.Ps[(size_t)b], P.Pl[(size_t)b],
-                            P.Os[(size_t)b], P.Ol[(size_t)b], P.hub_cap, s);
+                            P.Os[(size_t)b], P.Ol[(size_t)b], P.hub_cap, any_wide ? &wide : nullptr, s);
         if (e != hipSuccess) rc = set_error(MR_ERR_HIP, "pGOrder batch %d failed: %s", b, hipGetErrorString(e));
         cur ^= 1;
         if (rc == MR_OK && (b % kChunk == kChunk - 1 || b == B - 1)) rc = publish(b);
     }
     if (rc == MR_OK && hipStreamSynchronize(s) != hipSuccess) rc = set_error(MR_ERR_HIP, "pGOrder loop failed");
+    if (rc == MR_OK && any_wide) {
+        k::PgState hst{};
+        if (hipMemcpy(&hst, st, sizeof(hst), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = set_error(MR_ERR_HIP, "pGOrder state copy failed");
+        else if (hst.overflow)
+            rc = set_error(MR_ERR_NUMERIC, "pGOrder wide selection listed more vertices than the batch holds");
+    }
+    // on failure too, every queued copy into the pinned buffers must land
+    // before their destructors free them
+    if (rc != MR_OK) (void)hipStreamSynchronize(s);
     const double gpu_ms = ms_since(t_gpu);
     if (rc == MR_OK) {
         std::vector<int32_t> chk(ghost.p, ghost.p + n);
@@ -544,8 +586,8 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     if (n64 == 0) return MR_OK;
     int W = width;
     if (W <= 0) MR_TRY(mr_get_width(n64, nnz, &W));
-    if (W < 32 || W > 1024 || (W & (W - 1)))
-        return set_error(MR_ERR_UNSUPPORTED, "pGOrder GPU loop supports power-of-two widths 32..1024 (got %d)", W);
+    if (W < 32 || W > (1 << 22) || (W & (W - 1)))
+        return set_error(MR_ERR_UNSUPPORTED, "pGOrder GPU loop supports power-of-two widths 32..2^22 (got %d)", W);
     const int32_t n = (int32_t)n64;
     GorderPlan P;
     const int32_t B = pg_schedule(n, W, P.Ps, P.Pl, P.Os, P.Ol);
@@ -578,10 +620,10 @@ int mr_gorder5_order(mr_mat m, int vsize, int width1, int width2, uint32_t* gpu_
     const int64_t n64 = m->n_rows, nnz = m->nnz;
     if (n64 == 0) return MR_OK;
     if (vsize <= 0) MR_TRY(mr_get_width(n64, nnz, &vsize));
-    if (vsize < 1 || vsize > 1024)
-        return set_error(MR_ERR_UNSUPPORTED, "gorder5: batch size must be in [1, 1024] (got %d)", vsize);
-    if (width1 < 1 || (int64_t)width1 * vsize > 1024)
-        return set_error(MR_ERR_UNSUPPORTED, "gorder5: the GPU window (width1 x batch size) must hold 1..1024 vertices");
+    if (vsize < 1 || vsize > (1 << 22))
+        return set_error(MR_ERR_UNSUPPORTED, "gorder5: batch size must be in [1, 2^22] (got %d)", vsize);
+    if (width1 < 1 || (int64_t)width1 * vsize > (int64_t)1 << 24)
+        return set_error(MR_ERR_UNSUPPORTED, "gorder5: the GPU window (width1 x batch size) must hold 1..2^24 vertices");
     if (width2 < 1) return set_error(MR_ERR_INVALID_ARG, "gorder5: width2 must be >= 1");
     const int32_t n = (int32_t)n64;
     GorderPlan P;

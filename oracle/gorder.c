@@ -211,6 +211,12 @@ int oracle_pgorder_gpu_loop(int32_t n, const int32_t* rp, const int32_t* ci,
 }
 
 /* ------------------------------------------------------------------------ */
+static int cmp_pair_first(const void* a, const void* b)
+{
+    const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
 void oracle_order_on_cpu(int32_t n, const int32_t* rp, const int32_t* ci,
                          const int32_t* tp, const int32_t* ti, int32_t* batch_of,
                          uint32_t* slice, int32_t batch_id, int32_t len,
@@ -245,16 +251,8 @@ void oracle_order_on_cpu(int32_t n, const int32_t* rp, const int32_t* ci,
     /* map vertex -> local index: a sorted copy for bsearch */
     int32_t* sorted = (int32_t*)malloc(sizeof(int32_t) * (size_t)len * 2);
     for (int32_t i = 0; i < len; ++i) { sorted[2 * i] = local[i]; sorted[2 * i + 1] = i; }
-    /* insertion sort by vertex (len <= W) */
-    for (int32_t i = 1; i < len; ++i) {
-        int32_t kv = sorted[2 * i], ki = sorted[2 * i + 1], j = i - 1;
-        while (j >= 0 && sorted[2 * j] > kv) {
-            sorted[2 * (j + 1)] = sorted[2 * j];
-            sorted[2 * (j + 1) + 1] = sorted[2 * j + 1];
-            --j;
-        }
-        sorted[2 * (j + 1)] = kv; sorted[2 * (j + 1) + 1] = ki;
-    }
+    /* by vertex (the vertices of a slice are distinct) */
+    qsort(sorted, (size_t)len, 2 * sizeof(int32_t), cmp_pair_first);
 #define LOCAL_OF(v, out) do { int32_t lo_ = 0, hi_ = len - 1; out = -1;          \
         while (lo_ <= hi_) { int32_t mid_ = (lo_ + hi_) >> 1;                    \
             if (sorted[2 * mid_] == (v)) { out = sorted[2 * mid_ + 1]; break; } \

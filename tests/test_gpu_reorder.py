@@ -109,6 +109,37 @@ def pgorder_cases():
     out.append(("rmat_s13_w32_compacted", *rmat_csr(13, 16, seed=13), 32))
     out.append(("rmat_s15_w64_compacted", *rmat_csr(15, 16, seed=15), 64))
     out.extend(selection_path_cases())
+    out.extend(wide_cases())
+    return out
+
+
+def wide_cases():
+    """Widths above 1024: from the batch where |P| or the window passes 1024
+    the selection is spread over the grid (tie key found digit by digit, two
+    stable radix sorts, a multi-workgroup fill) and the window's prefix sums
+    come from a device scan.
+
+    width 0 takes get_width: R-MAT s20 with two edges per row gives 16384,
+    one edge per row (nnz < n, so nnz / n = 0) gives 262144, the widths of
+    every published R-MAT s24 run.  The explicit widths run the wide path on
+    small graphs for many batches: ties of one exact count larger than the
+    need (the id digits), counts of 131072 (an inexact bin: the count digits
+    too), batches filled from untouched vertices, hubs above the cap, and the
+    column-list compaction every 32 batches."""
+    rng = np.random.default_rng(43)
+    out = [("rmat_s20_ef2_width16384", *rmat_csr(20, 2, seed=7), 0),
+           ("rmat_s20_ef1_width262144", *rmat_csr(20, 1, seed=7), 0),
+           ("rmat_s14_w2048", *rmat_csr(14, 16, seed=1), 2048),
+           ("rmat_s17_ef4_w2048_compacted", *rmat_csr(17, 4, seed=17), 2048)]
+    rp, ci = random_csr(rng, 9000, empty_frac=0.4, max_row=12)
+    out.append(("random_9000_w4096", rp, ci, 4096))
+    out.append(("no_edges_9000_w2048", np.zeros(9001, np.int32), np.zeros(0, np.int32), 2048))
+    n = 30000
+    rows = [np.sort(rng.choice(n, 12000, replace=False)) if i in (5, 9000) else
+            np.sort(rng.choice(n, int(rng.integers(0, 6)), replace=False)) for i in range(n)]
+    out.append(("hubs_30000_w4096", *csr_rows(rows, n), 4096))
+    for name, rp, ci, _ in selection_path_cases():
+        out.append((name.replace("_w1024", "_w2048"), rp, ci, 2048))
     return out
 
 
@@ -148,6 +179,9 @@ def test_pgorder_bit_exact(ctx, mr, oracle, case):
     n = len(rp) - 1
     mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
     gpu_order, final_order, st = mat.pgorder_order(W, 5)
+    if W == 0:
+        W = oracle.get_width(n, len(ci))
+        assert st["width"] == W, (st["width"], W)
     og, of, operm = oracle.pgorder(n, rp, ci, W, 5)
     assert np.array_equal(gpu_order, og), "GPU placement loop differs"
     assert np.array_equal(final_order, of), "CPU Gorder phase differs"
@@ -278,6 +312,11 @@ def gorder5_cases():
     out.append(("hubs_30000", *csr_rows(rows, n), 1024, 1, 5))
     out.append(("rmat_s13_v32_compacted", *rmat_csr(13, 16, seed=13), 32, 1, 5))
     out.extend((name, rp, ci, W, 1, 5) for name, rp, ci, W in selection_path_cases())
+    # batches and windows above 1024 (the wide selection), a batch size that is
+    # not a power of two
+    out.append(("rmat_s14_v2048_w2", *rmat_csr(14, 16, seed=1), 2048, 2, 5))
+    rp, ci = random_csr(rng, 9000, empty_frac=0.3, max_row=12)
+    out.append(("random_9000_v1500_w3", rp, ci, 1500, 3, 5))
     return out
 
 
@@ -301,7 +340,7 @@ def test_gorder5_bit_exact(ctx, mr, oracle, case):
 def test_gorder5_argument_limits(ctx, mr):
     rp, ci = rmat_csr(10, 8, seed=1)
     mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
-    for args in ((2048, 1, 5), (512, 3, 5), (64, 0, 5), (64, 1, 0)):
+    for args in ((1 << 23, 1, 5), (1 << 22, 5, 5), (64, 0, 5), (64, 1, 0)):
         with pytest.raises(mr.MRError):
             mat.reorder_gorder5(*args)
     mat.close()
