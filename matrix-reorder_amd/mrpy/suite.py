@@ -12,15 +12,15 @@ spmv.  Per matrix it records:
 
   origin     MERBIT time on the generated order (autotuned variant);
   predicted  the RD headroom predictor's speedup (sm_89 and MI355X models);
-  pgorder    pGOrder time (width 1024: get_width asks for 16384 at s24, beyond
-             the GPU loop's 1024) and the MERBIT time after it;
+  pgorder    pGOrder time (width 0 = get_width, 16384 at s24, as the
+             reference runs it) and the MERBIT time after it;
 
 beside the published origin / gorder1 times of the same id
 (suite/rmat_s24_published.json).
 
 Usage, from the package directory:
     python -m mrpy.suite --ids 1-25 --out ../gpurun_out/suite/rmat_1_25.json
-    python -m mrpy.suite --summarize run1.json run2.json --out suite/rmat_s24_mi355x.json
+    python -m mrpy.suite --summarize run1.json run2.json --out ../gpurun_out/suite/rmat_s24_mi355x.json
 """
 from __future__ import annotations
 
@@ -64,7 +64,7 @@ def log(*a):
     print("[suite]", *a, file=sys.stderr, flush=True)
 
 
-def run(ids, out_path: str, width: int = 1024) -> list:
+def run(ids, out_path: str, width: int = 0) -> list:
     import mrpy as mr
 
     pub = {m["id"]: m for m in json.loads(PUBLISHED.read_text())["matrices"]}
@@ -155,7 +155,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m mrpy.suite")
     ap.add_argument("--ids", default="1-75")
     ap.add_argument("--out", default="")
-    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--summarize", nargs="*", default=None, help="merge these run files instead of running")
     a = ap.parse_args(argv)
     if a.summarize is not None:
