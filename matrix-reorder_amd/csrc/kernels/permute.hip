@@ -13,14 +13,15 @@
 //                group of 2..64 lanes of a wave64 (xor shuffles), 64/w rows
 //                per wave
 //   class 7      len <= 4096: one workgroup, bitonic sort in LDS
-//   class 8      longer: all such rows at once, one device-wide stable radix
-//                sort of (row slot, new column) keys with positions as values
+//   class 8      longer: all such rows at once, two device-wide stable radix
+//                sorts (radix.hip) of their entries, by new column and then by
+//                row slot (equal columns stay in row order)
 // No floating point is touched: the output is bit-identical to a sequential
 // relabel + stable sort.
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "dtypes.hpp"
 #include "launch_perm.hpp"
+#include "launch_radix.hpp"
 
 namespace mr {
 namespace k {
@@ -185,34 +186,44 @@ __global__ __launch_bounds__(256) void sort_block_kernel(int32_t count, const in
 
 // long rows: key = (slot << cbits) | new column, value = position in the old
 // row; one block per row
+// entry g = boff[i] + t of slot i: key = its new column, value = g, slot[g] = i
 __global__ __launch_bounds__(256) void long_keys_kernel(int32_t count, const int32_t* __restrict__ rows,
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                         const int32_t* __restrict__ perm,
                                                         const int32_t* __restrict__ inv,
-                                                        const int64_t* __restrict__ boff, int cbits,
-                                                        uint64_t* __restrict__ keys, int32_t* __restrict__ pos)
+                                                        const int64_t* __restrict__ boff, uint32_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ ent, int32_t* __restrict__ slot)
 {
     const int32_t i = blockIdx.x;
     const int32_t p = rows[i], o = inv[p];
     const int32_t ob = rp[o], len = rp[o + 1] - ob;
     const int64_t base = boff[i];
     for (int32_t t = threadIdx.x; t < len; t += blockDim.x) {
-        keys[base + t] = ((uint64_t)(uint32_t)i << cbits) | (uint32_t)perm[ci[ob + t]];
-        pos[base + t] = t;
+        keys[base + t] = (uint32_t)perm[ci[ob + t]];
+        ent[base + t] = (uint32_t)(base + t);
+        slot[base + t] = i;
     }
 }
 
-// slot i's sorted entries sit at [boff[i], boff[i] + len): columns and values
+// keys of the second sort: the slot of each column-sorted entry
+__global__ void long_slot_keys_kernel(int64_t total, const uint32_t* __restrict__ ent, const int32_t* __restrict__ slot,
+                                      uint32_t* __restrict__ keys)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x)
+        keys[j] = (uint32_t)slot[ent[j]];
+}
+
+// slot i's sorted entries sit at [boff[i], boff[i] + len): ent holds the
+// entry each came from, so its old position in the row is ent - boff[i]
 template <typename E>
 __global__ __launch_bounds__(256) void long_write_kernel(int32_t count, const int32_t* __restrict__ rows,
-                                                         const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                         const int32_t* __restrict__ perm,
                                                          const int32_t* __restrict__ inv,
                                                          const int32_t* __restrict__ new_rp,
                                                          const int64_t* __restrict__ boff,
-                                                         const uint64_t* __restrict__ keys,
-                                                         const int32_t* __restrict__ pos, const E* __restrict__ val,
-                                                         uint64_t cmask, int32_t* __restrict__ new_ci,
-                                                         E* __restrict__ new_val)
+                                                         const uint32_t* __restrict__ ent, const E* __restrict__ val,
+                                                         int32_t* __restrict__ new_ci, E* __restrict__ new_val)
 {
     const int32_t i = blockIdx.x;
     const int32_t p = rows[i], o = inv[p];
@@ -220,8 +231,9 @@ __global__ __launch_bounds__(256) void long_write_kernel(int32_t count, const in
     const int32_t nb = new_rp[p];
     const int64_t base = boff[i];
     for (int32_t t = threadIdx.x; t < len; t += blockDim.x) {
-        new_ci[nb + t] = (int32_t)(keys[base + t] & cmask);
-        new_val[nb + t] = val[ob + pos[base + t]];
+        const int32_t tt = (int32_t)((int64_t)ent[base + t] - base);
+        new_ci[nb + t] = perm[ci[ob + tt]];
+        new_val[nb + t] = val[ob + tt];
     }
 }
 
@@ -348,19 +360,17 @@ hipError_t perm_long_lengths(int32_t count, const int32_t* rows, const int32_t* 
     return hipGetLastError();
 }
 
-static size_t long_sort_temp(int64_t total, unsigned bits)
+// five arrays of total entries (keys, entries, their sorted copies, slots)
+static size_t long_arrays_bytes(int64_t total)
 {
-    size_t tmp = 0;
-    rocprim::radix_sort_pairs(nullptr, tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr, (const int32_t*)nullptr,
-                              (int32_t*)nullptr, (size_t)total, 0u, bits, (hipStream_t)0);
-    return tmp;
+    return (5 * sizeof(uint32_t) * (size_t)(total > 0 ? total : 1) + 255) / 256 * 256;
 }
 
 size_t perm_long_ws_bytes(int64_t total, int32_t count, int64_t n)
 {
-    const unsigned bits = bits_for(count > 0 ? count - 1 : 0) + bits_for(n > 0 ? n - 1 : 0);
-    return (2 * sizeof(uint64_t) + 2 * sizeof(int32_t)) * (size_t)(total > 0 ? total : 1) +
-           long_sort_temp(total, bits) + 256;
+    (void)count;
+    (void)n;
+    return long_arrays_bytes(total) + radix_sort_ws_bytes(total) + 256;
 }
 
 hipError_t perm_sort_long(int32_t count, const int32_t* rows, const int32_t* rp, const int32_t* ci, const void* val,
@@ -369,36 +379,48 @@ hipError_t perm_sort_long(int32_t count, const int32_t* rows, const int32_t* rp,
                           size_t ws_bytes, hipStream_t s)
 {
     if (count <= 0 || total <= 0) return hipSuccess;
+    // (slot, new column) order with equal columns in row order: stable by
+    // column, then stable by slot
+    if (total > ((int64_t)1 << 31) - 1) return hipErrorInvalidValue;
     const unsigned cbits = bits_for(n > 0 ? n - 1 : 0);
-    const unsigned bits = bits_for(count - 1) + cbits;
-    uint64_t* keys = static_cast<uint64_t*>(ws);
-    uint64_t* keys_out = keys + total;
-    int32_t* pos = reinterpret_cast<int32_t*>(keys_out + total);
-    int32_t* pos_out = pos + total;
-    void* tmp = pos_out + total;
-    size_t tmp_bytes = long_sort_temp(total, bits);
-    if (ws_bytes < (2 * sizeof(uint64_t) + 2 * sizeof(int32_t)) * (size_t)total + tmp_bytes) return hipErrorInvalidValue;
-    long_keys_kernel<<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, perm, inv, boff, (int)cbits, keys, pos);
+    const unsigned sbits = bits_for(count - 1);
+    const size_t arrays = long_arrays_bytes(total);
+    if (ws_bytes < arrays + radix_sort_ws_bytes(total)) return hipErrorInvalidValue;
+    uint32_t* ka = static_cast<uint32_t*>(ws);
+    uint32_t* ea = ka + total;
+    uint32_t* kb = ea + total;
+    uint32_t* eb = kb + total;
+    int32_t* slot = reinterpret_cast<int32_t*>(eb + total);
+    void* tmp = static_cast<char*>(ws) + arrays;
+    const size_t tmp_bytes = ws_bytes - arrays;
+    long_keys_kernel<<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, perm, inv, boff, ka, ea, slot);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, static_cast<const uint64_t*>(keys), keys_out,
-                                  static_cast<const int32_t*>(pos), pos_out, (size_t)total, 0u, bits, s);
+    e = radix_sort_pairs(ka, ea, kb, eb, total, (int)cbits, tmp, tmp_bytes, s);
     if (e != hipSuccess) return e;
-    const uint64_t cmask = ((uint64_t)1 << cbits) - 1;
+    {
+        int64_t g = (total + 255) / 256;
+        if (g > 65536) g = 65536;
+        long_slot_keys_kernel<<<(unsigned)g, 256, 0, s>>>(total, eb, slot, ka);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = radix_sort_pairs(ka, eb, kb, ea, total, (int)sbits, tmp, tmp_bytes, s);
+    if (e != hipSuccess) return e;
     switch (val_bytes) {
     case 2:
-        long_write_kernel<uint16_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, inv, new_rp, boff, keys_out, pos_out,
-                                                                    static_cast<const uint16_t*>(val), cmask, new_ci,
+        long_write_kernel<uint16_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, perm, inv, new_rp, boff, ea,
+                                                                    static_cast<const uint16_t*>(val), new_ci,
                                                                     static_cast<uint16_t*>(new_val));
         break;
     case 4:
-        long_write_kernel<uint32_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, inv, new_rp, boff, keys_out, pos_out,
-                                                                    static_cast<const uint32_t*>(val), cmask, new_ci,
+        long_write_kernel<uint32_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, perm, inv, new_rp, boff, ea,
+                                                                    static_cast<const uint32_t*>(val), new_ci,
                                                                     static_cast<uint32_t*>(new_val));
         break;
     case 8:
-        long_write_kernel<uint64_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, inv, new_rp, boff, keys_out, pos_out,
-                                                                    static_cast<const uint64_t*>(val), cmask, new_ci,
+        long_write_kernel<uint64_t><<<(unsigned)count, 256, 0, s>>>(count, rows, rp, ci, perm, inv, new_rp, boff, ea,
+                                                                    static_cast<const uint64_t*>(val), new_ci,
                                                                     static_cast<uint64_t*>(new_val));
         break;
     default: return hipErrorInvalidValue;

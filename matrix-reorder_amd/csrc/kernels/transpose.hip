@@ -2,13 +2,13 @@
 //
 // The entries of a CSR are stored row by row, so a stable sort of the
 // (column, row) pairs by column alone leaves each column's rows ascending:
-// one rocPRIM radix sort over the column bits, then the column starts read
+// one stable LSD radix sort (radix.hip) over the column bits, then the column starts read
 // off the sorted keys.  No atomics and no per-column sort, and the result does
 // not depend on scheduling.  Used for the row lists rows(y) = { x : A[x][y] }
 // that the pGOrder window score and the host orderers walk.
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
 
+#include "launch_radix.hpp"
 #include "launch_transpose.hpp"
 
 namespace mr {
@@ -52,20 +52,13 @@ unsigned key_bits(int64_t n_cols)
     return b;
 }
 
-size_t sort_temp_bytes(int64_t nnz, unsigned bits)
-{
-    size_t tmp = 0;
-    rocprim::radix_sort_pairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const int32_t*)nullptr,
-                              (int32_t*)nullptr, (size_t)nnz, 0u, bits, (hipStream_t)0);
-    return tmp;
-}
-
 }  // namespace
 
 size_t transpose_ws_bytes(int64_t nnz, int64_t n_cols)
 {
+    (void)n_cols;
     const size_t pairs = 2 * sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1);
-    return pairs + sort_temp_bytes(nnz, key_bits(n_cols)) + 256;
+    return pairs + radix_sort_ws_bytes(nnz) + 256;
 }
 
 hipError_t transpose_sorted(int64_t n_rows, int64_t n_cols, int64_t nnz, const int32_t* rp, const int32_t* ci,
@@ -77,13 +70,13 @@ hipError_t transpose_sorted(int64_t n_rows, int64_t n_cols, int64_t nnz, const i
     int32_t* rows = reinterpret_cast<int32_t*>(keys + nnz);
     void* tmp = rows + nnz;
     const size_t used = 2 * sizeof(int32_t) * (size_t)nnz;
-    size_t tmp_bytes = sort_temp_bytes(nnz, bits);
+    const size_t tmp_bytes = radix_sort_ws_bytes(nnz);
     if (ws_bytes < used + tmp_bytes) return hipErrorInvalidValue;
     row_ids_kernel<<<grid_for(n_rows * 8, 16384), 256, 0, s>>>(n_rows, rp, rows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, reinterpret_cast<const uint32_t*>(ci), keys,
-                                  static_cast<const int32_t*>(rows), ti, (size_t)nnz, 0u, bits, s);
+    e = radix_sort_pairs(reinterpret_cast<const uint32_t*>(ci), reinterpret_cast<const uint32_t*>(rows), keys,
+                         reinterpret_cast<uint32_t*>(ti), nnz, (int)bits, tmp, tmp_bytes, s);
     if (e != hipSuccess) return e;
     col_starts_kernel<<<grid_for(nnz + 1, 65536), 256, 0, s>>>(nnz, n_cols, keys, tp);
     return hipGetLastError();
