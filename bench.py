@@ -207,8 +207,12 @@ def section_main(mr, ctx, a, extra):
         mat.merbit_options(1)
         extra["merbit_workgroup_tiles_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, 30, 5)
         mat.merbit_options(pol)
-        extra["csr_vector_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_VECTOR, 20, 3)
-        extra["csr_scalar_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_SCALAR, 3, 1)
+        extra["csr_vector_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_VECTOR, 30, 5)
+        extra["csr_scalar_ms"], _ = time_spmv(ctx, mr, mat, xp, yd, mr.CSR_SCALAR, 30, 5)
+        for key in ("csr_vector", "csr_scalar"):
+            extra[f"{key}_gbs"] = csr_bytes(n, n, nnz, 4) / (extra[f"{key}_ms"] * 1e-3) / 1e9
+        log(f"  CSR vector {extra['csr_vector_ms'] * 1e3:.0f} us ({extra['csr_vector_gbs']:.0f} GB/s), "
+            f"scalar {extra['csr_scalar_ms'] * 1e3:.0f} us ({extra['csr_scalar_gbs']:.0f} GB/s)")
         # lower-precision values of the same reordered matrix vs the FP64 product
         m64 = mat.convert(mr.F64)
         x64 = ctx.upload(np.ascontiguousarray(xp.to_host().astype(np.float64)))

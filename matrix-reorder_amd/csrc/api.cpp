@@ -222,6 +222,34 @@ static int check_mat(mr_mat m)
     return use_device(m->ctx);
 }
 
+// the CSR kernels' row list, grouped by length class (once per structure)
+static int csr_bin_build(mr_mat m, hipStream_t st)
+{
+    DevBuf counts, offs, cursor;
+    MR_TRY(counts.alloc(sizeof(int32_t) * k::kCsrClasses));
+    MR_TRY(offs.alloc(sizeof(int32_t) * k::kCsrClasses));
+    MR_TRY(cursor.alloc(sizeof(int32_t) * k::kCsrClasses));
+    MR_TRY(m->csr_rows.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(m->n_rows, 1)));
+    MR_HIP_TRY(hipMemsetAsync(counts.p, 0, counts.bytes, st));
+    MR_HIP_TRY(hipMemsetAsync(cursor.p, 0, cursor.bytes, st));
+    MR_HIP_TRY(k::csr_bin_count(m->n_rows, m->rp.as<int32_t>(), counts.as<int32_t>(), st));
+    int32_t h[k::kCsrClasses], o[k::kCsrClasses];
+    MR_HIP_TRY(hipMemcpyAsync(h, counts.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    MR_HIP_TRY(hipStreamSynchronize(st));
+    int32_t run = 0;
+    for (int c = 0; c < k::kCsrClasses; ++c) {
+        o[c] = run;
+        run += h[c];
+    }
+    MR_HIP_TRY(hipMemcpyAsync(offs.p, o, sizeof(o), hipMemcpyHostToDevice, st));
+    MR_HIP_TRY(k::csr_bin_fill(m->n_rows, m->rp.as<int32_t>(), offs.as<int32_t>(), cursor.as<int32_t>(),
+                               m->csr_rows.as<int32_t>(), st));
+    MR_HIP_TRY(hipStreamSynchronize(st));   // o[] and the counters are released below
+    std::memcpy(m->csr_counts, h, sizeof(h));
+    m->csr_binned = true;
+    return MR_OK;
+}
+
 }  // namespace mr
 
 using namespace mr;
@@ -626,21 +654,9 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
     }
     case MR_KERNEL_CSR_SCALAR:
     case MR_KERNEL_CSR_VECTOR:
-        if (kernel == MR_KERNEL_CSR_VECTOR && m->n_long < 0) {
-            DevBuf cnt;
-            MR_TRY(cnt.alloc(sizeof(int32_t)));
-            MR_TRY(m->long_rows.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(m->n_rows, 1)));
-            MR_HIP_TRY(hipMemsetAsync(cnt.p, 0, sizeof(int32_t), st));
-            MR_HIP_TRY(k::csr_long_rows(m->n_rows, m->rp.as<int32_t>(), m->long_rows.as<int32_t>(),
-                                        cnt.as<int32_t>(), st));
-            int32_t nl = 0;
-            MR_HIP_TRY(hipMemcpyAsync(&nl, cnt.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            MR_HIP_TRY(hipStreamSynchronize(st));
-            m->n_long = nl;
-        }
-        MR_HIP_TRY(k::csr_spmv(m->dtype, kernel == MR_KERNEL_CSR_VECTOR, m->n_rows, m->rp.as<int32_t>(),
-                               m->ci.as<int32_t>(), m->val.p, x, y, m->avg_row, m->long_rows.as<int32_t>(),
-                               m->n_long > 0 ? m->n_long : 0, st));
+        if (!m->csr_binned) MR_TRY(csr_bin_build(m, st));
+        MR_HIP_TRY(k::csr_spmv(m->dtype, kernel == MR_KERNEL_CSR_VECTOR, m->csr_rows.as<int32_t>(), m->csr_counts,
+                               m->rp.as<int32_t>(), m->ci.as<int32_t>(), m->val.p, x, y, st));
         return MR_OK;
     default:
         return set_error(MR_ERR_INVALID_ARG, "unknown kernel %d", kernel);

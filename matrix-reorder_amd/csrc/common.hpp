@@ -158,8 +158,9 @@ struct mr_mat_s {
     std::vector<int32_t> perm;   // accumulated relabelling perm[old] = new
     MerbitPlan merbit;
     double avg_row = 0.0;
-    mr::DevBuf long_rows;    // rows above 4096 nonzeros (CSR vector kernel)
-    int32_t n_long = -1;     // -1: not computed for the current structure
+    mr::DevBuf csr_rows;     // rows grouped by length class (CSR kernels)
+    int32_t csr_counts[9] = {0};
+    bool csr_binned = false; // false: not built for the current structure
     // the columns with most nonzeros (ties: lower id), as many as an LDS x-cache
     // can hold, and their counts; hot_valid = computed for the current structure
     std::vector<int32_t> hot_order, hot_deg;

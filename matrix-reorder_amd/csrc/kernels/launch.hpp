@@ -24,12 +24,17 @@ hipError_t validate_csr(int64_t n_rows, int64_t n_cols, const int32_t* rp, const
                         int64_t nnz, int32_t* flag, hipStream_t s);
 
 // ---- CSR SpMV / SpMM (spmv_csr.hip) ----
-// rows longer than 4096 are listed once (count zeroed by the caller) and
-// multiplied by a workgroup each inside csr_spmv(vector=1)
-hipError_t csr_long_rows(int64_t n_rows, const int32_t* rp, int32_t* list, int32_t* count, hipStream_t s);
-hipError_t csr_spmv(int dt, int vector, int64_t n_rows, const int32_t* rp, const int32_t* ci,
-                    const void* val, const void* x, void* y, double avg_row, const int32_t* long_rows,
-                    int32_t n_long, hipStream_t s);
+// row-length classes of the CSR kernels: <= 1, <= 2, 4, 8, 16, 32, 64,
+// <= 4096, longer
+constexpr int kCsrClasses = 9;
+// per-class row counts (counts[kCsrClasses] zeroed by the caller)
+hipError_t csr_bin_count(int64_t n_rows, const int32_t* rp, int32_t* counts, hipStream_t s);
+// rows of class c to list[offs[c] ...] (cursor[kCsrClasses] zeroed by the caller)
+hipError_t csr_bin_fill(int64_t n_rows, const int32_t* rp, const int32_t* offs, int32_t* cursor, int32_t* list,
+                        hipStream_t s);
+// y = A x over the binned rows; counts are the host copy of the class counts
+hipError_t csr_spmv(int dt, int vector, const int32_t* list, const int32_t* counts, const int32_t* rp,
+                    const int32_t* ci, const void* val, const void* x, void* y, hipStream_t s);
 hipError_t csr_spmm(int dt, int64_t n_rows, int k, const int32_t* rp, const int32_t* ci,
                     const void* val, const void* X, int64_t ldx, void* Y, int64_t ldy,
                     hipStream_t s);

@@ -145,7 +145,7 @@ int relabel(mr_mat m, const std::vector<int32_t>& perm, double* ms_out)
     m->ci = std::move(d_newci);
     m->val = std::move(d_newval);
     for (auto& p : m->perm) p = perm[(size_t)p];
-    m->n_long = -1;   // row lengths moved: the long-row list is rebuilt on next use
+    m->csr_binned = false;   // row lengths moved: the CSR row bins are rebuilt on next use
     m->hot_valid = false;   // column ids moved: the hot columns are recounted on next use
     m->merbit.pn_built = false;   // and the column panels are rebuilt
     if (ms_out) *ms_out = ms_since(t0);
