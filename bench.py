@@ -349,8 +349,12 @@ def section_batched(mr, ctx, a, extra):
 
 
 def section_serving(mr, ctx, a, extra, batch_mats):
-    """32 resident BF16 matrices: R-MAT s22 and s21 ef32 reordered, plus 30 of
-    the batched set; Poisson loads with uploads of R-MAT s18 every 1000th."""
+    """32 resident BF16 matrices: R-MAT s22 and s21 ef32 reordered and tuned
+    (single vectors through their best MERBIT variant), plus 30 of the batched
+    set; Poisson loads with uploads of R-MAT s18 every 1000th.  Each rate runs
+    with no deadline (every request completes, latency grows with the queue)
+    and, at the overload rates, with a 2 ms queue deadline: requests queued
+    longer are shed (timed_out) and achieved_rate is the sustained capacity."""
     t = time.perf_counter()
     resident = []
     for scale, ef, seed in ((22, 16, 2), (21, 32, 21)):
@@ -358,22 +362,34 @@ def section_serving(mr, ctx, a, extra, batch_mats):
         val = mr.gen_values(500 + scale, len(ci))
         m = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.BF16)
         m.reorder_pgorder(0, 5)
+        m.merbit_autotune(3)
         resident.append(m)
     for m in batch_mats[:30]:
         resident.append(m.convert(mr.BF16))
-    srv = mr.Server(ctx, slots=4, max_jobs=256, coalesce=1)
-    for m in resident:
-        srv.add(m)
-    log(f"serving: 32 resident BF16 matrices ({time.perf_counter() - t:.1f} s setup)")
-    res = {}
-    for rate in a.serve_rates:
+    log(f"serving: 32 resident BF16 matrices ({time.perf_counter() - t:.1f} s setup), large ones on policies "
+        f"{[m.merbit_policy() for m in resident[:2]]}")
+
+    def run(srv, rate, tag):
         nreq = int(min(max(rate * a.serve_seconds, 2000), 60000))
         r = srv.run_load(rate=rate, n_requests=nreq, seed=int(rate), ks=(1, 4, 16), upload_every=a.serve_upload_every,
                          upload_scale=18, upload_edge_factor=16)
-        res[f"{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
-        log(f"  {rate:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, {r['gflops']:.0f} GFLOP/s, "
-            f"{r['completed']} done, {r['uploads_admitted']} uploads ({r['upload_ms_mean']:.0f} ms each), "
+        log(f"  {tag}{rate:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, {r['gflops']:.0f} GFLOP/s, "
+            f"{r['completed']} done, {r['timed_out']} shed, {r['achieved_rate']:.0f} done/s, "
+            f"{r['uploads_admitted']} uploads ({r['upload_ms_mean']:.0f} ms each), "
             f"{r['mean_requests_per_batch']:.1f} req/batch")
+        return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+
+    res = {}
+    srv = mr.Server(ctx, slots=4, max_jobs=256, coalesce=1)
+    for m in resident:
+        srv.add(m)
+    for rate in a.serve_rates:
+        res[f"{int(rate)}"] = run(srv, rate, "")
+    srv.close()
+    srv = mr.Server(ctx, slots=4, max_jobs=256, coalesce=1, timeout_ms=2.0)
+    for m in resident:
+        srv.add(m)
+    res["deadline_2ms"] = {f"{int(rate)}": run(srv, rate, "deadline 2 ms, ") for rate in a.serve_rates if rate >= 20000}
     # an upload of a matrix seen before: pGOrder replaced by the plan cache
     import tempfile
     with tempfile.TemporaryDirectory(prefix="mr_plans_") as plans:

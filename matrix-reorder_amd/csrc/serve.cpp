@@ -25,6 +25,7 @@
 
 #include "common.hpp"
 #include "kernels/launch.hpp"
+#include "kernels/launch_panel.hpp"
 #include "kernels/launch_serve.hpp"
 #include "parallel.hpp"
 
@@ -61,6 +62,7 @@ struct Job {
     mr_mat m = nullptr;
     int K = 0;
     bool lp = false;                  // runs through the lane-padded kernels
+    bool pn = false;                  // runs through the column-panel kernels
     int64_t ldx = 0;
     std::vector<int64_t> reqs;
     std::vector<int> off;             // first column of each request
@@ -110,11 +112,18 @@ int class_of(int K)
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
-// jobs on matrices with a lane-padded plan run through the lane-padded kernels
-// (one launch each) instead of the fused batch
+// single-vector jobs on matrices tuned to the column-panel variant run its two
+// kernels (one launch each, partial sums in the slot's arena)
+bool use_panel(const mr_mat m, int K)
+{
+    return K == 1 && (m->merbit.policy & 8) && m->dtype != MR_F64 && m->merbit.pn_built;
+}
+
+// other jobs on matrices with a lane-padded or column-panel plan run through
+// the lane-padded kernels (one launch each) instead of the fused batch
 bool use_lp(const mr_mat m, int K)
 {
-    return (m->merbit.policy & 4) && m->dtype != MR_F64 && m->merbit.xc_built && m->merbit.pcm_built &&
+    return (m->merbit.policy & 12) && m->dtype != MR_F64 && m->merbit.xc_built && m->merbit.pcm_built &&
            (K == 1 || k::merbit_lp_mm_fits(m->merbit.xs, m->merbit.xw, std::min(K, 16)));
 }
 
@@ -157,6 +166,12 @@ namespace {
 // device bytes of a job with K columns on matrix m
 size_t job_bytes(const mr_mat m, int K, int sigma, int64_t* tiles)
 {
+    if (use_panel(m, K)) {   // x, y and one float per partial sum
+        if (tiles) *tiles = 0;
+        return align256(sizeof(float) * (size_t)std::max<int64_t>(m->n_cols, 1)) +
+               align256(sizeof(float) * (size_t)std::max<int64_t>(m->n_rows, 1)) +
+               align256(sizeof(float) * (size_t)std::max<int64_t>(m->merbit.pparts, 1)) + align256(sizeof(int32_t));
+    }
     if (use_lp(m, K)) {
         const int64_t t = m->merbit.xtiles;
         if (tiles) *tiles = t;
@@ -268,7 +283,8 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
     size_t cur = s->table_bytes;
     for (Job& J : sl.jobs) {
         const mr_mat m = J.m;
-        J.lp = use_lp(m, J.K);
+        J.pn = use_panel(m, J.K);
+        J.lp = !J.pn && use_lp(m, J.K);
         J.ldx = J.lp ? lp_ldx(J.K) : J.K;
         const int G = J.lp ? (J.K == 1 ? 1 : 16) : carry_group(J.K);
         job_bytes(m, J.K, sigma, &J.tiles);
@@ -276,8 +292,9 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
         cur += align256(vb * (size_t)std::max<int64_t>(m->n_cols, 1) * (size_t)J.ldx);
         J.y_off = cur;
         cur += align256(vb * (size_t)std::max<int64_t>(m->n_rows, 1) * (size_t)J.K);
-        J.c_off = cur;
-        cur += align256(vb * (size_t)G * (size_t)std::max<int64_t>(J.tiles, 1));
+        J.c_off = cur;   // carries, or the partial sums of a column-panel job
+        cur += J.pn ? align256(vb * (size_t)std::max<int64_t>(m->merbit.pparts, 1))
+                    : align256(vb * (size_t)G * (size_t)std::max<int64_t>(J.tiles, 1));
         J.r_off = cur;
         cur += align256(sizeof(int32_t) * (size_t)std::max<int64_t>(J.tiles, 1));
     }
@@ -289,7 +306,7 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
     ClassTab ct[kClasses];
     std::vector<std::vector<size_t>> members(kClasses);
     for (size_t j = 0; j < sl.jobs.size(); ++j)
-        if (!sl.jobs[j].lp) members[(size_t)class_of(sl.jobs[j].K)].push_back(j);
+        if (!sl.jobs[j].lp && !sl.jobs[j].pn) members[(size_t)class_of(sl.jobs[j].K)].push_back(j);
     for (int c = 0; c < kClasses; ++c) {
         if (members[(size_t)c].empty()) continue;
         ClassTab& T = ct[c];
@@ -382,8 +399,18 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
         else
             MR_HIP_TRY(k::merbit_batch_spmm(s->dtype, 1 << c, tab, beg, T.n, T.total, (int)grid, st));
     }
-    // large matrices: one lane-padded launch per job (SpMM in 16-column passes)
+    // large matrices: one launch per job -- the column-panel kernels for a
+    // single vector, else the lane-padded ones (SpMM in 16-column passes)
     for (const Job& J : sl.jobs) {
+        if (J.pn) {
+            const MerbitPlan& p = J.m->merbit;
+            MR_HIP_TRY(k::panel_spmv(J.m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
+                                     p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg,
+                                     dev + J.x_off, J.m->n_cols, J.m->n_rows, p.prs.as<int32_t>(),
+                                     p.pgidx.as<int32_t>(), p.pwrow.as<int32_t>(), p.pn_chunks, p.pchunk_max,
+                                     dev + J.c_off, dev + J.y_off, st));
+            continue;
+        }
         if (!J.lp) continue;
         const mr_mat m = J.m;
         const MerbitPlan& p = m->merbit;
@@ -796,7 +823,10 @@ int mr_serve_add(mr_server s, mr_mat m, int* id)
     if (m->n_rows > 0) {
         MR_TRY(merbit_coords_sigma(m, s->cfg.sigma, s->ctx->stream, &bx, &by, &nb));
         MR_TRY(merbit_mm_grid(m, s->ctx->stream));
-        if (m->merbit.policy & 4) MR_TRY(merbit_lp_mm_build(m, s->ctx->stream));
+        // a column-panel plan serves single vectors; its SpMM runs on the
+        // lane-padded tiles, as mr_spmm does
+        if ((m->merbit.policy & 8) && m->dtype != MR_F64) MR_TRY(merbit_panel_build(m, s->ctx->stream));
+        if (m->merbit.policy & 12) MR_TRY(merbit_lp_mm_build(m, s->ctx->stream));
         MR_HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     }
     std::lock_guard<std::mutex> g(s->mu);

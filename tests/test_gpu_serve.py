@@ -211,15 +211,18 @@ def test_server_argument_errors(ctx, mr):
     f32.close()
 
 
-def test_lane_padded_route_matches_library(ctx, mr):
-    """Matrices with the lane-padded plan (policy bit 2) are served through the
-    lane-padded SpMV / SpMM kernels; without coalescing every result equals
-    the library's mr_spmv / mr_spmm on the same matrix bit for bit."""
+@pytest.mark.parametrize("policy", [4, 8, 9])
+def test_tuned_routes_match_library(ctx, mr, policy):
+    """Matrices tuned to the lane-padded variant (policy bit 2) are served
+    through its SpMV / SpMM kernels, those tuned to the column-panel variant
+    (bit 3) through the panel kernels for single vectors and the lane-padded
+    SpMM for several; without coalescing every result equals the library's
+    mr_spmv / mr_spmm on the same matrix bit for bit."""
     rp, ci = rmat_csr(12, 16, seed=31)
     n = len(rp) - 1
     rng = np.random.default_rng(9)
     big = mr.Mat.from_csr(ctx, rp, ci, rng.uniform(-1, 1, len(ci)), n_cols=n, store=mr.BF16)
-    big.merbit_options(4)
+    big.merbit_options(policy)
     mats, host = resident(ctx, mr)
     srv = mr.Server(ctx, verify=1, coalesce=0, slots=2, max_jobs=8)
     bid = srv.add(big)
