@@ -631,10 +631,7 @@ int mr_spmv(mr_mat m, int kernel, const void* x, void* y, mr_stream s)
         if (!p.built) MR_TRY(merbit_build(m, st));
         if ((p.policy & 8) && m->dtype != MR_F64) {
             MR_TRY(merbit_panel_build(m, st));
-            MR_HIP_TRY(k::panel_spmv(m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
-                                     p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg, x,
-                                     m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
-                                     p.pwrow.as<int32_t>(), p.pn_chunks, p.pchunk_max, p.ppart.p, y, st));
+            MR_HIP_TRY(k::panel_spmv(m->dtype, panel_view(m), x, p.ppart.p, y, st));
             return MR_OK;
         }
         if ((p.policy & 4) && m->dtype != MR_F64) {
@@ -697,10 +694,7 @@ static int spmv_ws_impl(mr_mat m, const void* x, void* y, void* ws, hipStream_t 
     if (!p.built) MR_TRY(merbit_build(m, st));
     if (pn) {
         MR_TRY(merbit_panel_build(m, st));
-        MR_HIP_TRY(k::panel_spmv(m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
-                                 p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg, x,
-                                 m->n_cols, m->n_rows, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
-                                 p.pwrow.as<int32_t>(), p.pn_chunks, p.pchunk_max, ws, y, st));
+        MR_HIP_TRY(k::panel_spmv(m->dtype, panel_view(m), x, ws, y, st));
         return MR_OK;
     }
     if (lp) {

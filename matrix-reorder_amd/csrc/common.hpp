@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/mr.h"
+#include "kernels/launch_panel.hpp"
 
 namespace mr {
 
@@ -132,15 +133,15 @@ struct MerbitPlan {
     // column-panel variant (policy bit 3): x in LDS one panel of 32768 columns
     // at a time; tiles of 64 x ps items, pw waves per workgroup; pcol / pval =
     // panel-major streams, ptseg = first partial of every tile, pjobs / pwg =
-    // the workgroups' (panel, tile range) jobs, prs / pgidx = every row's
-    // partials in order, ppart = the partial sums (pparts floats)
+    // the workgroups' (panel, tile range) jobs, ppart = the partial sums
+    // (pparts floats); prsi = the row-sum stream (prtiles tiles of 512 items:
+    // each row's partial indices and an end marker), ptrow = first row of
+    // every row-sum tile; plrow = the pn_long rows too long for a tile, whose
+    // partials prs / pgidx list (released when there are none)
     int ps = 8, pw = 16;
-    int64_t ptiles = 0, pparts = 0, psegs = 0;
-    int pn_wg = 0, pn_jobs = 0;
-    // pwrow = pn_chunks + 1 row bounds of the partial-sum chunks (at most
-    // pchunk_max partials each)
-    int pn_chunks = 0, pchunk_max = 0;
-    mr::DevBuf pcol, pval, ptseg, pjobs, pwg, prs, pgidx, ppart, pwrow;
+    int64_t ptiles = 0, pparts = 0, psegs = 0, prtiles = 0;
+    int pn_wg = 0, pn_jobs = 0, pn_long = 0;
+    mr::DevBuf pcol, pval, ptseg, pjobs, pwg, prs, pgidx, ppart, prsi, ptrow, plrow;
     bool pn_built = false;
     double pn_build_ms = 0.0;
     int max_lds = 0;
@@ -187,4 +188,6 @@ int merbit_lp_mm_build(mr_mat m, hipStream_t s);
 int merbit_lp_mm_rows(mr_mat m, int kk);
 // plan of the column-panel variant, built on first use (panel_plan.cpp)
 int merbit_panel_build(mr_mat m, hipStream_t s);
+// the device arrays of m's built column-panel plan
+k::PanelView panel_view(mr_mat m);
 }  // namespace mr

@@ -11,21 +11,40 @@ namespace k {
 constexpr int kPanelBits = 15;
 constexpr int kPanelCols = 1 << kPanelBits;   // columns per panel: 128 KB of F32 x in LDS
 
+// row-sum stream: tiles of 64 lanes x 8 items; an item is a partial index
+// (>= 0), the end of a row, or padding
+constexpr int kRowSumTile = 512;
+constexpr int32_t kRowEnd = -1;
+constexpr int32_t kRowPad = -2;
+
+// the device arrays of a built panel plan, as the launcher reads them
+struct PanelView {
+    int S = 8, W = 16, nt = 0;        // items per lane, waves per workgroup, non-temporal streams
+    const uint16_t* pcol = nullptr;   // panel-major columns
+    const void* pval = nullptr;       // panel-major values
+    const int32_t* tseg = nullptr;    // first partial of every tile
+    const int32_t* jobs = nullptr;    // (panel, first tile, end tile) per job
+    const int32_t* wg = nullptr;      // first job of every workgroup, plus the end
+    int n_wg = 0;
+    int64_t n_cols = 0, n_rows = 0;
+    const int32_t* items = nullptr;   // row-sum stream
+    const int32_t* trow = nullptr;    // first row of every row-sum tile, plus n_rows
+    int64_t n_rtiles = 0;
+    const int32_t* lrow = nullptr;    // rows too long for a row-sum tile
+    int n_long = 0;
+    const int32_t* rs = nullptr;      // their partials: gidx[rs[r] .. rs[r + 1])
+    const int32_t* gidx = nullptr;
+};
+
 // LDS bytes of a workgroup: one panel of x plus every wave's partial staging
 size_t panel_lds_bytes(int items_per_lane, int waves);
-// y = A x over the panel plan; part holds one float per partial.  wrow
-// (n_chunks + 1 rows) cuts the rows for the partial sums; chunks of at most
-// chunk_max partials are added through LDS (chunk_max * 4 bytes <=
-// panel_reduce_lds_cap()), else straight from global memory
-hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const void* pval, const int32_t* tseg,
-                      const int32_t* jobs, const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
-                      const int32_t* rs, const int32_t* gidx, const int32_t* wrow, int n_chunks, int chunk_max,
-                      void* part, void* y, hipStream_t s);
-// LDS bytes the partial-sum kernel may use for one chunk
-size_t panel_reduce_lds_cap();
-// partial-balanced row chunks: wrow[0..n_chunks], *maxc = max chunk (atomicMax
-// into a caller-zeroed int)
-hipError_t panel_chunks(int64_t n_rows, const int32_t* rs, int n_chunks, int32_t* wrow, int32_t* maxc, hipStream_t s);
+// y = A x over the panel plan; part holds one float per partial
+hipError_t panel_spmv(int dt, const PanelView& v, const void* x, void* part, void* y, hipStream_t s);
+// the row-sum stream: items[n_items] all padding, then every row r at its
+// packed position rpos[r]: its partial indices gidx[rs[r] .. rs[r + 1]) and
+// an end marker (a bare end marker for a row longer than a tile)
+hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs, const int32_t* gidx,
+                              const int32_t* rpos, int32_t* items, hipStream_t s);
 
 // plan build steps (see panel_plan.cpp for the order)
 hipError_t panel_seg_flags(int64_t n_rows, const int32_t* rp, const int32_t* ci, int32_t* F, hipStream_t s);

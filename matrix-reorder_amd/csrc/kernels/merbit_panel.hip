@@ -21,8 +21,13 @@
 //    and a segment still open at a tile end is closed there, so no sum ever
 //    crosses a tile: there are no carries and no fix-up pass;
 //  * each closed segment writes one partial sum; tseg[t] = index of tile t's
-//    first partial.  A second kernel adds each row's partials, listed row by
-//    row in panel order (rs / gidx), in a fixed order: y is bit-reproducible.
+//    first partial.  A second kernel adds each row's partials in panel order,
+//    so y is bit-reproducible.  Its input is a stream too: every row's
+//    partial indices followed by a row-end marker, packed into wave tiles of
+//    64 lanes x 8 items that no row crosses (padding fills a tile's tail), so
+//    a lane issues eight independent partial loads and no sum leaves its
+//    tile.  Rows with more partials than a tile holds are added by a third,
+//    one-workgroup-per-row kernel.
 //
 // The plan (streams, partial lists and the workgroups' job table) is built on
 // the device by the kernels at the end of this file; the host only lays the
@@ -228,121 +233,175 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
     }
 }
 
-// y[r] = sum of row r's partials, in the order listed.  The rows are cut into
-// gridDim.x contiguous chunks of about equal partial counts (wrow); the
-// workgroups are dispatched round-robin over the 8 XCDs and chunk
-// (b % 8) * (gridDim.x / 8) + b / 8 keeps each XCD on one contiguous eighth of
-// the rows, so the partial lines those rows share are fetched into one L2.  A
-// workgroup gathers its chunk's partials into LDS (coalesced index reads,
-// four gathers in flight per thread), then one thread per row adds its run.
-__global__ __launch_bounds__(256) void panel_reduce_kernel(const int32_t* __restrict__ rs,
-                                                           const int32_t* __restrict__ gidx,
-                                                           const float* __restrict__ part,
-                                                           const int32_t* __restrict__ wrow, float* __restrict__ y)
+// y[r] = sum of row r's partials.  Tile t of the row-sum stream holds rows
+// trow[t] .. trow[t + 1] - 1: each row's partial indices (>= 0), then its
+// end marker (kRowEnd); kRowPad fills the tile's tail.  Lane l owns the 8
+// consecutive items 8l .. 8l + 7, stored as two 16-byte chunks at chunk rows
+// 0 and 1 (c * 64 + l), so each load instruction of a wave reads 1 KB of
+// consecutive bytes.
+//
+// A wave takes two consecutive tiles: both item loads, then all sixteen
+// partial gathers, then the two sums (one tile per wave left the gathers
+// latency-bound; three or four tiles per wave ran slower at R-MAT s22, and so
+// did persistent waves, whose next-tile loads and y stores share the in-order
+// memory counter the gathers wait on).  The blocks are dispatched round-robin
+// over the 8 XCDs; block b takes tile group (b % 8) * (gridDim.x / 8) + b / 8,
+// which keeps each XCD on one contiguous eighth of the rows, so the partial
+// lines those rows share are fetched into one L2.
+__device__ __forceinline__ void rowsum_load(const int32_t* __restrict__ items, int64_t t, int lane, u32x4* a, u32x4* b)
 {
-    extern __shared__ float sp[];
-    const int per_xcd = gridDim.x / 8;
-    const int c = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    const int32_t r0 = wrow[c], r1 = wrow[c + 1];
-    const int32_t j0 = rs[r0], j1 = rs[r1];
-    int32_t j = j0 + (int32_t)threadIdx.x;
-    for (; j + 3 * 256 < j1; j += 4 * 256) {
-        const int32_t g0 = gidx[j], g1 = gidx[j + 256], g2 = gidx[j + 512], g3 = gidx[j + 768];
-        const float a0 = part[g0], a1 = part[g1], a2 = part[g2], a3 = part[g3];
-        sp[j - j0] = a0;
-        sp[j + 256 - j0] = a1;
-        sp[j + 512 - j0] = a2;
-        sp[j + 768 - j0] = a3;
-    }
-    for (; j < j1; j += 256) sp[j - j0] = part[gidx[j]];
-    __syncthreads();
-    for (int32_t r = r0 + (int32_t)threadIdx.x; r < r1; r += 256) {
-        const int32_t b = rs[r] - j0, e = rs[r + 1] - j0;
-        float acc = 0.0f;
-        for (int32_t q = b; q < e; ++q) acc += sp[q];
-        y[r] = acc;
-    }
+    const u32x4* ip = reinterpret_cast<const u32x4*>(items + t * kRowSumTile) + lane;
+    *a = __builtin_nontemporal_load(ip);
+    *b = __builtin_nontemporal_load(ip + 64);
 }
 
-// the same sum straight from global memory, G lanes per row (plans whose
-// largest chunk would not fit the LDS)
-template <int G>
-__global__ __launch_bounds__(256) void panel_reduce_direct_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
-                                                                  const int32_t* __restrict__ gidx,
-                                                                  const float* __restrict__ part, float* __restrict__ y)
+// the tile's eight partials per lane, each loaded at an in-bounds index
+// (markers read part[0]; the sum masks them), so the loads go out together
+__device__ __forceinline__ void rowsum_gather(const u32x4 a, const u32x4 b, const float* __restrict__ part, float* g)
 {
-    const int q = threadIdx.x & (G - 1);
-    const int64_t groups = (int64_t)gridDim.x * (blockDim.x / G);
-    // the G lanes of a group share r, so they leave the loop together
-    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < n_rows; r += groups) {
-        const int32_t b = rs[r], e = rs[r + 1];
-        float acc = 0.0f;
-        for (int32_t j = b + q; j < e; j += G) acc += part[gidx[j]];
+    const int32_t it[8] = {(int32_t)a.x, (int32_t)a.y, (int32_t)a.z, (int32_t)a.w,
+                           (int32_t)b.x, (int32_t)b.y, (int32_t)b.z, (int32_t)b.w};
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
-        if (q == 0) y[r] = acc;
+    for (int jj = 0; jj < 8; ++jj) {
+        const uint32_t neg = (uint32_t)(it[jj] >> 31);   // all ones for a marker
+        g[jj] = part[(uint32_t)it[jj] & ~neg];
     }
 }
 
-// chunks of about equal cost = partials + rows (empty rows still cost a
-// store): wrow[b] = first row r with rs[r] + r >= b * (total + n_rows) / n_chunks
-__global__ void panel_chunks_kernel(int64_t n_rows, const int32_t* __restrict__ rs, int n_chunks,
-                                    int32_t* __restrict__ wrow)
+__device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, const float* g, int32_t r0, int lane,
+                                              float* stg, float* __restrict__ y)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > n_chunks) return;
-    const int64_t total = (int64_t)rs[n_rows] + n_rows;
-    const int64_t target = (int64_t)b * total / n_chunks;
-    int64_t lo = 0, hi = n_rows;   // rs[r] + r is strictly increasing in r
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)rs[mid] + mid >= target) hi = mid;
-        else lo = mid + 1;
+    const int32_t it[8] = {(int32_t)a.x, (int32_t)a.y, (int32_t)a.z, (int32_t)a.w,
+                           (int32_t)b.x, (int32_t)b.y, (int32_t)b.z, (int32_t)b.w};
+    float p[8];
+    uint32_t ends = 0;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const uint32_t neg = (uint32_t)(it[jj] >> 31);
+        p[jj] = __uint_as_float(__float_as_uint(g[jj]) & ~neg);
+        ends |= (uint32_t)(it[jj] == kRowEnd) << jj;
     }
-    wrow[b] = b == n_chunks ? (int32_t)n_rows : (int32_t)lo;
+    // rank of this lane's first row in the tile
+    const int cnt = __popc(ends);
+    int inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    const int rk0 = inc - cnt;
+    int rk = rk0;
+    float acc = 0.0f, head = 0.0f;
+    bool has_end = false;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        acc += p[jj];
+        if ((ends >> jj) & 1u) {
+            if (!has_end) {
+                head = acc;
+                has_end = true;
+            } else {
+                stg[rk] = acc;
+            }
+            ++rk;
+            acc = 0.0f;
+        }
+    }
+    // segmented scan of (tail, has_end): the sum each lane's first row
+    // continues from the lanes before it.  No row crosses the tile, so the
+    // last lane's tail is zero.
+    float sv = acc;
+    int fl = has_end ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float vu = __shfl_up(sv, d, 64);
+        const int fu = __shfl_up(fl, d, 64);
+        if (lane >= d) {
+            if (!fl) sv += vu;
+            fl |= fu;
+        }
+    }
+    const float up = __shfl_up(sv, 1, 64);
+    if (has_end) stg[rk0] = (lane == 0 ? 0.0f : up) + head;
+    __builtin_amdgcn_wave_barrier();
+    // the tile's rows are consecutive: full-wave stores of y[r0 .. r0 + total)
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
+    for (int q = lane; q < total; q += 64) y[r0 + q] = stg[q];
+    // the next tile's staging writes come after every lane's reads above
+    __builtin_amdgcn_wave_barrier();
 }
 
-// *maxc = the largest chunk's partial count
-__global__ void panel_chunk_max_kernel(int n_chunks, const int32_t* __restrict__ rs, const int32_t* __restrict__ wrow,
-                                       int32_t* __restrict__ maxc)
+__global__ __launch_bounds__(256) void panel_rowsum_kernel(const int32_t* __restrict__ items,
+                                                           const int32_t* __restrict__ trow, int64_t n_tiles,
+                                                           const float* __restrict__ part, float* __restrict__ y)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < n_chunks) atomicMax(maxc, rs[wrow[b + 1]] - rs[wrow[b]]);
+    __shared__ float stg_all[4][kRowSumTile];
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int per_xcd = gridDim.x / 8;
+    const int64_t g = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int64_t t = (g * 4 + wid) * 2;
+    if (t >= n_tiles) return;   // the whole wave leaves; no block barrier follows
+    const bool two = t + 1 < n_tiles;
+    u32x4 a0, b0, a1, b1;
+    rowsum_load(items, t, lane, &a0, &b0);
+    rowsum_load(items, two ? t + 1 : t, lane, &a1, &b1);   // in bounds either way
+    float g0[8], g1[8];
+    rowsum_gather(a0, b0, part, g0);
+    rowsum_gather(a1, b1, part, g1);
+    rowsum_reduce(a0, b0, g0, trow[t], lane, stg_all[wid], y);
+    if (two) rowsum_reduce(a1, b1, g1, trow[t + 1], lane, stg_all[wid], y);
+}
+
+// rows with more partials than a row-sum tile holds (their tile entry is a
+// bare end marker, so the row-sum kernel wrote 0 there first): one workgroup
+// per row, a fixed-order sum (lane-strided, xor tree in the wave, the four
+// waves added in order)
+__global__ __launch_bounds__(256) void panel_long_rows_kernel(const int32_t* __restrict__ lrow,
+                                                              const int32_t* __restrict__ rs,
+                                                              const int32_t* __restrict__ gidx,
+                                                              const float* __restrict__ part, float* __restrict__ y)
+{
+    __shared__ float ws[4];
+    const int32_t r = lrow[blockIdx.x];
+    const int32_t b = rs[r], e = rs[r + 1];
+    float acc = 0.0f;
+    for (int32_t j = b + (int32_t)threadIdx.x; j < e; j += 256) acc += part[gidx[j]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) y[r] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 template <typename VT, int S, int W, bool NT>
-hipError_t launch_panel(const uint16_t* pcol, const void* pval, const int32_t* tseg, const int32_t* jobs,
-                        const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
-                        const int32_t* rs, const int32_t* gidx, const int32_t* wrow, int n_chunks, int chunk_max,
-                        void* part, void* y, hipStream_t s)
+hipError_t launch_panel(const PanelView& v, const void* x, void* part, void* y, hipStream_t s)
 {
     const size_t lds = panel_lds_bytes(S, W);
     auto kern = panel_spmv_kernel<VT, S, W, NT>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    if (n_wg > 0) {
-        kern<<<(unsigned)n_wg, 64 * W, lds, s>>>(pcol, static_cast<const VT*>(pval), tseg, jobs, wg,
-                                                 static_cast<const float*>(x), n_cols, static_cast<float*>(part));
+    if (v.n_wg > 0) {
+        kern<<<(unsigned)v.n_wg, 64 * W, lds, s>>>(v.pcol, static_cast<const VT*>(v.pval), v.tseg, v.jobs, v.wg,
+                                                   static_cast<const float*>(x), v.n_cols, static_cast<float*>(part));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (n_chunks > 0 && (size_t)chunk_max * sizeof(float) <= panel_reduce_lds_cap()) {
-        const size_t rl = sizeof(float) * (size_t)(chunk_max > 0 ? chunk_max : 1);
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(panel_reduce_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)rl);
+    if (v.n_rtiles > 0) {
+        // two tiles per wave, four waves per block, a multiple of 8 blocks
+        int64_t g = ((v.n_rtiles + 1) / 2 + 3) / 4;
+        g = (g + 7) / 8 * 8;
+        panel_rowsum_kernel<<<(unsigned)g, 256, 0, s>>>(v.items, v.trow, v.n_rtiles, static_cast<const float*>(part),
+                                                        static_cast<float*>(y));
+        e = hipGetLastError();
         if (e != hipSuccess) return e;
-        panel_reduce_kernel<<<(unsigned)n_chunks, 256, rl, s>>>(rs, gidx, static_cast<const float*>(part), wrow,
-                                                               static_cast<float*>(y));
-        return hipGetLastError();
     }
-    constexpr int G = 8;
-    int64_t g = (n_rows * G + 255) / 256;
-    if (g > 16384) g = 16384;
-    if (g < 1) g = 1;
-    panel_reduce_direct_kernel<G><<<(unsigned)g, 256, 0, s>>>(n_rows, rs, gidx, static_cast<const float*>(part),
-                                                              static_cast<float*>(y));
-    return hipGetLastError();
+    if (v.n_long > 0) {
+        panel_long_rows_kernel<<<(unsigned)v.n_long, 256, 0, s>>>(v.lrow, v.rs, v.gidx, static_cast<const float*>(part),
+                                                                  static_cast<float*>(y));
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 // ---- plan build --------------------------------------------------------------
@@ -493,6 +552,38 @@ __global__ void pn_gidx_kernel(int64_t n_seg, int tile, const int32_t* __restric
     }
 }
 
+// the row-sum stream: pads everywhere, then every row's partial indices and
+// end marker at its packed position rpos[r] (logical item i of a tile
+// belongs to lane i / 8, slot i % 8; lane l's chunk c at chunk row c)
+__global__ void pn_rowsum_fill_kernel(int64_t n, int32_t* __restrict__ items)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        items[i] = kRowPad;
+}
+
+__device__ __forceinline__ int64_t rowsum_slot(int64_t q)
+{
+    const int64_t tb = q - q % kRowSumTile;
+    const int i = (int)(q - tb), l = i / 8, jj = i % 8;
+    return tb + (int64_t)((jj / 4) * 64 + l) * 4 + jj % 4;
+}
+
+__global__ void pn_rowsum_scatter_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
+                                         const int32_t* __restrict__ gidx, const int32_t* __restrict__ rpos,
+                                         int32_t* __restrict__ items)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = rs[r], len = rs[r + 1] - b;
+        const int64_t q = rpos[r];
+        if (len + 1 > kRowSumTile) {   // a long row: its own kernel adds it
+            items[rowsum_slot(q)] = kRowEnd;
+            continue;
+        }
+        for (int32_t i = 0; i < len; ++i) items[rowsum_slot(q + i)] = gidx[b + i];
+        items[rowsum_slot(q + len)] = kRowEnd;
+    }
+}
+
 }  // namespace
 
 hipError_t panel_seg_flags(int64_t n_rows, const int32_t* rp, const int32_t* ci, int32_t* F, hipStream_t s)
@@ -591,30 +682,27 @@ hipError_t panel_gidx(int64_t n_seg, int tile, const int32_t* pos, const int32_t
     return hipGetLastError();
 }
 
-size_t panel_reduce_lds_cap() { return 64 * 1024; }
-
-hipError_t panel_chunks(int64_t n_rows, const int32_t* rs, int n_chunks, int32_t* wrow, int32_t* maxc, hipStream_t s)
+hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs, const int32_t* gidx,
+                              const int32_t* rpos, int32_t* items, hipStream_t s)
 {
-    if (n_chunks <= 0) return hipSuccess;
-    const unsigned g = (unsigned)((n_chunks + 1 + 255) / 256);
-    panel_chunks_kernel<<<g, 256, 0, s>>>(n_rows, rs, n_chunks, wrow);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    panel_chunk_max_kernel<<<(unsigned)((n_chunks + 255) / 256), 256, 0, s>>>(n_chunks, rs, wrow, maxc);
+    if (n_items > 0) {
+        pn_rowsum_fill_kernel<<<grid_of(n_items), 256, 0, s>>>(n_items, items);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n_rows <= 0) return hipSuccess;
+    pn_rowsum_scatter_kernel<<<grid_of(n_rows), 256, 0, s>>>(n_rows, rs, gidx, rpos, items);
     return hipGetLastError();
 }
 
 size_t panel_lds_bytes(int S, int W) { return sizeof(float) * ((size_t)kPanelCols + (size_t)W * 64 * S); }
 
-hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const void* pval, const int32_t* tseg,
-                      const int32_t* jobs, const int32_t* wg, int n_wg, const void* x, int64_t n_cols, int64_t n_rows,
-                      const int32_t* rs, const int32_t* gidx, const int32_t* wrow, int n_chunks, int chunk_max,
-                      void* part, void* y, hipStream_t s)
+hipError_t panel_spmv(int dt, const PanelView& v, const void* x, void* part, void* y, hipStream_t s)
 {
-    if (n_rows <= 0) return hipSuccess;
+    if (v.n_rows <= 0) return hipSuccess;
     if (dt == DT_F64) return hipErrorInvalidValue;
-    return dispatch_dt(dt, [&](auto v) -> hipError_t {
-        using VT = decltype(v);
+    return dispatch_dt(dt, [&](auto tag) -> hipError_t {
+        using VT = decltype(tag);
         if constexpr (sizeof(VT) == 8) {
             return hipErrorInvalidValue;
         } else {
@@ -624,17 +712,15 @@ hipError_t panel_spmv(int dt, int S, int W, int nt, const uint16_t* pcol, const 
                 if constexpr ((SS / 2) % 4 != 0 || (SS * sizeof(VT)) % 16 != 0) {
                     return hipErrorInvalidValue;
                 } else {
-                    return nt ? launch_panel<VT, SS, WW, true>(pcol, pval, tseg, jobs, wg, n_wg, x, n_cols, n_rows, rs,
-                                                               gidx, wrow, n_chunks, chunk_max, part, y, s)
-                              : launch_panel<VT, SS, WW, false>(pcol, pval, tseg, jobs, wg, n_wg, x, n_cols, n_rows,
-                                                                rs, gidx, wrow, n_chunks, chunk_max, part, y, s);
+                    return v.nt ? launch_panel<VT, SS, WW, true>(v, x, part, y, s)
+                                : launch_panel<VT, SS, WW, false>(v, x, part, y, s);
                 }
             };
             using I8 = std::integral_constant<int, 8>;
             using I16 = std::integral_constant<int, 16>;
-            if (S == 8 && W == 16) return go(I8{}, I16{});
-            if (S == 8 && W == 8) return go(I8{}, I8{});
-            if (S == 16 && W == 8) return go(I16{}, I8{});
+            if (v.S == 8 && v.W == 16) return go(I8{}, I16{});
+            if (v.S == 8 && v.W == 8) return go(I8{}, I8{});
+            if (v.S == 16 && v.W == 8) return go(I16{}, I8{});
             return hipErrorInvalidValue;
         }
     });

@@ -10,7 +10,10 @@
 //      workgroups' jobs (below);
 //   4. every nonzero is scattered to its item; items closing a segment or a
 //      tile are flagged, and the prefix sum of the flags numbers the partials;
-//   5. every row lists its partials in row-major segment order (rs / gidx).
+//   5. every row lists its partials in row-major segment order (rs / gidx);
+//   6. the row-sum stream: the host packs the rows into 512-item tiles that no
+//      row crosses, the device writes each row's partial indices and an end
+//      marker at its packed position.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -33,55 +36,61 @@ struct JobTable {
 };
 
 // Greedy split of the panels' tiles into at most `groups` contiguous job
-// lists of cost <= budget: a tile costs 1, loading a panel into LDS costs
-// `load` tiles.  Returns the number of workgroups used.
-int split_jobs(const std::vector<int32_t>& tile0, const std::vector<int32_t>& ntile, double budget, double load,
-               JobTable* out)
+// lists of cost <= budget.  C is the prefix sum of the tile costs (tile t
+// costs C[t + 1] - C[t]); loading a panel into LDS costs `load`.  Returns the
+// number of workgroups used.
+int split_jobs(const std::vector<int32_t>& tile0, const std::vector<int32_t>& ntile, const std::vector<double>& C,
+               double budget, double load, JobTable* out)
 {
     out->jobs.clear();
     out->wg.assign(1, 0);
     double rem = budget;
     bool open = false;   // the current workgroup has a job
     for (size_t p = 0; p < ntile.size(); ++p) {
-        int32_t t = tile0[p], left = ntile[p];
-        while (left > 0) {
-            if (open && rem < load + 1.0) {
+        int32_t t = tile0[p];
+        const int32_t te = tile0[p] + ntile[p];
+        while (t < te) {
+            if (open && rem < load + (C[(size_t)t + 1] - C[(size_t)t])) {
                 out->wg.push_back((int32_t)(out->jobs.size() / 3));
                 rem = budget;
                 open = false;
             }
-            const int32_t take = (int32_t)std::max<double>(1.0, std::min<double>((double)left, std::floor(rem - load)));
+            // the most tiles from t whose cost fits rem - load, at least one
+            const double lim = C[(size_t)t] + (rem - load);
+            int32_t e = (int32_t)(std::upper_bound(C.begin() + t + 1, C.begin() + te + 1, lim) - C.begin()) - 1;
+            if (e <= t) e = t + 1;
             out->jobs.push_back((int32_t)p);
             out->jobs.push_back(t);
-            out->jobs.push_back(t + take);
-            rem -= take + load;
+            out->jobs.push_back(e);
+            rem -= (C[(size_t)e] - C[(size_t)t]) + load;
             open = true;
-            t += take;
-            left -= take;
+            t = e;
         }
     }
     if (open) out->wg.push_back((int32_t)(out->jobs.size() / 3));
     return (int)out->wg.size() - 1;
 }
 
-JobTable plan_jobs(const std::vector<int32_t>& tile0, const std::vector<int32_t>& ntile, int groups, double load)
+JobTable plan_jobs(const std::vector<int32_t>& tile0, const std::vector<int32_t>& ntile, const std::vector<double>& C,
+                   int groups, double load)
 {
-    double total = 0.0;
+    double total = 0.0, tmax = 1.0;
     for (size_t p = 0; p < ntile.size(); ++p)
-        if (ntile[p] > 0) total += ntile[p] + load;
+        if (ntile[p] > 0) total += (C[(size_t)tile0[p] + ntile[p]] - C[(size_t)tile0[p]]) + load;
+    for (size_t t = 0; t + 1 < C.size(); ++t) tmax = std::max(tmax, C[t + 1] - C[t]);
     JobTable best;
     if (total <= 0.0) {
         best.wg.assign(1, 0);
         return best;
     }
     // smallest budget whose greedy split fits the workgroups
-    double lo = std::max(1.0, total / groups), hi = total + load + 1.0;
+    double lo = std::max(tmax + load, total / groups), hi = total + load + tmax;
     JobTable t;
-    if (split_jobs(tile0, ntile, lo, load, &t) <= groups) return t;
-    split_jobs(tile0, ntile, hi, load, &best);
+    if (split_jobs(tile0, ntile, C, lo, load, &t) <= groups) return t;
+    split_jobs(tile0, ntile, C, hi, load, &best);
     for (int it = 0; it < 40 && hi - lo > 0.5; ++it) {
         const double mid = 0.5 * (lo + hi);
-        if (split_jobs(tile0, ntile, mid, load, &t) <= groups) {
+        if (split_jobs(tile0, ntile, C, mid, load, &t) <= groups) {
             hi = mid;
             best = t;
         } else {
@@ -183,9 +192,6 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
         tiles += nt;
     }
     const int64_t n_items = tiles * tile;
-    const double tile_bytes = (double)tile * (2.0 + (double)dtype_size(m->dtype));
-    const double load = (double)(sizeof(float) * k::kPanelCols) / tile_bytes;
-    const JobTable jt = plan_jobs(tile0, ntile, std::max(1, m->ctx->cu_count), load);
     DevBuf d_padd, pos;
     MR_TRY(d_padd.alloc(sizeof(int32_t) * (size_t)P));
     MR_TRY(pos.alloc(sizeof(int32_t) * (size_t)ns1));
@@ -220,6 +226,23 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     MR_TRY(p.ptseg.alloc(sizeof(int32_t) * (size_t)(tiles + 1)));
     MR_HIP_TRY(k::panel_tile_base(tiles, tile, pidx.as<int32_t>(), p.ptseg.as<int32_t>(), s));
     MR_HIP_TRY(k::panel_set_tail(p.ptseg.as<int32_t>(), tiles, nullptr, n_parts, s));
+    // jobs, from per-tile costs.  The costs were fitted to per-workgroup clock
+    // stamps of the panel kernel at R-MAT s22 (FP32): 0.136 us per tile of
+    // 3 KB, 0.1 ns per partial sum the tile writes and 6.7 us per job (the x
+    // panel load and the pipeline ramp).  In bytes: a partial costs 2.25 B, a
+    // job 1.15 x the panel's x bytes.  Partials vary inside a panel: its first
+    // rows (the long ones after a locality order) write few, its last rows
+    // about one per item.
+    constexpr double kPartialBytes = 2.25, kJobLoad = 1.15;
+    const double tile_bytes = (double)tile * (2.0 + (double)dtype_size(m->dtype));
+    std::vector<int32_t> h_tseg((size_t)tiles + 1);
+    MR_HIP_TRY(hipMemcpyAsync(h_tseg.data(), p.ptseg.p, sizeof(int32_t) * h_tseg.size(), hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> C((size_t)tiles + 1, 0.0);
+    for (int64_t t = 0; t < tiles; ++t)
+        C[(size_t)t + 1] = C[(size_t)t] + 1.0 + kPartialBytes * (double)(h_tseg[(size_t)t + 1] - h_tseg[(size_t)t]) / tile_bytes;
+    const double load = kJobLoad * (double)(sizeof(float) * k::kPanelCols) / tile_bytes;
+    const JobTable jt = plan_jobs(tile0, ntile, C, std::max(1, m->ctx->cu_count), load);
     // 5. every row's partials, in row-major segment order
     DevBuf npieces, goff;
     MR_TRY(npieces.alloc(sizeof(int32_t) * (size_t)ns1));
@@ -247,18 +270,51 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
                                    s));
     MR_HIP_TRY(k::panel_gidx(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), goff.as<int32_t>(), pidx.as<int32_t>(),
                              p.pgidx.as<int32_t>(), s));
-    // row chunks of about 2048 partials + rows for the partial sums, a
-    // multiple of 8 (one contiguous eighth of the rows per XCD)
-    int64_t chunks = (n_parts + n + 2047) / 2048;
-    chunks = std::min<int64_t>(std::max<int64_t>(chunks, 8), 1 << 20);
-    chunks = (chunks + 7) / 8 * 8;
-    if (n <= 0) chunks = 0;
-    int32_t chunk_max = 0;
-    MR_TRY(p.pwrow.alloc(sizeof(int32_t) * (size_t)(chunks + 1)));
-    if (chunks > 0) {
-        MR_HIP_TRY(hipMemsetAsync(total.p, 0, sizeof(int32_t), s));
-        MR_HIP_TRY(k::panel_chunks(n, p.prs.as<int32_t>(), (int)chunks, p.pwrow.as<int32_t>(), total.as<int32_t>(), s));
-        MR_HIP_TRY(hipMemcpyAsync(&chunk_max, total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    // 6. the row-sum stream.  Rows are packed in order into 512-item tiles:
+    //    a row takes its partials plus an end marker and moves to the next
+    //    tile when it does not fit the current one; a row with more partials
+    //    than a tile holds takes a bare end marker and is listed as long.
+    std::vector<int32_t> hrs((size_t)n + 1), rpos((size_t)std::max<int64_t>(n, 1)), trow, lrow;
+    MR_HIP_TRY(hipMemcpyAsync(hrs.data(), p.prs.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    constexpr int64_t RT = k::kRowSumTile;
+    int64_t cur = 0, rt = 0;
+    if (n > 0) trow.push_back(0);
+    for (int64_t r = 0; r < n; ++r) {
+        int64_t L = (int64_t)hrs[(size_t)r + 1] - hrs[(size_t)r] + 1;
+        if (L > RT) {
+            lrow.push_back((int32_t)r);
+            L = 1;
+        }
+        if (cur + L > RT) {
+            ++rt;
+            cur = 0;
+            trow.push_back((int32_t)r);
+        }
+        rpos[(size_t)r] = (int32_t)(rt * RT + cur);
+        cur += L;
+        if (rt * RT + cur >= INT32_MAX)
+            return set_error(MR_ERR_UNSUPPORTED, "column panels: the row-sum stream exceeds the int32 item index");
+    }
+    const int64_t n_rtiles = n > 0 ? rt + 1 : 0;
+    trow.push_back((int32_t)n);
+    {
+        DevBuf d_rpos;
+        MR_TRY(d_rpos.alloc(sizeof(int32_t) * rpos.size()));
+        MR_TRY(p.prsi.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(n_rtiles * RT, 1)));
+        MR_TRY(p.ptrow.alloc(sizeof(int32_t) * trow.size()));
+        MR_HIP_TRY(hipMemcpyAsync(d_rpos.p, rpos.data(), sizeof(int32_t) * rpos.size(), hipMemcpyHostToDevice, s));
+        MR_HIP_TRY(hipMemcpyAsync(p.ptrow.p, trow.data(), sizeof(int32_t) * trow.size(), hipMemcpyHostToDevice, s));
+        MR_HIP_TRY(k::panel_rowsum_build(n, n_rtiles * RT, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
+                                         d_rpos.as<int32_t>(), p.prsi.as<int32_t>(), s));
+        MR_HIP_TRY(hipStreamSynchronize(s));   // d_rpos and the host vectors are freed on return
+    }
+    if (lrow.empty()) {
+        p.prs.release();
+        p.pgidx.release();
+    } else {
+        MR_TRY(p.plrow.alloc(sizeof(int32_t) * lrow.size()));
+        MR_HIP_TRY(hipMemcpyAsync(p.plrow.p, lrow.data(), sizeof(int32_t) * lrow.size(), hipMemcpyHostToDevice, s));
     }
     MR_TRY(p.pjobs.alloc(sizeof(int32_t) * std::max<size_t>(jt.jobs.size(), 3)));
     MR_TRY(p.pwg.alloc(sizeof(int32_t) * jt.wg.size()));
@@ -271,12 +327,41 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     p.psegs = n_seg;
     p.pn_wg = (int)jt.wg.size() - 1;
     p.pn_jobs = (int)(jt.jobs.size() / 3);
-    p.pn_chunks = (int)chunks;
-    p.pchunk_max = chunk_max;
+    p.prtiles = n_rtiles;
+    p.pn_long = (int)lrow.size();
     p.pn_built = true;
     p.pn_build_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return MR_OK;
+}
+
+}  // namespace mr
+
+namespace mr {
+
+k::PanelView panel_view(mr_mat m)
+{
+    const MerbitPlan& p = m->merbit;
+    k::PanelView v;
+    v.S = p.ps;
+    v.W = p.pw;
+    v.nt = p.policy & 1;
+    v.pcol = p.pcol.as<uint16_t>();
+    v.pval = p.pval.p;
+    v.tseg = p.ptseg.as<int32_t>();
+    v.jobs = p.pjobs.as<int32_t>();
+    v.wg = p.pwg.as<int32_t>();
+    v.n_wg = p.pn_wg;
+    v.n_cols = m->n_cols;
+    v.n_rows = m->n_rows;
+    v.items = p.prsi.as<int32_t>();
+    v.trow = p.ptrow.as<int32_t>();
+    v.n_rtiles = p.prtiles;
+    v.lrow = p.plrow.as<int32_t>();
+    v.n_long = p.pn_long;
+    v.rs = p.prs.as<int32_t>();
+    v.gidx = p.pgidx.as<int32_t>();
+    return v;
 }
 
 }  // namespace mr

@@ -403,12 +403,7 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
     // single vector, else the lane-padded ones (SpMM in 16-column passes)
     for (const Job& J : sl.jobs) {
         if (J.pn) {
-            const MerbitPlan& p = J.m->merbit;
-            MR_HIP_TRY(k::panel_spmv(J.m->dtype, p.ps, p.pw, p.policy & 1, p.pcol.as<uint16_t>(), p.pval.p,
-                                     p.ptseg.as<int32_t>(), p.pjobs.as<int32_t>(), p.pwg.as<int32_t>(), p.pn_wg,
-                                     dev + J.x_off, J.m->n_cols, J.m->n_rows, p.prs.as<int32_t>(),
-                                     p.pgidx.as<int32_t>(), p.pwrow.as<int32_t>(), p.pn_chunks, p.pchunk_max,
-                                     dev + J.c_off, dev + J.y_off, st));
+            MR_HIP_TRY(k::panel_spmv(J.m->dtype, panel_view(J.m), dev + J.x_off, dev + J.c_off, dev + J.y_off, st));
             continue;
         }
         if (!J.lp) continue;

@@ -32,6 +32,20 @@ def panel_cases():
     rows = [np.unique(np.concatenate([rng.integers(0, 200_000, 12), [199_999]])) for _ in range(500)]
     rp, ci = csr_rows(rows, 200_000)
     cases.append(("nonsquare_tail", rp, ci, 200_000))
+    # rows around the row-sum tile size (512 items: partials plus an end
+    # marker): one entry in each of 510, 511 (a tile exactly), 512 and 600
+    # panels, between short and empty rows
+    nc = 600 * PANEL
+    rows = []
+    for i in range(400):
+        if i in (7, 100, 101, 250, 399):
+            k = {7: 600, 100: 511, 101: 512, 250: 510, 399: 600}[i]
+            pan = np.sort(rng.choice(600, k, replace=False))
+            rows.append(pan * PANEL + rng.integers(0, PANEL, k))
+        else:
+            rows.append(np.sort(rng.choice(nc, int(rng.integers(0, 9)), replace=False)))
+    rp, ci = csr_rows(rows, nc)
+    cases.append(("rowsum_long_rows", rp, ci, nc))
     return cases
 
 
