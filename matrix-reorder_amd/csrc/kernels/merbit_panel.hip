@@ -159,8 +159,8 @@ __device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* v
     for (int q = lane; q < total; q += 64) part[pbase + q] = stg[q];
 }
 
-// jobs[3j .. 3j+2] = (panel, first tile, end tile); workgroup b runs jobs
-// [wg[b], wg[b+1]) in order.  part[tseg[t] + i] = i-th segment closed in tile t.
+// jobs[3j .. 3j+2] = (panel, first tile, end tile); job list i runs jobs
+// [wg[i], wg[i+1]) in order.  part[tseg[t] + i] = i-th segment closed in tile t.
 template <typename VT, int S, int W, bool NT>
 __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __restrict__ pcol,
                                                             const VT* __restrict__ pval,
@@ -180,8 +180,19 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
     const bool x16 = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
     float* stg = xs + kPanelCols + wid * (64 * S);   // this wave's partial staging
 
-    const int j1 = wg[blockIdx.x + 1];
-    for (int j = wg[blockIdx.x]; j < j1; ++j) {
+    // the next job's x panel, loaded into registers while this job's tiles
+    // run (full panels of a 16-byte aligned x): a job switch then writes LDS
+    // from registers instead of waiting for memory
+    constexpr int XN = kPanelCols / 4 / (64 * W);   // 16-byte vectors per thread
+    u32x4 xn[XN];
+    bool have_xn = false;
+    // job lists in order are spread so that each XCD runs a contiguous eighth
+    // of them (blocks go round-robin over the 8 XCDs; gridDim.x is a multiple
+    // of 8): neighbouring lists share a panel, whose second load then hits
+    // that XCD's L2
+    const int li = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+    const int j1 = wg[li + 1];
+    for (int j = wg[li]; j < j1; ++j) {
         const int32_t pn = jobs[3 * j], t0 = jobs[3 * j + 1], t1 = jobs[3 * j + 2];
         const int64_t c0 = (int64_t)pn * kPanelCols;
         const int nc = (int)((n_cols - c0) < kPanelCols ? (n_cols - c0) : kPanelCols);
@@ -201,7 +212,10 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
             load_tile<VT, S, NT>(pcol, pval, tseg, t + W < t1 ? t + W : tl, lane, cb, vb, &pb);
         }
         __syncthreads();   // the previous job's tiles are done with xs
-        if (x16) {
+        if (have_xn) {
+#pragma unroll
+            for (int q = 0; q < XN; ++q) reinterpret_cast<u32x4*>(xs)[threadIdx.x + q * 64 * W] = xn[q];
+        } else if (x16) {
             const float4* xv = reinterpret_cast<const float4*>(x + c0);
             for (int i = threadIdx.x; 4 * i < nc; i += 64 * W) {
                 if (4 * i + 3 < nc) {
@@ -214,6 +228,18 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
             for (int i = threadIdx.x; i < nc; i += 64 * W) xs[i] = x[c0 + i];
         }
         __syncthreads();
+        // issued after this job's first tile loads and before the rest: the
+        // first tile's wait covers these loads too, the later ones do not
+        have_xn = false;
+        if (j + 1 < j1) {
+            const int64_t c1 = (int64_t)jobs[3 * (j + 1)] * kPanelCols;
+            if (x16 && c1 + kPanelCols <= n_cols) {
+                const u32x4* xv = reinterpret_cast<const u32x4*>(x + c1);
+#pragma unroll
+                for (int q = 0; q < XN; ++q) xn[q] = xv[threadIdx.x + q * 64 * W];
+                have_xn = true;
+            }
+        }
 
         // this wave's tiles t + k W, k < n; the main loop runs whole rounds of
         // three with no exit inside, so each register set stays in place

@@ -227,13 +227,13 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     MR_HIP_TRY(k::panel_tile_base(tiles, tile, pidx.as<int32_t>(), p.ptseg.as<int32_t>(), s));
     MR_HIP_TRY(k::panel_set_tail(p.ptseg.as<int32_t>(), tiles, nullptr, n_parts, s));
     // jobs, from per-tile costs.  The costs were fitted to per-workgroup clock
-    // stamps of the panel kernel at R-MAT s22 (FP32): 0.136 us per tile of
-    // 3 KB, 0.1 ns per partial sum the tile writes and 6.7 us per job (the x
-    // panel load and the pipeline ramp).  In bytes: a partial costs 2.25 B, a
-    // job 1.15 x the panel's x bytes.  Partials vary inside a panel: its first
-    // rows (the long ones after a locality order) write few, its last rows
-    // about one per item.
-    constexpr double kPartialBytes = 2.25, kJobLoad = 1.15;
+    // stamps of the panel kernel at R-MAT s22 (FP32): 0.14 us per tile of
+    // 3 KB, 0.15 ns per partial sum the tile writes and 5.9 us per job (the x
+    // panel load of the first job and the pipeline ramp; later panels are
+    // prefetched into registers).  In bytes: a partial costs 3.3 B, a job the
+    // panel's x bytes.  Partials vary inside a panel: its first rows (the long
+    // ones after a locality order) write few, its last rows about one per item.
+    constexpr double kPartialBytes = 3.3, kJobLoad = 1.0;
     const double tile_bytes = (double)tile * (2.0 + (double)dtype_size(m->dtype));
     std::vector<int32_t> h_tseg((size_t)tiles + 1);
     MR_HIP_TRY(hipMemcpyAsync(h_tseg.data(), p.ptseg.p, sizeof(int32_t) * h_tseg.size(), hipMemcpyDeviceToHost, s));
@@ -242,7 +242,11 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     for (int64_t t = 0; t < tiles; ++t)
         C[(size_t)t + 1] = C[(size_t)t] + 1.0 + kPartialBytes * (double)(h_tseg[(size_t)t + 1] - h_tseg[(size_t)t]) / tile_bytes;
     const double load = kJobLoad * (double)(sizeof(float) * k::kPanelCols) / tile_bytes;
-    const JobTable jt = plan_jobs(tile0, ntile, C, std::max(1, m->ctx->cu_count), load);
+    JobTable jt = plan_jobs(tile0, ntile, C, std::max(1, m->ctx->cu_count), load);
+    // the kernel maps its blocks onto the job lists by XCD: a multiple of 8
+    // lists, the added ones empty
+    if (jt.wg.size() > 1)
+        while ((jt.wg.size() - 1) % 8 != 0) jt.wg.push_back(jt.wg.back());
     // 5. every row's partials, in row-major segment order
     DevBuf npieces, goff;
     MR_TRY(npieces.alloc(sizeof(int32_t) * (size_t)ns1));
