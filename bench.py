@@ -34,6 +34,7 @@ Prints ONE JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import math
 import os
@@ -51,6 +52,10 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 counter summaries of the headline kernels, by MERBIT variant
 TRAFFIC_PROFILES = {"panel": ROOT / "profiles" / "merbit_panel_s22_fp32_pmc.md",
                     "lane_padded": ROOT / "profiles" / "merbit_lp_s22_fp32_pmc.md"}
+_KERNELS = ROOT / "matrix-reorder_amd" / "csrc"
+# the sources each summary measured (their hash is recorded in the summary)
+TRAFFIC_SOURCES = {"panel": [_KERNELS / "kernels" / "merbit_panel.hip", _KERNELS / "panel_plan.cpp"],
+                   "lane_padded": [_KERNELS / "kernels" / "merbit_lp.hip"]}
 
 
 def log(*a):
@@ -97,18 +102,32 @@ def csr_bytes(n_rows, n_cols, nnz, val_bytes, vec_bytes=4, k=1):
     return nnz * (val_bytes + 4) + (n_rows + 1) * 4 + k * (n_cols + n_rows) * vec_bytes
 
 
+def kernel_source_hash(policy):
+    """First 16 hex digits of the SHA-256 over the sources of the MERBIT variant
+    `policy` selects (kernel and plan files, in a fixed order)."""
+    files = TRAFFIC_SOURCES["panel" if policy & 8 else "lane_padded"]
+    h = hashlib.sha256()
+    for f in files:
+        h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profile(policy):
     """HBM bytes per SpMV of the headline kernel(s) from the committed rocprofv3
     counter summary (FETCH_SIZE and WRITE_SIZE scaled by the copy kernel).
-    The summary names the MERBIT policy it was measured with; a run that picked
-    another policy gets None rather than a figure of other code."""
+    The summary names the MERBIT policy it was measured with and the hash of
+    the sources it measured; a run that picked another policy, or runs changed
+    sources, gets None rather than a figure of other code."""
     path = TRAFFIC_PROFILES["panel" if policy & 8 else "lane_padded"]
     if not path.exists():
         return None
     text = path.read_text()
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     pm = re.search(r"^policy:\s*(\d+)", text, re.M)
+    hm = re.search(r"^source_sha16:\s*([0-9a-f]+)", text, re.M)
     if not m or (pm and int(pm.group(1)) != policy):
+        return None
+    if not hm or hm.group(1) != kernel_source_hash(policy):
         return None
     return float(m.group(1))
 
