@@ -394,18 +394,23 @@ def section_serving(mr, ctx, a, extra, batch_mats):
                          upload_scale=18, upload_edge_factor=16)
         log(f"  {tag}{rate:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, {r['gflops']:.0f} GFLOP/s, "
             f"{r['completed']} done, {r['timed_out']} shed, {r['achieved_rate']:.0f} done/s, "
+            f"heavy {r['completed_heavy']} done / {r['timed_out_heavy']} shed, p99 light {r['p99_light_us']:.0f} us, "
+            f"heavy {r['p99_heavy_us']:.0f} us, "
             f"{r['uploads_admitted']} uploads ({r['upload_ms_mean']:.0f} ms each), "
             f"{r['mean_requests_per_batch']:.1f} req/batch")
         return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
 
     res = {}
-    srv = mr.Server(ctx, slots=4, max_jobs=256, coalesce=1)
+    cfg = {"slots": a.serve_slots, "heavy_slots": a.serve_heavy_slots, "max_jobs": 256, "coalesce": 1,
+           "upload_threads": a.serve_upload_threads}
+    res["config"] = dict(cfg)
+    srv = mr.Server(ctx, **cfg)
     for m in resident:
         srv.add(m)
     for rate in a.serve_rates:
         res[f"{int(rate)}"] = run(srv, rate, "")
     srv.close()
-    srv = mr.Server(ctx, slots=4, max_jobs=256, coalesce=1, timeout_ms=2.0)
+    srv = mr.Server(ctx, timeout_ms=2.0, **cfg)
     for m in resident:
         srv.add(m)
     res["deadline_2ms"] = {f"{int(rate)}": run(srv, rate, "deadline 2 ms, ") for rate in a.serve_rates if rate >= 20000}
@@ -528,6 +533,9 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5000.0, 20000.0, 50000.0])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-upload-every", type=int, default=1000)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-heavy-slots", type=int, default=1)
+    ap.add_argument("--serve-upload-threads", type=int, default=0, help="host threads of an upload's pGOrder (0 = half)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (this build has no multi-GPU path; "

@@ -418,11 +418,13 @@ typedef struct {
     int sigma;              /* MERBIT tile width of single-vector jobs (8)          */
     int grid;               /* persistent workgroups per launch (0 = one per tile)  */
     int upload_threads;     /* host threads of a background reorder (0 = half)      */
-    int pad;
+    int heavy_slots;        /* slots heavy batches may hold at once (0 = 1)         */
     int64_t slot_bytes;     /* device arena per slot (1 GiB)                        */
     int64_t hbm_budget;     /* bytes uploaded matrices may hold (0 = free HBM only) */
     int64_t hbm_reserve;    /* device bytes left free when admitting uploads (4 GiB) */
-    double timeout_ms;      /* queued longer -> MR_ERR_TIMEOUT (0 = never)          */
+    double timeout_ms;      /* deadline after arrival (0 = none): a request queued
+                               past it, or a heavy one predicted to finish past it
+                               (learned service rates), ends MR_ERR_TIMEOUT       */
 } mr_serve_config;
 typedef struct {
     double rate;            /* Poisson arrivals per second                          */
@@ -444,6 +446,12 @@ typedef struct {
     double offered_rate, achieved_rate;
     double batches, mean_jobs_per_batch, mean_requests_per_batch;
     double upload_ms_mean;  /* generate + admit + pGOrder per admitted upload       */
+    /* requests streaming >= 32 MiB (matrix, X and Y) are heavy: light ones are
+     * dispatched first, heavy batches hold at most heavy_slots of the slots
+     * (and never the last free one), and with a deadline a heavy request
+     * predicted to finish late is shed */
+    int64_t completed_heavy, timed_out_heavy;
+    double p99_light_us, p99_heavy_us;
 } mr_serve_report;
 int mr_serve_config_default(mr_serve_config* c);
 int mr_serve_create(mr_ctx ctx, const mr_serve_config* c, mr_server* out);

@@ -35,6 +35,12 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr int kClasses = 7;   // K = 1 (SpMV), then carry widths G = 2, 4, ..., 64
+// requests that stream at least this many bytes (matrix, X and Y) are heavy:
+// they queue apart, and light requests are dispatched first
+constexpr double kHeavyBytes = 32.0 * 1024 * 1024;
+// effective rates of heavy single-vector and multi-vector jobs before any
+// heavy batch has completed (bytes per us); then learned from batch times
+constexpr double kHeavyRate0[2] = {4.0e6, 1.0e6};
 
 struct SMat {
     mr_mat m = nullptr;
@@ -52,6 +58,7 @@ struct Req {
     int status = -1;                    // -1 while queued or running
     uint64_t hash = 0;
     double flops = 0.0, bytes = 0.0;
+    bool heavy = false;                 // bytes >= kHeavyBytes
     std::vector<float> y;               // verify level 2
 };
 
@@ -78,6 +85,10 @@ struct Slot {
     DevBuf arena;                 // [tables | hashes | job data]
     void* pinned = nullptr;       // [tables staging | hash readback]
     bool busy = false;
+    bool heavy = false;           // the batch holds heavy requests
+    double t_launch = 0.0;        // us since the server epoch
+    double est_us = 0.0;          // predicted service time (heavy batches)
+    int kinds = 0;                // bit 0: a single-vector job, bit 1: a multi-vector job
     std::vector<Job> jobs;
     std::vector<int64_t> reqs;    // request-table order
 };
@@ -142,8 +153,12 @@ struct mr_server_s {
     std::condition_variable up_cv;     // upload thread: task arrived / stop
     std::vector<SMat> mats;
     std::deque<Req> reqs;              // index = request id
-    std::deque<int64_t> queue;
+    std::deque<int64_t> queue;         // light requests
+    std::deque<int64_t> hqueue;        // heavy requests
     std::vector<Slot> slots;
+    int heavy_busy = 0;                // slots running a heavy batch
+    // bytes per us of heavy single-vector [0] and multi-vector [1] jobs
+    double heavy_rate[2] = {kHeavyRate0[0], kHeavyRate0[1]};
     std::deque<UploadTask> up_tasks;
     std::vector<UploadRes> up_res;
     std::thread dispatcher, uploader;
@@ -207,26 +222,38 @@ void finish_req(mr_server s, int64_t r, int status, double t)
     s->done_cv.notify_all();
 }
 
-// pop requests from the queue into jobs that fit the slot's arena (lock held)
-void form_batch(mr_server s, Slot& sl)
+// pop requests from the light or the heavy queue into jobs that fit the
+// slot's arena (lock held).  With a deadline (timeout_ms > 0):
+//  * a request queued past its deadline is shed (MR_ERR_TIMEOUT);
+//  * a heavy batch is predicted to take the bytes each job streams over the
+//    learned rate of its kind (one vector or several); a heavy request that
+//    would finish late even in a batch of its own is shed, and the batch stops
+//    growing before a request would make any of its members finish late (the
+//    rest wait for the next batch).
+void form_batch(mr_server s, Slot& sl, bool heavy)
 {
+    std::deque<int64_t>& q = heavy ? s->hqueue : s->queue;
     const double now = s->now_us();
+    const double tmo = s->cfg.timeout_ms * 1000.0;
     const size_t cap = sl.arena.bytes - s->table_bytes;
     size_t used = 0;
     int n_reqs = 0;
+    double est = 0.0;                         // predicted service time so far (us)
+    int kinds = 0;
+    double first_deadline = 1e300;            // earliest deadline in the batch
     std::unordered_map<int, size_t> open;   // matrix -> job still below 64 columns
     sl.jobs.clear();
     sl.reqs.clear();
-    while (!s->queue.empty() && n_reqs < s->cfg.max_jobs) {
-        const int64_t r = s->queue.front();
+    while (!q.empty() && n_reqs < s->cfg.max_jobs) {
+        const int64_t r = q.front();
         Req& R = s->reqs[(size_t)r];
-        if (s->cfg.timeout_ms > 0 && now - R.t_arr > s->cfg.timeout_ms * 1000.0) {
-            s->queue.pop_front();
+        if (tmo > 0 && now - R.t_arr > tmo) {
+            q.pop_front();
             finish_req(s, r, MR_ERR_TIMEOUT, now);
             continue;
         }
         if (R.status >= 0) {   // cancelled while queued
-            s->queue.pop_front();
+            q.pop_front();
             continue;
         }
         const mr_mat m = s->mats[(size_t)R.mat].m;
@@ -234,6 +261,24 @@ void form_batch(mr_server s, Slot& sl)
         if (s->cfg.coalesce) {
             auto it = open.find(R.mat);
             if (it != open.end() && sl.jobs[it->second].K + R.k <= 64) j = it->second;
+        }
+        // time the request adds: the whole request on a new job, only its
+        // vectors when it joins a job on the same matrix (which then runs as a
+        // multi-vector job)
+        const int kind = (j == SIZE_MAX && R.k == 1) ? 0 : 1;
+        const double add = (j == SIZE_MAX ? R.bytes : 4.0 * R.k * (double)(m->n_cols + m->n_rows)) /
+                           s->heavy_rate[kind];
+        if (heavy && tmo > 0) {
+            const double done_at = now + est + add;
+            const double dl = R.t_arr + tmo;
+            if (done_at > dl || done_at > first_deadline) {
+                if (sl.jobs.empty()) {   // late even on its own
+                    q.pop_front();
+                    finish_req(s, r, MR_ERR_TIMEOUT, now);
+                    continue;
+                }
+                break;
+            }
         }
         if (j != SIZE_MAX) {
             const int K = sl.jobs[j].K;
@@ -245,14 +290,17 @@ void form_batch(mr_server s, Slot& sl)
         }
         if (used + need > cap) {
             if (sl.jobs.empty()) {   // can never fit: refuse it
-                s->queue.pop_front();
+                q.pop_front();
                 finish_req(s, r, MR_ERR_ADMISSION, now);
                 continue;
             }
             break;
         }
-        s->queue.pop_front();
+        q.pop_front();
         used += need;
+        est += add;
+        kinds |= 1 << kind;
+        if (tmo > 0) first_deadline = std::min(first_deadline, R.t_arr + tmo);
         if (j == SIZE_MAX) {
             Job J;
             J.mat = R.mat;
@@ -271,6 +319,9 @@ void form_batch(mr_server s, Slot& sl)
         s->mats[(size_t)R.mat].last_use = ++s->stamp;
         ++n_reqs;
     }
+    sl.heavy = heavy;
+    sl.est_us = est;
+    sl.kinds = kinds;
 }
 
 // write the tables, copy them, and launch the batch on the slot's stream
@@ -409,7 +460,11 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
         if (!J.lp) continue;
         const mr_mat m = J.m;
         const MerbitPlan& p = m->merbit;
-        const int grid = std::max(s->ctx->cu_count, 1);
+        // persistent launches of one workgroup per CU would hold every CU for
+        // the whole pass (up to ~0.8 ms at R-MAT s22, k = 16): with other slots
+        // an eighth of the CUs is left to the light batches running beside them
+        const int cus = std::max(s->ctx->cu_count, 1);
+        const int grid = s->slots.size() > 1 ? std::max(1, cus - cus / 8) : cus;
         if (J.K == 1) {
             MR_HIP_TRY(k::merbit_lp_spmv(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles, p.pc.as<int32_t>(),
                                          p.pv.p, p.xty.as<int32_t>(), dev + J.x_off, p.hot.as<int32_t>(), p.xc,
@@ -478,7 +533,20 @@ void retire_slot(mr_server s, Slot& sl, std::unique_lock<std::mutex>& lk)
     ++s->batches;
     s->batch_jobs += (int64_t)sl.jobs.size();
     s->batch_reqs += i;
+    if (sl.heavy) {
+        // launch -> retire of a heavy batch, sharing the device with the other
+        // slots as it did, against its prediction: the rates of the kinds it
+        // ran move a fifth of the way (in log space) toward what it measured
+        const double dt = t - sl.t_launch;
+        if (dt > 0.0 && sl.est_us > 0.0) {
+            const double f = std::pow(sl.est_us / dt, 0.2);
+            for (int kd = 0; kd < 2; ++kd)
+                if (sl.kinds & (1 << kd)) s->heavy_rate[kd] = std::min(1.0e8, std::max(1.0e4, s->heavy_rate[kd] * f));
+        }
+        --s->heavy_busy;
+    }
     sl.busy = false;
+    sl.heavy = false;
     sl.jobs.clear();
     sl.reqs.clear();
     s->done_cv.notify_all();
@@ -494,12 +562,15 @@ void fail_all(mr_server s, const std::string& msg)
         for (const Job& J : sl.jobs)
             for (int64_t r : J.reqs) finish_req(s, r, MR_ERR_HIP, t);
         sl.busy = false;
+        sl.heavy = false;
         sl.jobs.clear();
     }
-    while (!s->queue.empty()) {
-        finish_req(s, s->queue.front(), MR_ERR_HIP, t);
-        s->queue.pop_front();
-    }
+    s->heavy_busy = 0;
+    for (std::deque<int64_t>* q : {&s->queue, &s->hqueue})
+        while (!q->empty()) {
+            finish_req(s, q->front(), MR_ERR_HIP, t);
+            q->pop_front();
+        }
     s->done_cv.notify_all();
 }
 
@@ -524,12 +595,21 @@ void dispatch_loop(mr_server s)
         }
         if (!s->dead) {
             for (Slot& sl : s->slots) {
-                if (sl.busy || s->queue.empty()) continue;
-                form_batch(s, sl);
+                if (sl.busy) continue;
+                // light requests first; heavy batches leave a slot to them
+                bool heavy;
+                if (!s->queue.empty()) heavy = false;
+                else if (!s->hqueue.empty() &&
+                         s->heavy_busy < std::max(1, std::min(s->cfg.heavy_slots, (int)s->slots.size() - 1)))
+                    heavy = true;
+                else continue;
+                form_batch(s, sl, heavy);
                 if (sl.jobs.empty()) continue;
                 for (const Job& J : sl.jobs)
                     for (int64_t r : J.reqs) sl.reqs.push_back(r);
                 sl.busy = true;
+                sl.t_launch = s->now_us();
+                if (sl.heavy) ++s->heavy_busy;
                 busy = true;
                 lk.unlock();
                 const int rc = launch_slot(s, sl, sl.stream);
@@ -540,8 +620,9 @@ void dispatch_loop(mr_server s)
                 }
             }
         }
-        if (s->stop && s->queue.empty() && !busy) break;
-        if (!busy && (s->queue.empty() || s->dead)) {
+        const bool queued = !s->queue.empty() || !s->hqueue.empty();
+        if (s->stop && !queued && !busy) break;
+        if (!busy && (!queued || s->dead)) {
             if (s->stop) break;
             s->cv.wait(lk);
         } else {
@@ -674,9 +755,11 @@ int submit_locked(mr_server s, int mat, int k, uint64_t seed, double t_arr, int6
     const mr_mat m = s->mats[(size_t)mat].m;
     R.flops = 2.0 * (double)m->nnz * (double)k;
     R.bytes = req_bytes(m, k);
+    R.heavy = R.bytes >= kHeavyBytes;
+    const bool heavy = R.heavy;
     s->reqs.push_back(std::move(R));
     const int64_t id = (int64_t)s->reqs.size() - 1;
-    s->queue.push_back(id);
+    (heavy ? s->hqueue : s->queue).push_back(id);
     ++s->mats[(size_t)mat].inflight;
     if (req) *req = id;
     s->cv.notify_one();
@@ -722,6 +805,7 @@ int mr_serve_config_default(mr_serve_config* c)
     c->sigma = 8;
     c->grid = 0;
     c->upload_threads = 0;
+    c->heavy_slots = 1;
     c->slot_bytes = 1LL << 30;
     c->hbm_budget = 0;
     c->hbm_reserve = 4LL << 30;
@@ -744,6 +828,8 @@ int mr_serve_create(mr_ctx ctx, const mr_serve_config* cfg, mr_server* out)
     if (c.grid < 0 || c.slot_bytes < 0 || c.hbm_budget < 0 || c.hbm_reserve < 0)
         return set_error(MR_ERR_INVALID_ARG, "serve: negative size");
     if (c.upload_threads <= 0) c.upload_threads = std::max(1, host_threads() / 2);
+    if (c.heavy_slots < 0) return set_error(MR_ERR_INVALID_ARG, "serve: heavy_slots must be >= 0");
+    if (c.heavy_slots == 0) c.heavy_slots = 1;
     MR_HIP_TRY(hipSetDevice(ctx->device));
     auto s = std::make_unique<mr_server_s>();
     s->ctx = ctx;
@@ -900,9 +986,10 @@ int mr_serve_cancel(mr_server s, int64_t req)
     if (!s) return set_error(MR_ERR_INVALID_ARG, "null server");
     std::lock_guard<std::mutex> g(s->mu);
     if (req < 0 || req >= (int64_t)s->reqs.size()) return set_error(MR_ERR_INVALID_ARG, "unknown request");
-    auto it = std::find(s->queue.begin(), s->queue.end(), req);
-    if (it == s->queue.end()) return set_error(MR_ERR_INVALID_ARG, "request %lld is not queued", (long long)req);
-    s->queue.erase(it);
+    std::deque<int64_t>& q = s->reqs[(size_t)req].heavy ? s->hqueue : s->queue;
+    auto it = std::find(q.begin(), q.end(), req);
+    if (it == q.end()) return set_error(MR_ERR_INVALID_ARG, "request %lld is not queued", (long long)req);
+    q.erase(it);
     finish_req(s, req, MR_ERR_CANCELLED, s->now_us());
     s->done_cv.notify_all();
     return MR_OK;
@@ -995,19 +1082,22 @@ int mr_serve_run_load(mr_server s, const mr_serve_load* load, mr_serve_report* r
     std::unique_lock<std::mutex> lk(s->mu);
     for (int64_t id : ids) s->done_cv.wait(lk, [&] { return s->reqs[(size_t)id].status >= 0; });
     double t_end = t0;
-    std::vector<double> lat;
+    std::vector<double> lat, lat_light, lat_heavy;
     lat.reserve(ids.size());
     double flops = 0.0, bytes = 0.0;
     for (int64_t id : ids) {
         const Req& R = s->reqs[(size_t)id];
         if (R.status == MR_OK) {
             lat.push_back(R.t_done - R.t_arr);
+            (R.heavy ? lat_heavy : lat_light).push_back(R.t_done - R.t_arr);
+            if (R.heavy) ++rep->completed_heavy;
             flops += R.flops;
             bytes += R.bytes;
             t_end = std::max(t_end, R.t_done);
             ++rep->completed;
         } else if (R.status == MR_ERR_TIMEOUT) {
             ++rep->timed_out;
+            if (R.heavy) ++rep->timed_out_heavy;
         } else if (R.status == MR_ERR_CANCELLED) {
             ++rep->cancelled;
         } else {
@@ -1034,12 +1124,17 @@ int mr_serve_run_load(mr_server s, const mr_serve_load* load, mr_serve_report* r
     rep->upload_ms_mean = rep->uploads_admitted ? up_ms / (double)rep->uploads_admitted : 0.0;
     rep->evictions = s->evictions - ev0;
     std::sort(lat.begin(), lat.end());
-    auto pct = [&](double q) {
-        if (lat.empty()) return 0.0;
-        size_t i = (size_t)std::ceil(q * (double)lat.size());
+    std::sort(lat_light.begin(), lat_light.end());
+    std::sort(lat_heavy.begin(), lat_heavy.end());
+    auto pct_of = [](const std::vector<double>& v, double q) {
+        if (v.empty()) return 0.0;
+        size_t i = (size_t)std::ceil(q * (double)v.size());
         if (i > 0) --i;
-        return lat[std::min(i, lat.size() - 1)];
+        return v[std::min(i, v.size() - 1)];
     };
+    auto pct = [&](double q) { return pct_of(lat, q); };
+    rep->p99_light_us = pct_of(lat_light, 0.99);
+    rep->p99_heavy_us = pct_of(lat_heavy, 0.99);
     rep->p50_us = pct(0.50);
     rep->p90_us = pct(0.90);
     rep->p99_us = pct(0.99);

@@ -67,7 +67,7 @@ class RDStats(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int), ("max_jobs", C.c_int), ("coalesce", C.c_int), ("verify", C.c_int),
-                ("sigma", C.c_int), ("grid", C.c_int), ("upload_threads", C.c_int), ("pad", C.c_int),
+                ("sigma", C.c_int), ("grid", C.c_int), ("upload_threads", C.c_int), ("heavy_slots", C.c_int),
                 ("slot_bytes", C.c_int64), ("hbm_budget", C.c_int64), ("hbm_reserve", C.c_int64),
                 ("timeout_ms", C.c_double)]
 
@@ -85,7 +85,9 @@ class ServeReport(C.Structure):
                 ("mean_us", C.c_double), ("max_us", C.c_double), ("wall_s", C.c_double), ("gflops", C.c_double),
                 ("gbytes_s", C.c_double), ("offered_rate", C.c_double), ("achieved_rate", C.c_double),
                 ("batches", C.c_double), ("mean_jobs_per_batch", C.c_double),
-                ("mean_requests_per_batch", C.c_double), ("upload_ms_mean", C.c_double)]
+                ("mean_requests_per_batch", C.c_double), ("upload_ms_mean", C.c_double),
+                ("completed_heavy", C.c_int64), ("timed_out_heavy", C.c_int64), ("p99_light_us", C.c_double),
+                ("p99_heavy_us", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

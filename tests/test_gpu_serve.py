@@ -283,3 +283,36 @@ def test_concurrent_adds_of_two_dtypes(ctx, mr):
         srv.close()
         a.close()
         b.close()
+
+
+def test_light_requests_first_and_heavy_deadlines(ctx, mr):
+    """Requests streaming 32 MiB or more (matrix and vectors) are heavy: a
+    light request queued behind twenty heavy ones is dispatched at the next
+    free slot (FIFO would finish it last), and with a deadline a heavy request
+    predicted to finish late (streamed bytes / the learned rate) is shed at
+    once, while a generous deadline serves it."""
+    n = 1_000_000
+    rp, ci = mr.gen_banded(n, 64, 6, seed=31)          # 6 M nonzeros: 36 MB of BF16 items
+    heavy = mr.Mat.from_csr(ctx, rp, ci, None, n_cols=n, store=mr.BF16)
+    mats, _ = resident(ctx, mr)
+    srv = mr.Server(ctx, slots=1, max_jobs=1, coalesce=0)
+    hid, lid = srv.add(heavy), srv.add(mats[1])
+    hreqs = [srv.submit(hid, 4, i) for i in range(20)]
+    lreq = srv.submit(lid, 1, 99)
+    st_l, lat_l, _ = srv.wait(lreq)
+    res = [srv.wait(r) for r in hreqs]
+    assert st_l == OK and all(r[0] == OK for r in res)
+    assert lat_l < 0.5 * res[-1][1], (lat_l, res[-1][1])
+    srv.close()
+    srv = mr.Server(ctx, timeout_ms=1e-3)
+    hid = srv.add(heavy)
+    assert srv.wait(srv.submit(hid, 1, 1))[0] == TIMEOUT
+    srv.close()
+    srv = mr.Server(ctx, timeout_ms=60_000.0)
+    hid = srv.add(heavy)
+    reqs = [srv.submit(hid, k, i) for i, k in enumerate((1, 4, 16))]
+    assert all(srv.wait(r)[0] == OK for r in reqs)
+    srv.close()
+    heavy.close()
+    for m in mats:
+        m.close()
