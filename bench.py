@@ -519,6 +519,42 @@ def section_cpu_reorders(mr, ctx, extra):
     extra["cpu_reorders"] = out
 
 
+def gorder_cpu_s22_child():
+    """Child process: the sequential CPU Gorder (gorder1, w = 5) on the headline
+    R-MAT s22 matrix; prints one JSON line with its time."""
+    import mrpy as mr
+
+    rp, ci = mr.gen_rmat(22, 16, (0.57, 0.19, 0.19), seed=2)
+    ctx = mr.Ctx(0)
+    m = mr.Mat.from_csr(ctx, rp, ci, None, store=mr.F32)
+    t = time.perf_counter()
+    _, st = m.reorder_gorder_cpu(5)
+    print(json.dumps({"wall_ms": (time.perf_counter() - t) * 1e3, **{k: v for k, v in st.items()}}), flush=True)
+    m.close()
+    ctx.close()
+
+
+def section_cpu_gorder_s22(a, extra):
+    """The reference's CPU Gorder (gorder1, window 5, one thread) on the
+    headline s22 matrix, in a child process under a time limit, beside the
+    pGOrder time of the main section."""
+    import subprocess
+
+    log(f"sequential CPU Gorder (w=5) on R-MAT s22, limit {a.cpu_gorder_timeout:.0f} s")
+    t = time.perf_counter()
+    try:
+        r = subprocess.run([sys.executable, str(pathlib.Path(__file__).resolve()), "--child-gorder-cpu"],
+                           capture_output=True, text=True, timeout=a.cpu_gorder_timeout)
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        out = json.loads(line[-1]) if r.returncode == 0 and line else {"error": r.stderr[-400:]}
+    except subprocess.TimeoutExpired:
+        out = {"timed_out_after_s": a.cpu_gorder_timeout}
+    out["child_wall_s"] = time.perf_counter() - t
+    out["pgorder_total_ms"] = extra.get("pgorder", {}).get("total_ms") if isinstance(extra.get("pgorder"), dict) else None
+    log(f"  {out}")
+    extra["cpu_gorder_s22"] = out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -533,10 +569,16 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5000.0, 20000.0, 50000.0])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-upload-every", type=int, default=1000)
+    ap.add_argument("--cpu-gorder-timeout", type=float, default=900.0,
+                    help="time limit of the cpu_gorder_s22 section (not in the default sections)")
+    ap.add_argument("--child-gorder-cpu", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-heavy-slots", type=int, default=1)
     ap.add_argument("--serve-upload-threads", type=int, default=0, help="host threads of an upload's pGOrder (0 = half)")
     a = ap.parse_args()
+    if a.child_gorder_cpu:
+        gorder_cpu_s22_child()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (this build has no multi-GPU path; "
               f"the sharding design is documented instead)", file=sys.stderr)
@@ -561,6 +603,8 @@ def main():
         m.close()
     if "reference_scale" in sections:
         section_reference_scale(mr, ctx, a, extra)
+    if "cpu_gorder_s22" in sections:
+        section_cpu_gorder_s22(a, extra)
     cpu = None
     if not a.no_cpu and not a.only_main:
         log("CPU baseline (oracle threaded CSR SpMV, FP32)")

@@ -10,7 +10,10 @@
 //   scalar   classes <= 6: one lane per row (rows of at most 64 entries);
 //   both     65..4096 entries: one wave per row; longer: one workgroup per row.
 // Row sums are reduced in a fixed order (xor shuffles inside the group; the
-// waves of a workgroup summed in order), so results are deterministic.  SpMM:
+// waves of a workgroup summed in order), so results are deterministic.  The
+// launch runs the block ranges from the last class down, so the rows over 4096
+// entries (17% of the nonzeros of R-MAT s22 after pGOrder, up to 98 k each)
+// start first.  SpMM:
 // one group per row, lanes over the k right-hand-side columns (row-major X,
 // Y), so each nonzero reads one contiguous X row segment.
 #include "dtypes.hpp"
@@ -169,7 +172,10 @@ __global__ __launch_bounds__(256) void csr_binned_kernel(CsrLaunch L, const int3
                                                          const typename Acc<VT>::type* __restrict__ x,
                                                          typename Acc<VT>::type* __restrict__ y)
 {
-    const int32_t bid = blockIdx.x;
+    // blocks are dispatched in index order: the longest rows (the class
+    // ranges' tail) go first, so their long walks overlap the short rows
+    // instead of finishing alone after them
+    const int32_t bid = (int32_t)(gridDim.x - 1 - blockIdx.x);
     int c = 0;
 #pragma unroll
     for (int q = 1; q < kCsrClasses; ++q) c = bid >= L.blk[q] ? q : c;

@@ -322,3 +322,55 @@ def check_lp_spmm(ctx, mr, oracle, case, k, ldx, store, shape):
     ctx.sync()
     assert np.array_equal(Yd2.to_host()[:, :k].view(np.uint32), Y[:, :k].view(np.uint32))
     mat.close()
+
+
+def large_csr_cases():
+    """Matrices of at least 2^20 nonzeros with every row class: rows over 4096
+    entries (one workgroup each, dispatched first), every shorter class, empty
+    rows; a wide one and a square one of 20000 columns."""
+    rng = np.random.default_rng(29)
+    out = []
+    for name, n, nc, n_long in (("wide", 100_000, 300_000, 24), ("square_20000", 20_000, 20_000, 90)):
+        lens = rng.choice([0, 0, 0, 1, 2, 3, 5, 9, 17, 40, 64, 65, 300, 4096], size=n,
+                          p=[0.2, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.05, 0.05, 0.04, 0.04, 0.01, 0.008, 0.002])
+        lens[rng.choice(n, n_long, replace=False)] = rng.integers(4097, min(nc, 30_000), n_long)
+        rows = [np.sort(rng.choice(nc, int(k), replace=False)) for k in lens]
+        rp = np.zeros(n + 1, np.int32)
+        rp[1:] = np.cumsum(lens)
+        ci = np.concatenate(rows).astype(np.int32)
+        assert len(ci) >= 1 << 20
+        out.append((name, rp, ci, nc))
+    return out
+
+
+LARGE_CASES = large_csr_cases()
+
+
+@pytest.mark.parametrize("case", LARGE_CASES, ids=[c[0] for c in LARGE_CASES])
+@pytest.mark.parametrize("kernel", ["csr_scalar", "csr_vector"])
+@pytest.mark.parametrize("store", ["f64", "f32", "bf16"])
+def test_csr_large_parity(ctx, mr, oracle, case, kernel, store):
+    name, rp, ci, nc = case
+    rng = np.random.default_rng(31)
+    val = rng.uniform(-1, 1, len(ci))
+    x = rng.uniform(-1, 1, nc)
+    dt = {"f64": mr.F64, "f32": mr.F32, "bf16": mr.BF16}[store]
+    mat = mr.Mat.from_csr(ctx, rp, ci, val, n_cols=nc, store=dt)
+    k = {"csr_scalar": mr.CSR_SCALAR, "csr_vector": mr.CSR_VECTOR}[kernel]
+    if store == "f64":
+        y = spmv(ctx, mr, mat, x, k)
+        yhat, absrow = oracle.spmv_f64(rp, ci, val, x)
+        tol = 1e-12
+    else:
+        x32 = x.astype(np.float32)
+        y = spmv(ctx, mr, mat, x32, k)
+        _, _, vs = mat.export()
+        v64 = ((vs.astype(np.uint32) << 16).view(np.float32).astype(np.float64) if store == "bf16"
+               else np.asarray(vs, np.float64))
+        yhat, absrow = oracle.spmv_f64(rp, ci, v64, x32.astype(np.float64))
+        tol = 1e-5
+    worst, l2, _ = oracle.spmv_parity(y, yhat, absrow)
+    assert worst <= tol and l2 <= tol, (worst, l2)
+    y2 = spmv(ctx, mr, mat, x if store == "f64" else x.astype(np.float32), k)
+    assert np.array_equal(np.asarray(y2).view(np.uint8), np.asarray(y).view(np.uint8))
+    mat.close()
