@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <cstddef>
+#include <climits>
 #include <cstdint>
 
 namespace mr {
@@ -11,11 +12,12 @@ namespace k {
 constexpr int kPanelBits = 15;
 constexpr int kPanelCols = 1 << kPanelBits;   // columns per panel: 128 KB of F32 x in LDS
 
-// row-sum stream: tiles of 64 lanes x 8 items; an item is a partial index
-// (>= 0), the end of a row, or padding
+// row-sum stream: tiles of 64 lanes x 8 items holding at most kRowSumTile
+// rows; an item is a partial index (>= 0), padding (kRowPad), or -c for
+// 1 <= c <= kRowSumTile: the end of the current row followed by c - 1 empty
+// rows (which the tile's y stores fill with 0)
 constexpr int kRowSumTile = 512;
-constexpr int32_t kRowEnd = -1;
-constexpr int32_t kRowPad = -2;
+constexpr int32_t kRowPad = INT32_MIN;
 
 // the device arrays of a built panel plan, as the launcher reads them
 struct PanelView {
@@ -40,11 +42,13 @@ struct PanelView {
 size_t panel_lds_bytes(int items_per_lane, int waves);
 // y = A x over the panel plan; part holds one float per partial
 hipError_t panel_spmv(int dt, const PanelView& v, const void* x, void* part, void* y, hipStream_t s);
-// the row-sum stream: items[n_items] all padding, then every row r at its
-// packed position rpos[r]: its partial indices gidx[rs[r] .. rs[r + 1]) and
-// an end marker (a bare end marker for a row longer than a tile)
+// the row-sum stream: items[n_items] all padding, then every row r with
+// rcnt[r] > 0 at its packed position rpos[r]: its partial indices
+// gidx[rs[r] .. rs[r + 1]) and the end marker -rcnt[r] (a bare end marker for
+// a row longer than a tile); rows with rcnt[r] = 0 are empty rows closed by
+// an earlier marker
 hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs, const int32_t* gidx,
-                              const int32_t* rpos, int32_t* items, hipStream_t s);
+                              const int32_t* rpos, const int32_t* rcnt, int32_t* items, hipStream_t s);
 
 // plan build steps (see panel_plan.cpp for the order)
 hipError_t panel_seg_flags(int64_t n_rows, const int32_t* rp, const int32_t* ci, int32_t* F, hipStream_t s);

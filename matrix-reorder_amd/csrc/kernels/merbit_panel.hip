@@ -260,8 +260,9 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
 }
 
 // y[r] = sum of row r's partials.  Tile t of the row-sum stream holds rows
-// trow[t] .. trow[t + 1] - 1: each row's partial indices (>= 0), then its
-// end marker (kRowEnd); kRowPad fills the tile's tail.  Lane l owns the 8
+// trow[t] .. trow[t + 1] - 1: each row's partial indices (>= 0), then an
+// end marker -c closing it and the c - 1 empty rows after it; kRowPad fills
+// the tile's tail.  Lane l owns the 8
 // consecutive items 8l .. 8l + 7, stored as two 16-byte chunks at chunk rows
 // 0 and 1 (c * 64 + l), so each load instruction of a wave reads 1 KB of
 // consecutive bytes.
@@ -301,14 +302,20 @@ __device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, cons
                            (int32_t)b.x, (int32_t)b.y, (int32_t)b.z, (int32_t)b.w};
     float p[8];
     uint32_t ends = 0;
+    int cnt = 0;   // rows this lane's end markers close
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
         const uint32_t neg = (uint32_t)(it[jj] >> 31);
         p[jj] = __uint_as_float(__float_as_uint(g[jj]) & ~neg);
-        ends |= (uint32_t)(it[jj] == kRowEnd) << jj;
+        const bool end = it[jj] < 0 && it[jj] != kRowPad;
+        ends |= (uint32_t)end << jj;
+        cnt += end ? -it[jj] : 0;
     }
-    // rank of this lane's first row in the tile
-    const int cnt = __popc(ends);
+    // empty rows are never written below: the staging starts at zero
+#pragma unroll
+    for (int q = 0; q < kRowSumTile / 64; ++q) stg[lane + 64 * q] = 0.0f;
+    __builtin_amdgcn_wave_barrier();
+    // row offset of this lane's first end marker in the tile
     int inc = cnt;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -329,7 +336,7 @@ __device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, cons
             } else {
                 stg[rk] = acc;
             }
-            ++rk;
+            rk += -it[jj];
             acc = 0.0f;
         }
     }
@@ -596,17 +603,19 @@ __device__ __forceinline__ int64_t rowsum_slot(int64_t q)
 
 __global__ void pn_rowsum_scatter_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
                                          const int32_t* __restrict__ gidx, const int32_t* __restrict__ rpos,
-                                         int32_t* __restrict__ items)
+                                         const int32_t* __restrict__ rcnt, int32_t* __restrict__ items)
 {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = rcnt[r];
+        if (c == 0) continue;   // an empty row closed by an earlier marker
         const int32_t b = rs[r], len = rs[r + 1] - b;
         const int64_t q = rpos[r];
         if (len + 1 > kRowSumTile) {   // a long row: its own kernel adds it
-            items[rowsum_slot(q)] = kRowEnd;
+            items[rowsum_slot(q)] = -c;
             continue;
         }
         for (int32_t i = 0; i < len; ++i) items[rowsum_slot(q + i)] = gidx[b + i];
-        items[rowsum_slot(q + len)] = kRowEnd;
+        items[rowsum_slot(q + len)] = -c;
     }
 }
 
@@ -709,7 +718,7 @@ hipError_t panel_gidx(int64_t n_seg, int tile, const int32_t* pos, const int32_t
 }
 
 hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs, const int32_t* gidx,
-                              const int32_t* rpos, int32_t* items, hipStream_t s)
+                              const int32_t* rpos, const int32_t* rcnt, int32_t* items, hipStream_t s)
 {
     if (n_items > 0) {
         pn_rowsum_fill_kernel<<<grid_of(n_items), 256, 0, s>>>(n_items, items);
@@ -717,7 +726,7 @@ hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs
         if (e != hipSuccess) return e;
     }
     if (n_rows <= 0) return hipSuccess;
-    pn_rowsum_scatter_kernel<<<grid_of(n_rows), 256, 0, s>>>(n_rows, rs, gidx, rpos, items);
+    pn_rowsum_scatter_kernel<<<grid_of(n_rows), 256, 0, s>>>(n_rows, rs, gidx, rpos, rcnt, items);
     return hipGetLastError();
 }
 

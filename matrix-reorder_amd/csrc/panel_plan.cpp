@@ -12,8 +12,9 @@
 //      tile are flagged, and the prefix sum of the flags numbers the partials;
 //   5. every row lists its partials in row-major segment order (rs / gidx);
 //   6. the row-sum stream: the host packs the rows into 512-item tiles that no
-//      row crosses, the device writes each row's partial indices and an end
-//      marker at its packed position.
+//      row crosses (runs of empty rows folded into the end marker before
+//      them), the device writes each row's partial indices and an end marker
+//      at its packed position.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -274,44 +275,60 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
                                    s));
     MR_HIP_TRY(k::panel_gidx(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), goff.as<int32_t>(), pidx.as<int32_t>(),
                              p.pgidx.as<int32_t>(), s));
-    // 6. the row-sum stream.  Rows are packed in order into 512-item tiles:
-    //    a row takes its partials plus an end marker and moves to the next
-    //    tile when it does not fit the current one; a row with more partials
-    //    than a tile holds takes a bare end marker and is listed as long.
-    std::vector<int32_t> hrs((size_t)n + 1), rpos((size_t)std::max<int64_t>(n, 1)), trow, lrow;
+    // 6. the row-sum stream.  Rows are packed in order into tiles of 512 items
+    //    and at most 512 rows: a row takes its partials plus an end marker and
+    //    moves to the next tile when either does not fit; an empty row after
+    //    an end marker in the same tile adds one to that marker's count
+    //    instead; a row with more partials than a tile holds takes a bare end
+    //    marker and is listed as long.
+    std::vector<int32_t> hrs((size_t)n + 1), rpos((size_t)std::max<int64_t>(n, 1)),
+        rcnt((size_t)std::max<int64_t>(n, 1), 0), trow, lrow;
     MR_HIP_TRY(hipMemcpyAsync(hrs.data(), p.prs.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
     MR_HIP_TRY(hipStreamSynchronize(s));
     constexpr int64_t RT = k::kRowSumTile;
-    int64_t cur = 0, rt = 0;
+    int64_t cur = 0, rt = 0, rows_in = 0;
+    int64_t last_end = -1;   // row that owns the tile's last end marker (-1: none)
     if (n > 0) trow.push_back(0);
     for (int64_t r = 0; r < n; ++r) {
-        int64_t L = (int64_t)hrs[(size_t)r + 1] - hrs[(size_t)r] + 1;
+        const int64_t len = (int64_t)hrs[(size_t)r + 1] - hrs[(size_t)r];
+        if (len == 0 && last_end >= 0 && rows_in < RT && rcnt[(size_t)last_end] < RT) {
+            ++rcnt[(size_t)last_end];
+            ++rows_in;
+            continue;
+        }
+        int64_t L = len + 1;
         if (L > RT) {
             lrow.push_back((int32_t)r);
             L = 1;
         }
-        if (cur + L > RT) {
+        if (cur + L > RT || rows_in >= RT) {
             ++rt;
             cur = 0;
+            rows_in = 0;
             trow.push_back((int32_t)r);
         }
         rpos[(size_t)r] = (int32_t)(rt * RT + cur);
+        rcnt[(size_t)r] = 1;
+        last_end = r;
         cur += L;
+        ++rows_in;
         if (rt * RT + cur >= INT32_MAX)
             return set_error(MR_ERR_UNSUPPORTED, "column panels: the row-sum stream exceeds the int32 item index");
     }
     const int64_t n_rtiles = n > 0 ? rt + 1 : 0;
     trow.push_back((int32_t)n);
     {
-        DevBuf d_rpos;
+        DevBuf d_rpos, d_rcnt;
         MR_TRY(d_rpos.alloc(sizeof(int32_t) * rpos.size()));
+        MR_TRY(d_rcnt.alloc(sizeof(int32_t) * rcnt.size()));
         MR_TRY(p.prsi.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(n_rtiles * RT, 1)));
         MR_TRY(p.ptrow.alloc(sizeof(int32_t) * trow.size()));
         MR_HIP_TRY(hipMemcpyAsync(d_rpos.p, rpos.data(), sizeof(int32_t) * rpos.size(), hipMemcpyHostToDevice, s));
+        MR_HIP_TRY(hipMemcpyAsync(d_rcnt.p, rcnt.data(), sizeof(int32_t) * rcnt.size(), hipMemcpyHostToDevice, s));
         MR_HIP_TRY(hipMemcpyAsync(p.ptrow.p, trow.data(), sizeof(int32_t) * trow.size(), hipMemcpyHostToDevice, s));
         MR_HIP_TRY(k::panel_rowsum_build(n, n_rtiles * RT, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
-                                         d_rpos.as<int32_t>(), p.prsi.as<int32_t>(), s));
-        MR_HIP_TRY(hipStreamSynchronize(s));   // d_rpos and the host vectors are freed on return
+                                         d_rpos.as<int32_t>(), d_rcnt.as<int32_t>(), p.prsi.as<int32_t>(), s));
+        MR_HIP_TRY(hipStreamSynchronize(s));   // the staging buffers and host vectors are freed on return
     }
     if (lrow.empty()) {
         p.prs.release();

@@ -46,6 +46,18 @@ def panel_cases():
             rows.append(np.sort(rng.choice(nc, int(rng.integers(0, 9)), replace=False)))
     rp, ci = csr_rows(rows, nc)
     cases.append(("rowsum_long_rows", rp, ci, nc))
+    # runs of empty rows (folded into the end marker before them, at most 512
+    # rows per row-sum tile): 1, 511, 512, 513 and 1500 in a row, one at the
+    # start and one at the end of the matrix
+    nc = 70_000
+    rows = [np.zeros(0, np.int64)] * 700
+    for run in (1, 511, 512, 513, 1500, 3):
+        rows += [np.sort(rng.choice(nc, int(rng.integers(1, 30)), replace=False)) for _ in range(int(rng.integers(1, 40)))]
+        rows += [np.zeros(0, np.int64)] * run
+    rows += [np.sort(rng.choice(nc, 600, replace=False))]          # a long row, then more empties
+    rows += [np.zeros(0, np.int64)] * 900
+    rp, ci = csr_rows(rows, nc)
+    cases.append(("empty_row_runs", rp, ci, nc))
     return cases
 
 
