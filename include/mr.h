@@ -327,8 +327,8 @@ int mr_merbit_configure(mr_mat m, int sigma);
  * with an LDS cache of the hottest columns of x (F32/F16/BF16 only; its tiles
  * are set by mr_merbit_xc_configure), bit 3 = the column-panel variant (F32/
  * F16/BF16 only; mr_merbit_panel_configure); bits 2 and 3 exclude each other.
- * Default 4 for F32/F16/BF16 matrices with at least 4,194,304 nonzeros, 1
- * otherwise.  Bits 0 and 1 do not change results; bits 2 and 3 change the
+ * Default 9 (column panels, non-temporal streams) for F32/F16/BF16 matrices
+ * with at least 4,194,304 nonzeros, 1 otherwise.  Bits 0 and 1 do not change results; bits 2 and 3 change the
  * partition and hence the rounding order (results stay deterministic) */
 int mr_merbit_options(mr_mat m, int policy);
 int mr_merbit_get_options(mr_mat m, int* policy);

@@ -501,9 +501,11 @@ int mr_csr_create(mr_ctx ctx, int64_t n_rows, int64_t n_cols, int64_t nnz, const
     for (int64_t i = 0; i < n_rows; ++i) m->perm[(size_t)i] = (int32_t)i;
     MR_HIP_TRY(hipStreamSynchronize(s));
     MR_TRY(merbit_build(m.get(), s));
-    // large FP32-accumulate matrices default to the lane-padded x-cache
-    // variant; small ones (cache fill would dominate) to workgroup tiles
-    m->merbit.policy = (store_dtype != MR_F64 && nnz >= (int64_t(1) << 22)) ? 4 : 1;
+    // large FP32-accumulate matrices default to the column-panel variant with
+    // non-temporal streams (the autotuner's pick at R-MAT s22; its plan builds
+    // in about the time of the lane-padded one's); small ones (panel loads
+    // would dominate) to workgroup tiles
+    m->merbit.policy = (store_dtype != MR_F64 && nnz >= (int64_t(1) << 22)) ? 9 : 1;
     *out = m.release();
     return MR_OK;
 }
@@ -762,12 +764,14 @@ int mr_spmm(mr_mat m, int kernel, int k, const void* X, int64_t ldx, void* Y, in
     MerbitPlan& p = m->merbit;
     if ((p.policy & 12) && m->dtype != MR_F64 && ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
         k::merbit_lp_mm_fits(p.xs, p.xw, std::min(k, 16))) {
-        // lane-padded tiles: 16 columns per pass, X rows cached in LDS
+        // lane-padded tiles: 16 columns per pass, X rows cached in LDS; the
+        // streaming hint of a column-panel policy belongs to the panels
         MR_TRY(merbit_lp_mm_build(m, st));
         const size_t vb = vec_size(m->dtype);
+        const int nt = (p.policy & 4) ? (p.policy & 1) : 0;
         for (int j0 = 0; j0 < k; j0 += 16) {
             const int kk = std::min(16, k - j0);
-            MR_HIP_TRY(k::merbit_lp_spmm(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles, p.pcm.as<int32_t>(),
+            MR_HIP_TRY(k::merbit_lp_spmm(m->dtype, p.xs, p.xw, nt, m->n_rows, p.xtiles, p.pcm.as<int32_t>(),
                                          p.pv.p, p.xty.as<int32_t>(), static_cast<const char*>(X) + vb * (size_t)j0,
                                          ldx, kk, merbit_lp_mm_rows(m, kk), static_cast<char*>(Y) + vb * (size_t)j0,
                                          ldy, p.xcarry_mm.p, p.xcarry_row.as<int32_t>(),

@@ -471,9 +471,11 @@ int launch_slot(mr_server s, Slot& sl, hipStream_t st)
                                          dev + J.y_off, dev + J.c_off, reinterpret_cast<int32_t*>(dev + J.r_off), grid,
                                          st));
         } else {
+            // the streaming hint of a column-panel policy belongs to the panels
+            const int nt = (p.policy & 4) ? (p.policy & 1) : 0;
             for (int j0 = 0; j0 < J.K; j0 += 16) {
                 const int kk = std::min(16, J.K - j0);
-                MR_HIP_TRY(k::merbit_lp_spmm(m->dtype, p.xs, p.xw, p.policy & 1, m->n_rows, p.xtiles,
+                MR_HIP_TRY(k::merbit_lp_spmm(m->dtype, p.xs, p.xw, nt, m->n_rows, p.xtiles,
                                              p.pcm.as<int32_t>(), p.pv.p, p.xty.as<int32_t>(),
                                              dev + J.x_off + vb * (size_t)j0, J.ldx, kk, merbit_lp_mm_rows(m, kk),
                                              dev + J.y_off + vb * (size_t)j0, J.K, dev + J.c_off,
