@@ -189,9 +189,8 @@ def collect(out_path: str, limit: int = 0, reps: int = 20, skip: int = 0) -> Non
             yd = ctx.empty(n, np.float32)
             # the pGOrder order: reference point of the speedups
             m0 = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
-            # the GPU loop takes widths up to 1024; get_width asks for 16384
-            # when sqrt(n / floor(nnz / n)) >= 2^9.5
-            pg, _ = m0.reorder_pgorder(min(mr.get_width(n, nnz), 1024), 5)
+            # pGOrder at get_width's width, as the reference calls gorder6
+            pg, _ = m0.reorder_pgorder(0, 5)
             grp, gci, gval = m0.export()
             m0.close()
             # other orders, each given as a relabelling of the original
