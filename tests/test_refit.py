@@ -54,17 +54,24 @@ def test_c_abi_model_equals_fitted_pipeline(refit, fitted, mr, kernel, model):
     np.testing.assert_allclose(got, want[:64], rtol=1e-12)
 
 
-def test_rd_params_of_each_model(refit, mr):
+def test_rd_params_of_each_model(refit, mr, fitted):
     ref = mr.rd_params()
     for model in (mr.MODEL_MERBIT, mr.MODEL_CSR):
         p = mr.rd_params(model)
         assert [p.bin_edges[i] for i in range(9)] == [ref.bin_edges[i] for i in range(9)]
         assert p.rd_cap == ref.rd_cap
+    # the MI355X models use the edge set the refit chose (by cross-validated
+    # MAE) for both kernels
+    _, chosen = fitted
+    if chosen["merbit"][0] == "mi355x_edges":
+        edges, cap = refit.MI355X_EDGES, refit.MI355X_EDGES[-1]
+    else:
+        edges, cap = tuple(ref.bin_edges[i] for i in range(9)), ref.rd_cap
     for model in (mr.MODEL_MERBIT_MI355X, mr.MODEL_CSR_MI355X):
         p = mr.rd_params(model)
         assert p.n_edges == 9
-        assert tuple(p.bin_edges[i] for i in range(9)) == refit.MI355X_EDGES
-        assert p.rd_cap == refit.MI355X_EDGES[-1]
+        assert tuple(p.bin_edges[i] for i in range(9)) == edges
+        assert p.rd_cap == cap
         assert p.samples == ref.samples and p.hll_p == ref.hll_p and p.sector_size == ref.sector_size
     with pytest.raises(mr.MRError):
         mr.rd_params(7)
