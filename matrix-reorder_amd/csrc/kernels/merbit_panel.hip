@@ -67,7 +67,7 @@ template <> __device__ __forceinline__ float value_at<__half>(const uint32_t* w,
 template <typename VT, int S, bool NT>
 __device__ __forceinline__ void load_tile(const uint16_t* __restrict__ pcol, const VT* __restrict__ pval,
                                           const int32_t* __restrict__ tseg, int32_t t, int lane, uint32_t* cw,
-                                          uint32_t* vw, int32_t* pb)
+                                          uint32_t* vw, int32_t* pb, int32_t* pe)
 {
     constexpr int T = 64 * S;
     constexpr int CW = S / 2;
@@ -89,16 +89,18 @@ __device__ __forceinline__ void load_tile(const uint16_t* __restrict__ pcol, con
         vw[4 * q] = b.x; vw[4 * q + 1] = b.y; vw[4 * q + 2] = b.z; vw[4 * q + 3] = b.w;
     }
     *pb = tseg[t];
+    *pe = tseg[t + 1];
 }
 
 // one tile: products from the LDS panel, a wave scan ranks the closed
 // segments, a segmented scan joins each lane's open head to the lanes before
-// it.  The tile's partials (a contiguous range part[pbase, pbase + total))
-// are ranked into the wave's LDS staging area and written out with full-wave
-// stores: one store instruction per 64 partials instead of one per item slot.
+// it.  The tile's partials (the contiguous range part[pbase, pend), numbered
+// by the plan) are ranked into the wave's LDS staging area and written out
+// with full-wave stores: one store instruction per 64 partials instead of one
+// per item slot.
 template <typename VT, int S, bool NT>
-__device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* vw, int32_t pbase, int lane,
-                                           const float* xs, float* stg, float* __restrict__ part)
+__device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* vw, int32_t pbase, int32_t pend,
+                                           int lane, const float* xs, float* stg, float* __restrict__ part)
 {
     float p[S];
     uint32_t ends = 0;
@@ -113,50 +115,37 @@ __device__ __forceinline__ void panel_tile(const uint32_t* cw, const uint32_t* v
 
     // rank of this lane's first partial in the tile
     const int cnt = __popc(ends);
-    int inc = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += o;
-    }
-    const int r0 = inc - cnt;
+    const int r0 = wave_incl_sum(cnt) - cnt;
+    // Without branches: every item's running sum is written at the rank of
+    // the next segment end at or after it, and a later write replaces it --
+    // the end's own sum in this lane, or, for the slot of a lane's first end,
+    // the write after the segmented scan below (a lane's items after its last
+    // end land there too).  An item that is not an end leaves at most 64 S - 1
+    // partials in the tile, so every rank written stays inside this wave's
+    // staging area.
     int r = r0;
     float acc = 0.0f, head = 0.0f;
-    bool has_end = false;
 #pragma unroll
     for (int jj = 0; jj < S; ++jj) {
         acc += p[jj];
-        if ((ends >> jj) & 1u) {
-            if (!has_end) {
-                head = acc;
-                has_end = true;
-            } else {
-                stg[r] = acc;
-            }
-            ++r;
-            acc = 0.0f;
-        }
+        const bool e = (ends >> jj) & 1u;
+        stg[r] = acc;
+        head = (e && r == r0) ? acc : head;
+        r += e ? 1 : 0;
+        acc = e ? 0.0f : acc;
     }
+    const bool has_end = ends != 0;
     // segmented scan of (tail, has_end): the open sum each lane's first
     // closed segment continues from the lanes before it
     float sv = acc;
     int fl = has_end ? 1 : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float vu = __shfl_up(sv, d, 64);
-        const int fu = __shfl_up(fl, d, 64);
-        if (lane >= d) {
-            if (!fl) sv += vu;
-            fl |= fu;
-        }
-    }
-    const float up = __shfl_up(sv, 1, 64);
-    if (has_end) stg[r0] = (lane == 0 ? 0.0f : up) + head;
+    wave_seg_scan(sv, fl);
+    const float up = wave_shr1(sv);
+    if (has_end) stg[r0] = up + head;
     // LDS operations of one wave complete in order: the reads below see the
-    // writes above
-    const int total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
-    // plain stores: the partial sums stay cached for the kernel that adds them
-    for (int q = lane; q < total; q += 64) part[pbase + q] = stg[q];
+    // writes above.  Plain stores: the partial sums stay cached for the
+    // kernel that adds them.
+    for (int q = pbase + lane; q < pend; q += 64) part[q] = stg[q - pbase];
 }
 
 // jobs[3j .. 3j+2] = (panel, first tile, end tile); job list i runs jobs
@@ -206,10 +195,10 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
         // all outstanding loads.
         const int32_t tl = t1 - 1;
         uint32_t ca[CW], va[VW], cb[CW], vb[VW], cc[CW], vc[VW];
-        int32_t pa = 0, pb = 0, pc = 0;
+        int32_t pa = 0, pb = 0, pc = 0, ea = 0, eb = 0, ec = 0;
         if (t < t1) {
-            load_tile<VT, S, NT>(pcol, pval, tseg, t, lane, ca, va, &pa);
-            load_tile<VT, S, NT>(pcol, pval, tseg, t + W < t1 ? t + W : tl, lane, cb, vb, &pb);
+            load_tile<VT, S, NT>(pcol, pval, tseg, t, lane, ca, va, &pa, &ea);
+            load_tile<VT, S, NT>(pcol, pval, tseg, t + W < t1 ? t + W : tl, lane, cb, vb, &pb, &eb);
         }
         __syncthreads();   // the previous job's tiles are done with xs
         if (have_xn) {
@@ -247,15 +236,15 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
         auto tile_at = [&](int k) { return t + k * W < t1 ? t + k * W : tl; };
         int k = 0;
         for (; k + 3 <= n; k += 3) {
-            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 2), lane, cc, vc, &pc);
-            panel_tile<VT, S, NT>(ca, va, pa, lane, xs, stg, part);
-            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 3), lane, ca, va, &pa);
-            panel_tile<VT, S, NT>(cb, vb, pb, lane, xs, stg, part);
-            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 4), lane, cb, vb, &pb);
-            panel_tile<VT, S, NT>(cc, vc, pc, lane, xs, stg, part);
+            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 2), lane, cc, vc, &pc, &ec);
+            panel_tile<VT, S, NT>(ca, va, pa, ea, lane, xs, stg, part);
+            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 3), lane, ca, va, &pa, &ea);
+            panel_tile<VT, S, NT>(cb, vb, pb, eb, lane, xs, stg, part);
+            load_tile<VT, S, NT>(pcol, pval, tseg, tile_at(k + 4), lane, cb, vb, &pb, &eb);
+            panel_tile<VT, S, NT>(cc, vc, pc, ec, lane, xs, stg, part);
         }
-        if (k < n) panel_tile<VT, S, NT>(ca, va, pa, lane, xs, stg, part);
-        if (k + 1 < n) panel_tile<VT, S, NT>(cb, vb, pb, lane, xs, stg, part);
+        if (k < n) panel_tile<VT, S, NT>(ca, va, pa, ea, lane, xs, stg, part);
+        if (k + 1 < n) panel_tile<VT, S, NT>(cb, vb, pb, eb, lane, xs, stg, part);
     }
 }
 
@@ -272,9 +261,12 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
 // latency-bound; three or four tiles per wave ran slower at R-MAT s22, and so
 // did persistent waves, whose next-tile loads and y stores share the in-order
 // memory counter the gathers wait on).  The blocks are dispatched round-robin
-// over the 8 XCDs; block b takes tile group (b % 8) * (gridDim.x / 8) + b / 8,
-// which keeps each XCD on one contiguous eighth of the rows, so the partial
-// lines those rows share are fetched into one L2.
+// over the 8 XCDs.  The tile groups are dealt to the XCDs in chunks of
+// kRowSumChunk consecutive blocks: the rows of a chunk are consecutive, so the
+// partial lines they share are fetched into one L2, and every XCD gets chunks
+// from all over the matrix.  (One contiguous eighth per XCD left the XCD with
+// the hub rows, whose partials spread over more lines, finishing last.)
+constexpr int kRowSumChunk = 16;
 __device__ __forceinline__ void rowsum_load(const int32_t* __restrict__ items, int64_t t, int lane, u32x4* a, u32x4* b)
 {
     const u32x4* ip = reinterpret_cast<const u32x4*>(items + t * kRowSumTile) + lane;
@@ -295,8 +287,8 @@ __device__ __forceinline__ void rowsum_gather(const u32x4 a, const u32x4 b, cons
     }
 }
 
-__device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, const float* g, int32_t r0, int lane,
-                                              float* stg, float* __restrict__ y)
+__device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, const float* g, int32_t r0, int32_t r1,
+                                              int lane, float* stg, float* __restrict__ y)
 {
     const int32_t it[8] = {(int32_t)a.x, (int32_t)a.y, (int32_t)a.z, (int32_t)a.w,
                            (int32_t)b.x, (int32_t)b.y, (int32_t)b.z, (int32_t)b.w};
@@ -316,13 +308,7 @@ __device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, cons
     for (int q = 0; q < kRowSumTile / 64; ++q) stg[lane + 64 * q] = 0.0f;
     __builtin_amdgcn_wave_barrier();
     // row offset of this lane's first end marker in the tile
-    int inc = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += o;
-    }
-    const int rk0 = inc - cnt;
+    const int rk0 = wave_incl_sum(cnt) - cnt;
     int rk = rk0;
     float acc = 0.0f, head = 0.0f;
     bool has_end = false;
@@ -345,21 +331,12 @@ __device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, cons
     // last lane's tail is zero.
     float sv = acc;
     int fl = has_end ? 1 : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float vu = __shfl_up(sv, d, 64);
-        const int fu = __shfl_up(fl, d, 64);
-        if (lane >= d) {
-            if (!fl) sv += vu;
-            fl |= fu;
-        }
-    }
-    const float up = __shfl_up(sv, 1, 64);
-    if (has_end) stg[rk0] = (lane == 0 ? 0.0f : up) + head;
+    wave_seg_scan(sv, fl);
+    const float up = wave_shr1(sv);
+    if (has_end) stg[rk0] = up + head;
     __builtin_amdgcn_wave_barrier();
-    // the tile's rows are consecutive: full-wave stores of y[r0 .. r0 + total)
-    const int total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
-    for (int q = lane; q < total; q += 64) y[r0 + q] = stg[q];
+    // the tile's rows are consecutive: full-wave stores of y[r0, r1)
+    for (int q = r0 + lane; q < r1; q += 64) y[q] = stg[q - r0];
     // the next tile's staging writes come after every lane's reads above
     __builtin_amdgcn_wave_barrier();
 }
@@ -370,8 +347,13 @@ __global__ __launch_bounds__(256) void panel_rowsum_kernel(const int32_t* __rest
 {
     __shared__ float stg_all[4][kRowSumTile];
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int per_xcd = gridDim.x / 8;
-    const int64_t g = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    // block b is the i-th block of XCD x = b % 8; gridDim.x is a multiple of 8.
+    // Its group: chunk i / kRowSumChunk of XCD x, dealt round-robin, or past
+    // the last whole round of chunks, a contiguous share of the rest.
+    const int per_xcd = gridDim.x / 8, xcd = blockIdx.x % 8, i = blockIdx.x / 8;
+    const int full = per_xcd / kRowSumChunk * kRowSumChunk;
+    const int64_t g = i < full ? ((int64_t)(i / kRowSumChunk) * 8 + xcd) * kRowSumChunk + i % kRowSumChunk
+                               : (int64_t)8 * full + (int64_t)xcd * (per_xcd - full) + (i - full);
     const int64_t t = (g * 4 + wid) * 2;
     if (t >= n_tiles) return;   // the whole wave leaves; no block barrier follows
     const bool two = t + 1 < n_tiles;
@@ -381,8 +363,8 @@ __global__ __launch_bounds__(256) void panel_rowsum_kernel(const int32_t* __rest
     float g0[8], g1[8];
     rowsum_gather(a0, b0, part, g0);
     rowsum_gather(a1, b1, part, g1);
-    rowsum_reduce(a0, b0, g0, trow[t], lane, stg_all[wid], y);
-    if (two) rowsum_reduce(a1, b1, g1, trow[t + 1], lane, stg_all[wid], y);
+    rowsum_reduce(a0, b0, g0, trow[t], trow[t + 1], lane, stg_all[wid], y);
+    if (two) rowsum_reduce(a1, b1, g1, trow[t + 1], trow[t + 2], lane, stg_all[wid], y);
 }
 
 // rows with more partials than a row-sum tile holds (their tile entry is a
