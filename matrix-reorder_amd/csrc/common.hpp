@@ -135,12 +135,13 @@ struct MerbitPlan {
     // panel-major streams, ptseg = first partial of every tile, pjobs / pwg =
     // the workgroups' (panel, tile range) jobs, ppart = the partial sums
     // (pparts floats); prsi = the row-sum stream (prtiles tiles of 512 items:
-    // each row's partial indices and an end marker), ptrow = first row of
-    // every row-sum tile; plrow = the pn_long rows too long for a tile, whose
-    // partials prs / pgidx list (released when there are none)
+    // each row's partial indices, prs_ib bits wide, the last one carrying the
+    // number of rows that end there), ptrow = first row of every row-sum
+    // tile; plrow = the pn_long rows too long for a tile, whose partials
+    // prs / pgidx list (released when there are none)
     int ps = 8, pw = 16;
     int64_t ptiles = 0, pparts = 0, psegs = 0, prtiles = 0;
-    int pn_wg = 0, pn_jobs = 0, pn_long = 0;
+    int pn_wg = 0, pn_jobs = 0, pn_long = 0, prs_ib = 31;
     mr::DevBuf pcol, pval, ptseg, pjobs, pwg, prs, pgidx, ppart, prsi, ptrow, plrow;
     bool pn_built = false;
     double pn_build_ms = 0.0;

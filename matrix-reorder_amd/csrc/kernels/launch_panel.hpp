@@ -12,11 +12,18 @@ namespace k {
 constexpr int kPanelBits = 15;
 constexpr int kPanelCols = 1 << kPanelBits;   // columns per panel: 128 KB of F32 x in LDS
 
-// row-sum stream: tiles of 64 lanes x 8 items holding at most kRowSumTile
-// rows; an item is a partial index (>= 0), padding (kRowPad), or -c for
-// 1 <= c <= kRowSumTile: the end of the current row followed by c - 1 empty
-// rows (which the tile's y stores fill with 0)
+// row-sum stream: tiles of 64 lanes x 8 = kRowSumTile items holding at most
+// kRowSumRows rows.  An item is
+//   >= 0     a partial: its index in the low ib bits (ib = the stream's index
+//            width), and above them c, the number of rows that end after it:
+//            0 if the row goes on, else the row ends and c - 1 empty rows follow;
+//   -c       (1 <= c <= kRowSumRows) a row without partials in the tile ends (a
+//            row too long for a tile, or an empty row no earlier end absorbed),
+//            and c - 1 empty rows follow;
+//   kRowPad  padding.
+// The empty rows get 0 from the tile's y stores.
 constexpr int kRowSumTile = 512;
+constexpr int kRowSumRows = 1024;
 constexpr int32_t kRowPad = INT32_MIN;
 
 // the device arrays of a built panel plan, as the launcher reads them
@@ -30,6 +37,7 @@ struct PanelView {
     int n_wg = 0;
     int64_t n_cols = 0, n_rows = 0;
     const int32_t* items = nullptr;   // row-sum stream
+    int rs_ib = 31;                   // its index width
     const int32_t* trow = nullptr;    // first row of every row-sum tile, plus n_rows
     int64_t n_rtiles = 0;
     const int32_t* lrow = nullptr;    // rows too long for a row-sum tile
@@ -44,10 +52,10 @@ size_t panel_lds_bytes(int items_per_lane, int waves);
 hipError_t panel_spmv(int dt, const PanelView& v, const void* x, void* part, void* y, hipStream_t s);
 // the row-sum stream: items[n_items] all padding, then every row r with
 // rcnt[r] > 0 at its packed position rpos[r]: its partial indices
-// gidx[rs[r] .. rs[r + 1]) and the end marker -rcnt[r] (a bare end marker for
-// a row longer than a tile); rows with rcnt[r] = 0 are empty rows closed by
-// an earlier marker
-hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs, const int32_t* gidx,
+// gidx[rs[r] .. rs[r + 1]), the last one carrying rcnt[r] << ib; a row without
+// partials or longer than a tile gets the marker -rcnt[r] instead.  Rows with
+// rcnt[r] = 0 are empty rows an earlier end closes.
+hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, int ib, const int32_t* rs, const int32_t* gidx,
                               const int32_t* rpos, const int32_t* rcnt, int32_t* items, hipStream_t s);
 
 // plan build steps (see panel_plan.cpp for the order)

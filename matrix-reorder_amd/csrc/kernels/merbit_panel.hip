@@ -249,12 +249,13 @@ __global__ __launch_bounds__(64 * W) void panel_spmv_kernel(const uint16_t* __re
 }
 
 // y[r] = sum of row r's partials.  Tile t of the row-sum stream holds rows
-// trow[t] .. trow[t + 1] - 1: each row's partial indices (>= 0), then an
-// end marker -c closing it and the c - 1 empty rows after it; kRowPad fills
-// the tile's tail.  Lane l owns the 8
-// consecutive items 8l .. 8l + 7, stored as two 16-byte chunks at chunk rows
-// 0 and 1 (c * 64 + l), so each load instruction of a wave reads 1 KB of
-// consecutive bytes.
+// trow[t] .. trow[t + 1] - 1: each row's partial indices (>= 0), the last one
+// carrying the count c of rows that end there (the row and the c - 1 empty
+// rows after it), or a marker -c for a row with no partials in the tile;
+// kRowPad fills the tile's tail.  Lane l owns the 8 consecutive items
+// 8l .. 8l + 7, stored as two 16-byte chunks at chunk rows 0 and 1
+// (c * 64 + l), so each load instruction of a wave reads 1 KB of consecutive
+// bytes.
 //
 // A wave takes two consecutive tiles: both item loads, then all sixteen
 // partial gathers, then the two sums (one tile per wave left the gathers
@@ -276,36 +277,38 @@ __device__ __forceinline__ void rowsum_load(const int32_t* __restrict__ items, i
 
 // the tile's eight partials per lane, each loaded at an in-bounds index
 // (markers read part[0]; the sum masks them), so the loads go out together
-__device__ __forceinline__ void rowsum_gather(const u32x4 a, const u32x4 b, const float* __restrict__ part, float* g)
+__device__ __forceinline__ void rowsum_gather(const u32x4 a, const u32x4 b, uint32_t imask,
+                                              const float* __restrict__ part, float* g)
 {
     const int32_t it[8] = {(int32_t)a.x, (int32_t)a.y, (int32_t)a.z, (int32_t)a.w,
                            (int32_t)b.x, (int32_t)b.y, (int32_t)b.z, (int32_t)b.w};
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
         const uint32_t neg = (uint32_t)(it[jj] >> 31);   // all ones for a marker
-        g[jj] = part[(uint32_t)it[jj] & ~neg];
+        g[jj] = part[(uint32_t)it[jj] & imask & ~neg];
     }
 }
 
-__device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, const float* g, int32_t r0, int32_t r1,
-                                              int lane, float* stg, float* __restrict__ y)
+__device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, const float* g, int ib, int32_t r0,
+                                              int32_t r1, int lane, float* stg, float* __restrict__ y)
 {
     const int32_t it[8] = {(int32_t)a.x, (int32_t)a.y, (int32_t)a.z, (int32_t)a.w,
                            (int32_t)b.x, (int32_t)b.y, (int32_t)b.z, (int32_t)b.w};
     float p[8];
+    int c[8];       // rows that end after each item
     uint32_t ends = 0;
-    int cnt = 0;   // rows this lane's end markers close
+    int cnt = 0;    // rows this lane's items close
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
         const uint32_t neg = (uint32_t)(it[jj] >> 31);
         p[jj] = __uint_as_float(__float_as_uint(g[jj]) & ~neg);
-        const bool end = it[jj] < 0 && it[jj] != kRowPad;
-        ends |= (uint32_t)end << jj;
-        cnt += end ? -it[jj] : 0;
+        c[jj] = neg ? (it[jj] == kRowPad ? 0 : -it[jj]) : (int)((uint32_t)it[jj] >> ib);
+        ends |= (uint32_t)(c[jj] != 0) << jj;
+        cnt += c[jj];
     }
     // empty rows are never written below: the staging starts at zero
 #pragma unroll
-    for (int q = 0; q < kRowSumTile / 64; ++q) stg[lane + 64 * q] = 0.0f;
+    for (int q = 0; q < kRowSumRows / 64; ++q) stg[lane + 64 * q] = 0.0f;
     __builtin_amdgcn_wave_barrier();
     // row offset of this lane's first end marker in the tile
     const int rk0 = wave_incl_sum(cnt) - cnt;
@@ -322,7 +325,7 @@ __device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, cons
             } else {
                 stg[rk] = acc;
             }
-            rk += -it[jj];
+            rk += c[jj];
             acc = 0.0f;
         }
     }
@@ -343,9 +346,11 @@ __device__ __forceinline__ void rowsum_reduce(const u32x4 a, const u32x4 b, cons
 
 __global__ __launch_bounds__(256) void panel_rowsum_kernel(const int32_t* __restrict__ items,
                                                            const int32_t* __restrict__ trow, int64_t n_tiles,
-                                                           const float* __restrict__ part, float* __restrict__ y)
+                                                           int ib, const float* __restrict__ part,
+                                                           float* __restrict__ y)
 {
-    __shared__ float stg_all[4][kRowSumTile];
+    __shared__ float stg_all[4][kRowSumRows];
+    const uint32_t imask = ib >= 31 ? 0x7fffffffu : (1u << ib) - 1u;
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // block b is the i-th block of XCD x = b % 8; gridDim.x is a multiple of 8.
     // Its group: chunk i / kRowSumChunk of XCD x, dealt round-robin, or past
@@ -361,10 +366,10 @@ __global__ __launch_bounds__(256) void panel_rowsum_kernel(const int32_t* __rest
     rowsum_load(items, t, lane, &a0, &b0);
     rowsum_load(items, two ? t + 1 : t, lane, &a1, &b1);   // in bounds either way
     float g0[8], g1[8];
-    rowsum_gather(a0, b0, part, g0);
-    rowsum_gather(a1, b1, part, g1);
-    rowsum_reduce(a0, b0, g0, trow[t], trow[t + 1], lane, stg_all[wid], y);
-    if (two) rowsum_reduce(a1, b1, g1, trow[t + 1], trow[t + 2], lane, stg_all[wid], y);
+    rowsum_gather(a0, b0, imask, part, g0);
+    rowsum_gather(a1, b1, imask, part, g1);
+    rowsum_reduce(a0, b0, g0, ib, trow[t], trow[t + 1], lane, stg_all[wid], y);
+    if (two) rowsum_reduce(a1, b1, g1, ib, trow[t + 1], trow[t + 2], lane, stg_all[wid], y);
 }
 
 // rows with more partials than a row-sum tile holds (their tile entry is a
@@ -406,8 +411,8 @@ hipError_t launch_panel(const PanelView& v, const void* x, void* part, void* y, 
         // two tiles per wave, four waves per block, a multiple of 8 blocks
         int64_t g = ((v.n_rtiles + 1) / 2 + 3) / 4;
         g = (g + 7) / 8 * 8;
-        panel_rowsum_kernel<<<(unsigned)g, 256, 0, s>>>(v.items, v.trow, v.n_rtiles, static_cast<const float*>(part),
-                                                        static_cast<float*>(y));
+        panel_rowsum_kernel<<<(unsigned)g, 256, 0, s>>>(v.items, v.trow, v.n_rtiles, v.rs_ib,
+                                                        static_cast<const float*>(part), static_cast<float*>(y));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -583,21 +588,21 @@ __device__ __forceinline__ int64_t rowsum_slot(int64_t q)
     return tb + (int64_t)((jj / 4) * 64 + l) * 4 + jj % 4;
 }
 
-__global__ void pn_rowsum_scatter_kernel(int64_t n_rows, const int32_t* __restrict__ rs,
+__global__ void pn_rowsum_scatter_kernel(int64_t n_rows, int ib, const int32_t* __restrict__ rs,
                                          const int32_t* __restrict__ gidx, const int32_t* __restrict__ rpos,
                                          const int32_t* __restrict__ rcnt, int32_t* __restrict__ items)
 {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = rcnt[r];
-        if (c == 0) continue;   // an empty row closed by an earlier marker
+        if (c == 0) continue;   // an empty row closed by an earlier end
         const int32_t b = rs[r], len = rs[r + 1] - b;
         const int64_t q = rpos[r];
-        if (len + 1 > kRowSumTile) {   // a long row: its own kernel adds it
+        if (len == 0 || len > kRowSumTile) {   // no partials, or a long row its own kernel adds
             items[rowsum_slot(q)] = -c;
             continue;
         }
-        for (int32_t i = 0; i < len; ++i) items[rowsum_slot(q + i)] = gidx[b + i];
-        items[rowsum_slot(q + len)] = -c;
+        for (int32_t i = 0; i + 1 < len; ++i) items[rowsum_slot(q + i)] = gidx[b + i];
+        items[rowsum_slot(q + len - 1)] = (int32_t)((uint32_t)gidx[b + len - 1] | ((uint32_t)c << ib));
     }
 }
 
@@ -699,7 +704,7 @@ hipError_t panel_gidx(int64_t n_seg, int tile, const int32_t* pos, const int32_t
     return hipGetLastError();
 }
 
-hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs, const int32_t* gidx,
+hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, int ib, const int32_t* rs, const int32_t* gidx,
                               const int32_t* rpos, const int32_t* rcnt, int32_t* items, hipStream_t s)
 {
     if (n_items > 0) {
@@ -708,7 +713,7 @@ hipError_t panel_rowsum_build(int64_t n_rows, int64_t n_items, const int32_t* rs
         if (e != hipSuccess) return e;
     }
     if (n_rows <= 0) return hipSuccess;
-    pn_rowsum_scatter_kernel<<<grid_of(n_rows), 256, 0, s>>>(n_rows, rs, gidx, rpos, rcnt, items);
+    pn_rowsum_scatter_kernel<<<grid_of(n_rows), 256, 0, s>>>(n_rows, ib, rs, gidx, rpos, rcnt, items);
     return hipGetLastError();
 }
 

@@ -276,32 +276,42 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     MR_HIP_TRY(k::panel_gidx(n_seg, tile, pos.as<int32_t>(), k0.as<int32_t>(), goff.as<int32_t>(), pidx.as<int32_t>(),
                              p.pgidx.as<int32_t>(), s));
     // 6. the row-sum stream.  Rows are packed in order into tiles of 512 items
-    //    and at most 512 rows: a row takes its partials plus an end marker and
-    //    moves to the next tile when either does not fit; an empty row after
-    //    an end marker in the same tile adds one to that marker's count
-    //    instead; a row with more partials than a tile holds takes a bare end
-    //    marker and is listed as long.
+    //    and at most 1024 rows: a row takes its partials (the last one carries
+    //    the end) and moves to the next tile when they or the row do not fit;
+    //    an empty row after an end in the same tile adds one to that end's
+    //    count instead (up to what the bits above the partial index hold); a
+    //    row with more partials than a tile holds, or an empty row no end
+    //    absorbs, takes a marker, and the long rows are listed.
     std::vector<int32_t> hrs((size_t)n + 1), rpos((size_t)std::max<int64_t>(n, 1)),
         rcnt((size_t)std::max<int64_t>(n, 1), 0), trow, lrow;
     MR_HIP_TRY(hipMemcpyAsync(hrs.data(), p.prs.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
     MR_HIP_TRY(hipStreamSynchronize(s));
-    constexpr int64_t RT = k::kRowSumTile;
+    constexpr int64_t RT = k::kRowSumTile, RR = k::kRowSumRows;
+    int ib = 1;   // partial index width
+    while (((int64_t)1 << ib) < n_parts) ++ib;
+    if (ib > 30)
+        return set_error(MR_ERR_UNSUPPORTED, "column panels: %lld partials exceed the row-sum stream's index",
+                         (long long)n_parts);
+    const int64_t cap_part = std::min<int64_t>(RR, ((int64_t)1 << (31 - ib)) - 1);   // rows one partial can end
     int64_t cur = 0, rt = 0, rows_in = 0;
-    int64_t last_end = -1;   // row that owns the tile's last end marker (-1: none)
+    int64_t last_end = -1;      // row that owns the tile's last end (-1: none)
+    int64_t last_cap = 0;       // how many rows that end may close
     if (n > 0) trow.push_back(0);
     for (int64_t r = 0; r < n; ++r) {
         const int64_t len = (int64_t)hrs[(size_t)r + 1] - hrs[(size_t)r];
-        if (len == 0 && last_end >= 0 && rows_in < RT && rcnt[(size_t)last_end] < RT) {
+        if (len == 0 && last_end >= 0 && rows_in < RR && rcnt[(size_t)last_end] < last_cap) {
             ++rcnt[(size_t)last_end];
             ++rows_in;
             continue;
         }
-        int64_t L = len + 1;
-        if (L > RT) {
-            lrow.push_back((int32_t)r);
+        int64_t L = len;
+        bool part_end = len > 0;
+        if (len == 0 || len > RT) {
+            if (len > RT) lrow.push_back((int32_t)r);
             L = 1;
+            part_end = false;
         }
-        if (cur + L > RT || rows_in >= RT) {
+        if (cur + L > RT || rows_in >= RR) {
             ++rt;
             cur = 0;
             rows_in = 0;
@@ -310,6 +320,7 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
         rpos[(size_t)r] = (int32_t)(rt * RT + cur);
         rcnt[(size_t)r] = 1;
         last_end = r;
+        last_cap = part_end ? cap_part : RR;
         cur += L;
         ++rows_in;
         if (rt * RT + cur >= INT32_MAX)
@@ -326,7 +337,7 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
         MR_HIP_TRY(hipMemcpyAsync(d_rpos.p, rpos.data(), sizeof(int32_t) * rpos.size(), hipMemcpyHostToDevice, s));
         MR_HIP_TRY(hipMemcpyAsync(d_rcnt.p, rcnt.data(), sizeof(int32_t) * rcnt.size(), hipMemcpyHostToDevice, s));
         MR_HIP_TRY(hipMemcpyAsync(p.ptrow.p, trow.data(), sizeof(int32_t) * trow.size(), hipMemcpyHostToDevice, s));
-        MR_HIP_TRY(k::panel_rowsum_build(n, n_rtiles * RT, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
+        MR_HIP_TRY(k::panel_rowsum_build(n, n_rtiles * RT, ib, p.prs.as<int32_t>(), p.pgidx.as<int32_t>(),
                                          d_rpos.as<int32_t>(), d_rcnt.as<int32_t>(), p.prsi.as<int32_t>(), s));
         MR_HIP_TRY(hipStreamSynchronize(s));   // the staging buffers and host vectors are freed on return
     }
@@ -349,6 +360,7 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     p.pn_wg = (int)jt.wg.size() - 1;
     p.pn_jobs = (int)(jt.jobs.size() / 3);
     p.prtiles = n_rtiles;
+    p.prs_ib = ib;
     p.pn_long = (int)lrow.size();
     p.pn_built = true;
     p.pn_build_ms =
@@ -376,6 +388,7 @@ k::PanelView panel_view(mr_mat m)
     v.n_cols = m->n_cols;
     v.n_rows = m->n_rows;
     v.items = p.prsi.as<int32_t>();
+    v.rs_ib = p.prs_ib;
     v.trow = p.ptrow.as<int32_t>();
     v.n_rtiles = p.prtiles;
     v.lrow = p.plrow.as<int32_t>();
