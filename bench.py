@@ -54,7 +54,8 @@ TRAFFIC_PROFILES = {"panel": ROOT / "profiles" / "merbit_panel_s22_fp32_pmc.md",
                     "lane_padded": ROOT / "profiles" / "merbit_lp_s22_fp32_pmc.md"}
 _KERNELS = ROOT / "matrix-reorder_amd" / "csrc"
 # the sources each summary measured (their hash is recorded in the summary)
-TRAFFIC_SOURCES = {"panel": [_KERNELS / "kernels" / "merbit_panel.hip", _KERNELS / "panel_plan.cpp"],
+TRAFFIC_SOURCES = {"panel": [_KERNELS / "kernels" / "merbit_panel.hip", _KERNELS / "kernels" / "launch_panel.hpp",
+                             _KERNELS / "kernels" / "ldst.hpp", _KERNELS / "panel_plan.cpp"],
                    "lane_padded": [_KERNELS / "kernels" / "merbit_lp.hip"]}
 
 
