@@ -275,12 +275,16 @@ __device__ __forceinline__ int find_job(const int64_t* s_begin, int n_jobs, int6
 template <typename VT, int S, bool NT>
 __global__ __launch_bounds__(kLanes) void merbit_batch_kernel(const BatchJobT<VT>* __restrict__ jobs,
                                                               const int64_t* __restrict__ begin, int n_jobs,
-                                                              int64_t total)
+                                                              int64_t total, int chunk)
 {
     __shared__ int64_t s_begin[kMaxJobs + 1];
     for (int i = threadIdx.x; i <= n_jobs; i += kLanes) s_begin[i] = begin[i];
     __syncthreads();
-    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    // one tile per block (chunk > 0): tiles dealt to the XCDs in chunks of
+    // consecutive tiles; else a grid-stride loop
+    const int64_t g0 = chunk > 0 ? xcd_chunk_tile(blockIdx.x, chunk) : (int64_t)blockIdx.x;
+    const int64_t step = chunk > 0 ? total : (int64_t)gridDim.x;
+    for (int64_t g = g0; g < total; g += step) {
         const int j = find_job(s_begin, n_jobs, g);
         const BatchJobT<VT> J = jobs[j];
         merbit_block<VT, S, NT>(g - s_begin[j], J.ci, J.val, J.flags, J.blk_x, J.blk_y, J.x, J.y, J.carry,
@@ -345,7 +349,9 @@ template <typename VT, int S, bool NT>
 hipError_t launch_batch(const void* jobs, const int64_t* begin, int n_jobs, int64_t total, int grid, hipStream_t s)
 {
     const BatchJobT<VT>* J = static_cast<const BatchJobT<VT>*>(jobs);
-    merbit_batch_kernel<VT, S, NT><<<(unsigned)grid, kLanes, 0, s>>>(J, begin, n_jobs, total);
+    const int chunk = grid >= total ? kBatchChunk : 0;
+    const int64_t g = chunk > 0 ? xcd_chunk_grid(total, chunk) : grid;
+    merbit_batch_kernel<VT, S, NT><<<(unsigned)g, kLanes, 0, s>>>(J, begin, n_jobs, total, chunk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     merbit_batch_fixup_kernel<VT><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(J, begin, n_jobs, total);

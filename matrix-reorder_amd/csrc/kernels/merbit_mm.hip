@@ -17,6 +17,7 @@
 // bit-reproducible.
 #include "dtypes.hpp"
 #include "launch.hpp"
+#include "ldst.hpp"
 
 namespace mr {
 namespace k {
@@ -175,12 +176,16 @@ __device__ __forceinline__ int find_job_mm(const int64_t* s_begin, int n_jobs, i
 template <typename VT, int G>
 __global__ __launch_bounds__(kLanes) void merbit_mm_batch_kernel(const MmJobT<VT>* __restrict__ jobs,
                                                                  const int64_t* __restrict__ begin, int n_jobs,
-                                                                 int64_t total)
+                                                                 int64_t total, int chunk)
 {
     __shared__ int64_t s_begin[kMaxBatchJobs + 1];
     for (int i = threadIdx.x; i <= n_jobs; i += kLanes) s_begin[i] = begin[i];
     __syncthreads();
-    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    // one tile per block (chunk > 0): tiles dealt to the XCDs in chunks of
+    // consecutive tiles; else a grid-stride loop
+    const int64_t g0 = chunk > 0 ? xcd_chunk_tile(blockIdx.x, chunk) : (int64_t)blockIdx.x;
+    const int64_t step = chunk > 0 ? total : (int64_t)gridDim.x;
+    for (int64_t g = g0; g < total; g += step) {
         const int j = find_job_mm(s_begin, n_jobs, g);
         const MmJobT<VT> J = jobs[j];
         merbit_mm_block<VT, G>(g - s_begin[j], J.ci, J.val, J.flags, J.blk_x, J.blk_y, J.x, J.ldx, J.y, J.ldy, J.kk,
@@ -261,7 +266,9 @@ hipError_t merbit_batch_spmm(int dt, int kk, const void* jobs, const int64_t* be
         auto go = [&](auto gtag) -> hipError_t {
             constexpr int G = decltype(gtag)::value;
             const MmJobT<VT>* J = static_cast<const MmJobT<VT>*>(jobs);
-            merbit_mm_batch_kernel<VT, G><<<(unsigned)grid, kLanes, 0, s>>>(J, begin, n_jobs, total);
+            const int chunk = grid >= total ? kBatchChunk : 0;
+            const int64_t gr = chunk > 0 ? xcd_chunk_grid(total, chunk) : grid;
+            merbit_mm_batch_kernel<VT, G><<<(unsigned)gr, kLanes, 0, s>>>(J, begin, n_jobs, total, chunk);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
             const int64_t threads = total * G;
