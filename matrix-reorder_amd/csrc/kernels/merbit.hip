@@ -39,6 +39,7 @@
 #include "dtypes.hpp"
 #include "launch.hpp"
 #include "ldst.hpp"
+#include "xcd_order.hpp"
 
 namespace mr {
 namespace k {

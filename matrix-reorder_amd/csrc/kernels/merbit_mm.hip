@@ -17,7 +17,7 @@
 // bit-reproducible.
 #include "dtypes.hpp"
 #include "launch.hpp"
-#include "ldst.hpp"
+#include "xcd_order.hpp"
 
 namespace mr {
 namespace k {
