@@ -132,9 +132,19 @@ def corpus():
     return out
 
 
+def sweep_corpus():
+    """The reference's published R-MAT s24 sweep (config/config3.txt ids
+    1-75, symmetric, the same generator as mrpy/suite.py with seed = id)."""
+    pub = json.loads((pathlib.Path(__file__).resolve().parent.parent / "suite" / "rmat_s24_published.json")
+                     .read_text())["matrices"]
+    return [(f"sweep_s24_id{m['id']}", "rmat", dict(scale=m["scale"], ef=m["ef"], abc=(m["a"], m["b"], m["b"]),
+                                                    sym=True, seed=m["id"]))
+            for m in pub]
+
+
 def build(mr, family: str, spec: dict, seed: int):
     if family == "rmat":
-        return mr.gen_rmat(spec["scale"], spec["ef"], spec["abc"], seed=seed, symmetric=spec["sym"])
+        return mr.gen_rmat(spec["scale"], spec["ef"], spec["abc"], seed=spec.get("seed", seed), symmetric=spec["sym"])
     if family == "banded":
         return mr.gen_banded(spec["n"], spec["bw"], spec["per_row"], seed)
     if family == "stencil":
@@ -161,7 +171,9 @@ def _time(ctx, mat, xd, yd, kernel, reps, warmup):
     return best * 1e3            # microseconds
 
 
-def collect(out_path: str, limit: int = 0, reps: int = 20, skip: int = 0) -> None:
+def collect(out_path: str, limit: int = 0, reps: int = 20, skip: int = 0, sweep: bool = False) -> None:
+    """Measure the corpus (or, with sweep, the published s24 sweep: the
+    generated and the pGOrder orders of each matrix only)."""
     import mrpy as mr
 
     ctx = mr.Ctx(0)
@@ -172,7 +184,7 @@ def collect(out_path: str, limit: int = 0, reps: int = 20, skip: int = 0) -> Non
             + [f"r{i + 1}" for i in range(FEATURES)] + [f"q{i + 1}" for i in range(FEATURES)])
     path = pathlib.Path(out_path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    items = list(enumerate(corpus()))[skip:]
+    items = list(enumerate(sweep_corpus() if sweep else corpus()))[skip:]
     if limit:
         items = items[:limit]
     t_start = time.perf_counter()
@@ -196,22 +208,24 @@ def collect(out_path: str, limit: int = 0, reps: int = 20, skip: int = 0) -> Non
             # other orders, each given as a relabelling of the original
             # matrix (perm[old] = new)
             rng = np.random.default_rng(seed + 3)
-            variants = [("pgorder", pg), ("original", None), ("random", rng.permutation(n).astype(np.int32))]
-            deg = np.diff(rp)
-            order = np.argsort(-deg, kind="stable")
-            dperm = np.empty(n, np.int32)
-            dperm[order] = np.arange(n, dtype=np.int32)
-            variants.append(("degree", dperm))
-            m1 = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
-            rperm, _ = m1.reorder_rcm()
-            m1.close()
-            variants.append(("rcm", rperm))
-            for pi, (rho, near, win) in enumerate(PARA):
-                ls = local_swap(n, rho, near, win, seed * 16 + pi)
-                # perturb the pGOrder order: old -> pGOrder position -> swapped
-                inv_ls = np.empty(n, np.int32)
-                inv_ls[ls] = np.arange(n, dtype=np.int32)
-                variants.append((f"swap_{rho}_{near}_{win}", inv_ls[pg]))
+            variants = [("pgorder", pg), ("original", None)]
+            if not sweep:
+                variants.append(("random", rng.permutation(n).astype(np.int32)))
+                deg = np.diff(rp)
+                order = np.argsort(-deg, kind="stable")
+                dperm = np.empty(n, np.int32)
+                dperm[order] = np.arange(n, dtype=np.int32)
+                variants.append(("degree", dperm))
+                m1 = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+                rperm, _ = m1.reorder_rcm()
+                m1.close()
+                variants.append(("rcm", rperm))
+                for pi, (rho, near, win) in enumerate(PARA):
+                    ls = local_swap(n, rho, near, win, seed * 16 + pi)
+                    # perturb the pGOrder order: old -> pGOrder position -> swapped
+                    inv_ls = np.empty(n, np.int32)
+                    inv_ls[ls] = np.arange(n, dtype=np.int32)
+                    variants.append((f"swap_{rho}_{near}_{win}", inv_ls[pg]))
             t_pg = None
             for vname, perm in variants:
                 m = mr.Mat.from_csr(ctx, rp, ci, val, store=mr.F32)
@@ -370,6 +384,8 @@ def main(argv=None) -> int:
     c.add_argument("--limit", type=int, default=0, help="first N matrices of the corpus only")
     c.add_argument("--reps", type=int, default=20)
     c.add_argument("--skip", type=int, default=0, help="start at this corpus index (seeds stay per index)")
+    c.add_argument("--sweep", action="store_true",
+                   help="the published R-MAT s24 sweep (generated and pGOrder orders) instead of the corpus")
     f = sub.add_parser("fit", help="fit the models on the CPU")
     f.add_argument("--csv", required=True)
     f.add_argument("--header", default="")
@@ -377,7 +393,7 @@ def main(argv=None) -> int:
     f.add_argument("--alpha", type=float, default=1.0)
     a = ap.parse_args(argv)
     if a.cmd == "collect":
-        collect(a.out, a.limit, a.reps, a.skip)
+        collect(a.out, a.limit, a.reps, a.skip, a.sweep)
         return 0
     report, chosen = fit_shared(a.csv, a.alpha)
     if a.header:
