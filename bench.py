@@ -165,7 +165,7 @@ def section_main(mr, ctx, a, extra):
         # the same prediction from the model refitted on MI355X timings
         pred_mi, feat_mi, _ = mat.locality_predict(mr.MODEL_MERBIT_MI355X)
         extra["rd_predicted_speedup_mi355x_model"] = pred_mi
-        extra["rd_features_mi355x_edges"] = [round(float(v), 6) for v in feat_mi]
+        extra["rd_features_mi355x_model"] = [round(float(v), 6) for v in feat_mi]
         log(f"  {extra['merbit_unreordered_ms'] * 1e3:.1f} us/SpMV, predicted pGOrder speedup {pred:.3f} "
             f"(sm_89 model), {pred_mi:.3f} (MI355X model)")
     perm_path = pathlib.Path(a.perm_cache) if a.perm_cache else None

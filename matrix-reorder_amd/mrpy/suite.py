@@ -90,7 +90,7 @@ def run(ids, out_path: str, width: int = 0) -> list:
         rec["predicted_speedup_sm89_model"] = sp
         rec["predicted_speedup_mi355x_model"] = sp_mi
         rec["rd_features"] = [float(v) for v in f_ref]
-        rec["rd_features_mi355x_edges"] = [float(v) for v in f_mi]
+        rec["rd_features_mi355x_model"] = [float(v) for v in f_mi]   # under that model's bin edges
         log(f"id {gid}: a={p['a']} ef={p['ef']} nnz={nnz} origin {rec['origin_ms'] * 1e3:.0f} us")
         t = time.perf_counter()
         _, st = mat.reorder_pgorder(width, 5)
