@@ -227,14 +227,16 @@ int merbit_panel_build(mr_mat m, hipStream_t s)
     MR_TRY(p.ptseg.alloc(sizeof(int32_t) * (size_t)(tiles + 1)));
     MR_HIP_TRY(k::panel_tile_base(tiles, tile, pidx.as<int32_t>(), p.ptseg.as<int32_t>(), s));
     MR_HIP_TRY(k::panel_set_tail(p.ptseg.as<int32_t>(), tiles, nullptr, n_parts, s));
-    // jobs, from per-tile costs.  The costs were fitted to per-workgroup clock
-    // stamps of the panel kernel at R-MAT s22 (FP32): 0.14 us per tile of
-    // 3 KB, 0.15 ns per partial sum the tile writes and 5.9 us per job (the x
-    // panel load of the first job and the pipeline ramp; later panels are
-    // prefetched into registers).  In bytes: a partial costs 3.3 B, a job the
-    // panel's x bytes.  Partials vary inside a panel: its first rows (the long
-    // ones after a locality order) write few, its last rows about one per item.
-    constexpr double kPartialBytes = 3.3, kJobLoad = 1.0;
+    // jobs, from per-tile costs.  A linear fit to per-workgroup clock stamps
+    // of the panel kernel at R-MAT s22 (FP32) gives 0.168 us per tile of 3 KB,
+    // 0.22 ns per partial sum the tile writes and 6.0 us per job (the x panel
+    // load of the first job and the pipeline ramp; later panels are
+    // prefetched into registers), 1.3 us residual.  In bytes: a partial costs
+    // about 4 B, a job about the panel's x bytes; SpMV times over a grid of
+    // both settled on 5 B and 1 panel (F32 and BF16 alike).  Partials vary
+    // inside a panel: its first rows (the long ones after a locality order)
+    // write few, its last rows about one per item.
+    constexpr double kPartialBytes = 5.0, kJobLoad = 1.0;
     const double tile_bytes = (double)tile * (2.0 + (double)dtype_size(m->dtype));
     std::vector<int32_t> h_tseg((size_t)tiles + 1);
     MR_HIP_TRY(hipMemcpyAsync(h_tseg.data(), p.ptseg.p, sizeof(int32_t) * h_tseg.size(), hipMemcpyDeviceToHost, s));
