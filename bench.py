@@ -98,6 +98,21 @@ def time_spmv(ctx, mr, mat, xd, yd, kernel, steps, warmup, stream=None):
     return ctx.elapsed(e0, e1) / steps, wall / steps
 
 
+def time_per_launch(ctx, mat, xd, yd, kernel, launches, warmup=10):
+    """ms of each of `launches` SpMVs, each bracketed by its own pair of
+    hipEvents on one stream."""
+    s = ctx.stream()
+    for _ in range(warmup):
+        mat.spmv(xd, yd, kernel, s)
+    ev = [(ctx.event(), ctx.event()) for _ in range(launches)]
+    for e0, e1 in ev:
+        ctx.record(e0, s)
+        mat.spmv(xd, yd, kernel, s)
+        ctx.record(e1, s)
+    ctx.sync(s)
+    return np.array([ctx.elapsed(e0, e1) for e0, e1 in ev])
+
+
 def csr_bytes(n_rows, n_cols, nnz, val_bytes, vec_bytes=4, k=1):
     """Algorithmic bytes of y = A X in canonical int32 CSR: every array once."""
     return nnz * (val_bytes + 4) + (n_rows + 1) * 4 + k * (n_cols + n_rows) * vec_bytes
@@ -202,6 +217,24 @@ def section_main(mr, ctx, a, extra):
     achieved = bytes_alg / (ms * 1e-3) / 1e9
     log(f"  {ms * 1e3:.1f} us/SpMV (wall {wall * 1e3:.1f} us), {gflops:.1f} GFLOP/s, {achieved:.0f} GB/s")
     res = {"ms": ms, "gflops": gflops, "achieved": achieved, "bytes": bytes_alg, "n": n, "nnz": nnz, "cfg": cfg}
+    # per-launch statistics: the median of 200 launches, each between its own
+    # events, and the reference's figure, the mean over 400 back-to-back
+    # launches (class.cu:1421-1432)
+    per = time_per_launch(ctx, mat, xp, yd, mr.MERBIT, 200)
+    extra["per_launch_us"] = {"median": float(np.median(per)) * 1e3, "p10": float(np.percentile(per, 10)) * 1e3,
+                              "p90": float(np.percentile(per, 90)) * 1e3, "launches": int(len(per))}
+    m400, _ = time_spmv(ctx, mr, mat, xp, yd, mr.MERBIT, 400, 10)
+    extra["mean_us_over_400"] = m400 * 1e3
+    # the stored format's bytes beside the canonical int32-CSR bytes (the
+    # column panels keep 16-bit panel columns, and add partial sums)
+    sb = mat.merbit_stored_bytes()
+    extra["stored_format_bytes"] = sb
+    extra["canonical_csr_bytes"] = bytes_alg
+    extra["stored_format_GBs"] = sb / (ms * 1e-3) / 1e9 if sb > 0 else None
+    log(f"  per launch: median {extra['per_launch_us']['median']:.1f} us (p10 {extra['per_launch_us']['p10']:.1f}, "
+        f"p90 {extra['per_launch_us']['p90']:.1f}) over 200; mean over 400: {m400 * 1e3:.1f} us")
+    log(f"  stored format {sb / 1e6:.1f} MB per SpMV ({extra['stored_format_GBs'] or 0:.0f} GB/s) vs canonical CSR "
+        f"{bytes_alg / 1e6:.1f} MB")
     # copy-kernel reference of the same session (measured bandwidth; also the
     # calibration kernel of the rocprofv3 byte counters)
     nb = 1 << 30

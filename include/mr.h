@@ -356,6 +356,14 @@ int mr_merbit_panel_configure(mr_mat m, int items_per_lane, int waves);
 /* tiles, partials, segments and workgroups are -1 before the plan is built */
 int mr_merbit_panel_info(mr_mat m, int* items_per_lane, int* waves, int64_t* tiles, int64_t* partials,
                          int64_t* segments, int* workgroups, double* build_ms);
+/* Bytes one SpMV moves in the stored format of the MERBIT variant the current
+ * policy selects, each array counted once: the streams and metadata the
+ * kernels read, the partial sums or carries they write and read back, x and
+ * y.  No reference counterpart: the reference times its kernel only.  For
+ * comparing with the canonical int32-CSR bytes (nnz * (4 + |val|) + (n + 1) *
+ * 4 + n_cols * |x| + n_rows * |y|) where a variant compresses its indices.
+ * -1 before that variant's plan is built. */
+int mr_merbit_stored_bytes(mr_mat m, int64_t* bytes);
 int mr_merbit_info(mr_mat m, int* sigma, int64_t* items, int64_t* n_blocks, double* build_ms);
 /* row-end bitmap over the merge path (bit p of word p/32 set when item p ends a
  * row) and the merge coordinates at every block start (+ sentinel) */

@@ -991,6 +991,35 @@ int mr_merbit_panel_info(mr_mat m, int* items_per_lane, int* waves, int64_t* til
     return MR_OK;
 }
 
+int mr_merbit_stored_bytes(mr_mat m, int64_t* bytes)
+{
+    if (!m || !bytes) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    const MerbitPlan& p = m->merbit;
+    const int64_t dsz = (int64_t)dtype_size(m->dtype);
+    const int64_t vec = m->dtype == MR_F64 ? 8 : 4;   // x / y elements
+    const int64_t xy = (m->n_cols + m->n_rows) * vec;
+    *bytes = -1;
+    if (p.policy & 8) {
+        if (!p.pn_built) return MR_OK;
+        const int64_t tile = 64LL * p.ps;
+        // item streams and tile bases; partials written, then gathered;
+        // row-sum stream and tile rows
+        *bytes = p.ptiles * tile * (2 + dsz) + p.ptiles * 4 + 2 * p.pparts * 4 +
+                 p.prtiles * k::kRowSumTile * 4 + (p.prtiles + 1) * 4 + xy;
+    } else if (p.policy & 4) {
+        if (!p.xc_built) return MR_OK;
+        // padded (column, value) streams, rows ended before every tile, the
+        // cached column ids, carries written and read back
+        *bytes = p.xtiles * 64LL * p.xs * (4 + dsz) + p.xtiles * 4 + (int64_t)p.xc * 4 + 2 * p.xtiles * (vec + 4) + xy;
+    } else {
+        if (!p.built) return MR_OK;
+        // columns and values, the row-end bitmap, block coordinates, carries
+        *bytes = m->nnz * (4 + dsz) + p.n_blocks * 256LL * p.sigma / 8 + (p.n_blocks + 1) * 8 +
+                 2 * p.n_blocks * (vec + 4) + xy;
+    }
+    return MR_OK;
+}
+
 int mr_merbit_xc_info(mr_mat m, int* items_per_lane, int* waves, int* cached_cols, int64_t* tiles)
 {
     if (!m) return set_error(MR_ERR_INVALID_ARG, "null matrix handle");

@@ -169,6 +169,7 @@ SIGNATURES = {
     "mr_merbit_panel_configure": [_P, C.c_int, C.c_int],
     "mr_merbit_panel_info": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64P, _I64P, _I64P, C.POINTER(C.c_int),
                              _DP],
+    "mr_merbit_stored_bytes": [_P, _I64P],
     "mr_merbit_xc_info": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _I64P],
     "mr_batch_create": [_P, C.c_int, C.POINTER(_P), C.c_int, C.POINTER(_P)],
     "mr_batch_destroy": [_P],
@@ -751,6 +752,13 @@ class Mat:
                                          C.byref(wg), C.byref(ms)))
         return {"items_per_lane": s.value, "waves": w.value, "tiles": t.value, "partials": pa.value,
                 "segments": sg.value, "workgroups": wg.value, "build_ms": ms.value}
+
+    def merbit_stored_bytes(self) -> int:
+        """Bytes one SpMV moves in the current variant's stored format (-1
+        before its plan is built)."""
+        b = C.c_int64(-1)
+        check(lib().mr_merbit_stored_bytes(self.h, C.byref(b)))
+        return b.value
 
     def merbit_info(self):
         s, it, nb, ms = C.c_int(), C.c_int64(), C.c_int64(), C.c_double()

@@ -245,3 +245,35 @@ def test_panel_unaligned_x_and_y(ctx, mr, name, store):
     yu = yb.to_host()[1:]
     assert np.array_equal(yu.view(np.uint32), y.view(np.uint32))
     mat.close()
+
+
+def test_stored_bytes_of_each_variant(ctx, mr):
+    """mr_merbit_stored_bytes: -1 before a plan exists; for each variant the
+    count its plan implies (column panels: item streams, tile bases, partials
+    written and read back, the row-sum stream of whole 512-item tiles and its
+    tile rows, x and y)."""
+    _, rp, ci, nc = next(c for c in CASES if c[0] == "random_wide")
+    n, nnz = len(rp) - 1, len(ci)
+    rng = np.random.default_rng(21)
+    mat = mr.Mat.from_csr(ctx, rp, ci, rng.uniform(-1, 1, nnz), n_cols=nc, store=mr.F32)
+    xy = (nc + n) * 4
+    mat.merbit_options(9)
+    assert mat.merbit_stored_bytes() == -1
+    x = rng.uniform(-1, 1, nc).astype(np.float32)
+    run(ctx, mat, x)
+    info = mat.merbit_panel_info()
+    base = info["tiles"] * 64 * info["items_per_lane"] * 6 + info["tiles"] * 4 + 2 * info["partials"] * 4 + xy
+    rest = mat.merbit_stored_bytes() - base            # row-sum tiles * 2048 + (tiles + 1) * 4
+    assert rest >= 2052 and (rest - 4) % 2052 == 0, rest
+    assert (rest - 4) // 2052 >= -(-(info["partials"]) // 512)
+    mat.merbit_options(1)
+    run(ctx, mat, x)
+    wi = mat.merbit_info()
+    assert mat.merbit_stored_bytes() == (nnz * 8 + wi["n_blocks"] * 256 * wi["sigma"] // 8 + (wi["n_blocks"] + 1) * 8
+                                         + 2 * wi["n_blocks"] * 8 + xy)
+    mat.merbit_options(5)
+    run(ctx, mat, x)
+    xi = mat.merbit_xc_info()
+    assert mat.merbit_stored_bytes() == (xi["tiles"] * 64 * xi["items_per_lane"] * 8 + xi["tiles"] * 4
+                                         + xi["cached_cols"] * 4 + 2 * xi["tiles"] * 8 + xy)
+    mat.close()
