@@ -183,6 +183,19 @@ def section_main(mr, ctx, a, extra):
         extra["rd_features_mi355x_model"] = [round(float(v), 6) for v in feat_mi]
         log(f"  {extra['merbit_unreordered_ms'] * 1e3:.1f} us/SpMV, predicted pGOrder speedup {pred:.3f} "
             f"(sm_89 model), {pred_mi:.3f} (MI355X model)")
+        # pGOrder's CPU phase apart from the GPU loop it overlaps: order_on_cpu
+        # on every slice on all host threads, the median of three runs
+        gpu_order, final_order, _ = mat.pgorder_order(0, 5)
+        runs = []
+        for _ in range(3):
+            fo, ms = mat.pgorder_cpu_phase(gpu_order, 0, 5, 0)
+            runs.append(ms)
+        if not np.array_equal(fo, final_order):
+            raise RuntimeError("the CPU phase alone differs from pGOrder's own")
+        extra["pgorder_cpu_phase_alone"] = {"median_ms": float(np.median(runs)), "runs_ms": [round(r, 1) for r in runs],
+                                            "threads": host_cores()}
+        log(f"  pGOrder CPU phase alone (order_on_cpu, {host_cores()} threads): median {np.median(runs):.0f} ms")
+        del gpu_order, final_order, fo
     perm_path = pathlib.Path(a.perm_cache) if a.perm_cache else None
     if perm_path is not None and perm_path.exists():
         log(f"relabelling with the cached pGOrder permutation {perm_path}")

@@ -185,6 +185,14 @@ int mr_reorder_pgorder(mr_mat m, int width, int window, int32_t* perm_out, mr_re
  * phase (class.cu:3093); either may be NULL */
 int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint32_t* final_order,
                      mr_reorder_stats* stats);
+/* gorder6's CPU phase alone (order_on_cpu, class.cu:2220-2314, on every slice
+ * from batch log2(width), class.cu:3074-3093, on a pool of `threads` host
+ * threads as the reference's ThreadPool(128); 0 = all the process may use):
+ * final_order = gpu_order refined, as mr_pgorder_order's final_order; *ms =
+ * the phase's wall time without the host copies of the lists.  For timing the
+ * phase apart from the GPU loop it overlaps in mr_reorder_pgorder. */
+int mr_pgorder_cpu_phase(mr_mat m, int width, int window, const uint32_t* gpu_order, uint32_t* final_order,
+                         int threads, double* ms);
 /* GRAPH::gorder5 (class.cu:2727-2873), the earlier pGOrder schedule: batches
  * of vsize (0 = get_width; at most 2^22), start scores = column counts, the
  * GPU window = the last width1 batches (width1 * vsize <= 2^24) with no hub

@@ -612,6 +612,68 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
     return MR_OK;
 }
 
+int mr_pgorder_cpu_phase(mr_mat m, int width, int window, const uint32_t* gpu_order, uint32_t* final_order,
+                         int threads, double* ms)
+{
+    if (!m || !gpu_order || !final_order) return set_error(MR_ERR_INVALID_ARG, "null argument");
+    if (m->n_rows != m->n_cols) return set_error(MR_ERR_UNSUPPORTED, "pGOrder needs a square matrix");
+    if (window < 1) return set_error(MR_ERR_INVALID_ARG, "window must be >= 1");
+    MR_HIP_TRY(hipSetDevice(m->ctx->device));
+    const int64_t n64 = m->n_rows, nnz = m->nnz;
+    if (ms) *ms = 0.0;
+    if (n64 == 0) return MR_OK;
+    int W = width;
+    if (W <= 0) MR_TRY(mr_get_width(n64, nnz, &W));
+    if (W < 32 || W > (1 << 22) || (W & (W - 1)))
+        return set_error(MR_ERR_UNSUPPORTED, "pGOrder supports power-of-two widths 32..2^22 (got %d)", W);
+    const int32_t n = (int32_t)n64;
+    std::vector<int32_t> Ps, Pl, Os, Ol;
+    const int32_t B = pg_schedule(n, W, Ps, Pl, Os, Ol);
+    for (int32_t b = 0; b < B; ++b)
+        if (Os[(size_t)b] + Ol[(size_t)b] > n) Ol[(size_t)b] = n - Os[(size_t)b];
+    // the slices gorder6 refines on the host: the windows of every batch from
+    // log2(width) (class.cu:3077-3093), as mr_pgorder_order
+    std::vector<int32_t> Cs, Cl;
+    for (int32_t b = (int32_t)std::log2((double)W); b < B; ++b) {
+        Cs.push_back(Os[(size_t)b]);
+        Cl.push_back(Ol[(size_t)b]);
+    }
+    // host lists: rows, and the column counts order_on_cpu reads
+    std::vector<int32_t> hrp((size_t)n + 1), hci((size_t)std::max<int64_t>(nnz, 1)), htp((size_t)n + 1, 0);
+    hipStream_t s = m->ctx->stream;
+    MR_HIP_TRY(hipMemcpyAsync(hrp.data(), m->rp.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    if (nnz > 0)
+        MR_HIP_TRY(hipMemcpyAsync(hci.data(), m->ci.p, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s));
+    MR_HIP_TRY(hipStreamSynchronize(s));
+    for (int64_t a = 0; a < nnz; ++a) ++htp[(size_t)hci[(size_t)a] + 1];
+    for (int32_t v = 0; v < n; ++v) htp[(size_t)v + 1] += htp[(size_t)v];
+    std::memcpy(final_order, gpu_order, sizeof(uint32_t) * (size_t)n);
+    std::vector<int32_t> local_idx((size_t)n, -1);
+    // ti is not read by order_batch (only the column counts are)
+    PgCpuShared g{hrp.data(), hci.data(), htp.data(), nullptr, local_idx.data(), (int32_t)window,
+                  (int32_t)(std::sqrt((double)n) / 4)};
+    const int32_t C = (int32_t)Cs.size();
+    int nth = threads > 0 ? threads : host_threads();
+    nth = std::max(1, std::min<int>(nth, std::max<int32_t>(C, 1)));
+    std::atomic<int32_t> next{0};
+    const auto t0 = clk::now();
+    auto worker = [&] {
+        PgCpuWorker w;
+        w.yslot.assign((size_t)n, -1);
+        for (int32_t c = next++; c < C; c = next++) {
+            const int32_t cs = Cs[(size_t)c], cl = Cl[(size_t)c];
+            for (int32_t i = 0; i < cl; ++i) local_idx[(size_t)gpu_order[cs + i]] = i;
+            order_batch(g, w, final_order + cs, cl);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nth; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    if (ms) *ms = ms_since(t0);
+    return MR_OK;
+}
+
 int mr_gorder5_order(mr_mat m, int vsize, int width1, int width2, uint32_t* gpu_order, uint32_t* final_order,
                      mr_reorder_stats* stats)
 {

@@ -134,6 +134,7 @@ SIGNATURES = {
     "mr_get_width": [C.c_int64, C.c_int64, C.POINTER(C.c_int)],
     "mr_reorder_pgorder": [_P, C.c_int, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_pgorder_order": [_P, C.c_int, C.c_int, _U32P, _U32P, C.POINTER(ReorderStats)],
+    "mr_pgorder_cpu_phase": [_P, C.c_int, C.c_int, _U32P, _U32P, C.c_int, _DP],
     "mr_reorder_gorder_cpu": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rcm": [_P, _I32P, C.POINTER(ReorderStats)],
     "mr_reorder_rabbit": [_P, C.c_int, _I32P, C.POINTER(ReorderStats)],
@@ -596,6 +597,19 @@ class Mat:
         st = ReorderStats()
         check(lib().mr_pgorder_order(self.h, width, window, ptr(g, C.c_uint32), ptr(f, C.c_uint32), C.byref(st)))
         return g[:n], f[:n], st.as_dict()
+
+    def pgorder_cpu_phase(self, gpu_order: np.ndarray, width: int = 0, window: int = 5, threads: int = 0):
+        """gorder6's CPU phase alone on `gpu_order` (mr_pgorder_order's GPU
+        placement): (final_order, wall ms)."""
+        n = self.info()[0]
+        g = np.ascontiguousarray(gpu_order, np.uint32)
+        if len(g) != n:
+            raise ValueError("gpu_order must hold one entry per row")
+        f = np.empty(max(n, 1), np.uint32)
+        ms = C.c_double()
+        check(lib().mr_pgorder_cpu_phase(self.h, width, window, ptr(g, C.c_uint32), ptr(f, C.c_uint32), threads,
+                                         C.byref(ms)))
+        return f[:n], ms.value
 
     def reorder_gorder5(self, vsize: int = 0, width1: int = 1, width2: int = 5):
         n = self.info()[0]

@@ -193,6 +193,26 @@ def test_pgorder_bit_exact(ctx, mr, oracle, case):
     mat.close()
 
 
+CPU_PHASE_CASES = ("rmat_s12_w128", "tiny_20_w32", "no_edges_5000", "hubs_30000_w1024", "ties_30000_w1024",
+                   "rmat_s20_ef2_width16384", "rmat_s14_w2048", "hubs_30000_w4096")
+
+
+@pytest.mark.parametrize("case", [c for c in pgorder_cases() if c[0] in CPU_PHASE_CASES], ids=lambda c: c[0])
+@pytest.mark.parametrize("threads", [1, 4])
+def test_pgorder_cpu_phase_alone(ctx, mr, oracle, case, threads):
+    """mr_pgorder_cpu_phase on the GPU loop's placement gives
+    mr_pgorder_order's final order, on one thread and on four."""
+    name, rp, ci, W = case
+    mat = mr.Mat.from_csr(ctx, rp, ci, store=mr.F32)
+    gpu_order, final_order, _ = mat.pgorder_order(W, 5)
+    alone, ms = mat.pgorder_cpu_phase(gpu_order, W, 5, threads)
+    assert np.array_equal(alone, final_order)
+    assert ms >= 0.0
+    with pytest.raises(ValueError):
+        mat.pgorder_cpu_phase(gpu_order[:-1], W, 5, threads)
+    mat.close()
+
+
 @pytest.mark.parametrize("scale,ef", [(10, 8), (12, 16)])
 def test_gorder_cpu_bit_exact(ctx, mr, oracle, scale, ef):
     rp, ci = rmat_csr(scale, ef, seed=2)
