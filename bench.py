@@ -23,8 +23,8 @@ Further sections (reported under "extra", each on its own synthetic inputs):
   reference_scale  the shape of the reference's published MERBIT runs (symmetric
              R-MAT s24 ef4, all-ones values and x), unreordered and after
              pGOrder, beside the published sm_89 figures;
-  cpu        the CPU baseline: the oracle's threaded CSR SpMV (FP32) on the
-             same matrix, plus every reorderer on R-MAT s18 -- host
+  cpu        the CPU baseline: the oracle's threaded CSR SpMV (FP32, and FP64
+             beside it) on the same matrix, plus every reorderer on R-MAT s18 -- host
              (sequential Gorder, RCM, Rabbit Order, tile-aware order) and
              device (degree, DBG, gorder5, pGOrder) -- each with the SpMV
              time it yields.
@@ -519,21 +519,35 @@ def section_reference_scale(mr, ctx, a, extra):
 
 
 def section_cpu(mr, ctx, a, data):
-    """The oracle's threaded CSR SpMV (FP32) on the headline matrix, ~10 s."""
+    """The oracle's threaded CSR SpMV on the headline matrix (unreordered), on
+    the host's cores: FP32 (the reported baseline) and FP64, each the mean over
+    up to 400 back-to-back iterations after warm-up (about 10 s in all) and
+    the median of 11 single-iteration runs."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
     rp, ci, val, x = data
     cores = max(1, min(host_cores(), 16))
-    _, ms = oracle.spmv_f32_threads(rp, ci, val, x, cores, 2)
-    per = ms / 2
-    iters = max(3, min(400, int(10.0e3 / max(per, 1e-3))))
-    _, ms = oracle.spmv_f32_threads(rp, ci, val, x, cores, iters)
-    per = ms / iters
-    gflops = 2.0 * len(ci) / (per * 1e-3) / 1e9
-    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+    out = {}
+    for name, fn, vt in (("fp32", oracle.spmv_f32_threads, np.float32), ("fp64", oracle.spmv_f64_threads, np.float64)):
+        v, xx = val.astype(vt), x.astype(vt)
+        _, ms = fn(rp, ci, v, xx, cores, 10)                      # warm-up
+        per = ms / 10
+        iters = max(3, min(400, int(5.0e3 / max(per, 1e-3))))
+        _, ms = fn(rp, ci, v, xx, cores, iters)
+        mean = ms / iters
+        med = float(np.median([fn(rp, ci, v, xx, cores, 1)[1] for _ in range(11)]))
+        out[name] = {"mean_ms": mean, "median_ms": med, "iters": iters,
+                     "gflops": 2.0 * len(ci) / (mean * 1e-3) / 1e9,
+                     "median_gflops": 2.0 * len(ci) / (med * 1e-3) / 1e9}
+    f32 = out["fp32"]
+    return {"value": round(f32["gflops"], 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
             "sample": f"oracle threaded CSR SpMV (FP32, nnz-balanced row ranges) on the unreordered R-MAT s22 "
-                      f"matrix, {iters} iterations, {per:.2f} ms/iter",
+                      f"matrix, mean over {f32['iters']} iterations after 10 warm-ups, {f32['mean_ms']:.2f} ms/iter "
+                      f"(median of 11 single iterations {f32['median_ms']:.2f} ms)",
+            "fp32_median_gflops": round(f32["median_gflops"], 3),
+            "fp64_gflops": round(out["fp64"]["gflops"], 3), "fp64_median_gflops": round(out["fp64"]["median_gflops"], 3),
+            "fp64_iters": out["fp64"]["iters"],
             "cpu_model": cpu_model(), "host_threads": host_cores(),
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
@@ -670,9 +684,10 @@ def main():
         section_cpu_gorder_s22(a, extra)
     cpu = None
     if not a.no_cpu and not a.only_main:
-        log("CPU baseline (oracle threaded CSR SpMV, FP32)")
+        log("CPU baseline (oracle threaded CSR SpMV, FP32 and FP64)")
         cpu = section_cpu(mr, ctx, a, data)
-        log(f"  {cpu['value']} GFLOP/s on {cpu['cores']} cores")
+        log(f"  FP32 {cpu['value']} GFLOP/s (median {cpu['fp32_median_gflops']}), FP64 {cpu['fp64_gflops']} GFLOP/s "
+            f"(median {cpu['fp64_median_gflops']}) on {cpu['cores']} cores")
         if "cpu_reorders" in sections:
             section_cpu_reorders(mr, ctx, extra)
 

@@ -144,6 +144,9 @@ void oracle_spmm_f64(int32_t n_rows, int32_t k, const int32_t* rp, const int32_t
 double oracle_spmv_f32_threads(int32_t n_rows, const int32_t* rp, const int32_t* ci,
                                const float* val, const float* x, float* y,
                                int32_t threads, int32_t iters);
+double oracle_spmv_f64_threads(int32_t n_rows, const int32_t* rp, const int32_t* ci,
+                               const double* val, const double* x, double* y,
+                               int32_t threads, int32_t iters);
 
 /* ---- MERBIT (include/merbit.cuh) ----------------------------------------- */
 /* reference 32-lane tile metadata, u32 descriptors (SIGMA < 15) */

@@ -61,6 +61,7 @@ _SIG = {
     "oracle_spmv_f64": (None, [C.c_int32, _I32P, _I32P, _DP, _DP, _DP, _DP]),
     "oracle_spmm_f64": (None, [C.c_int32, C.c_int32, _I32P, _I32P, _DP, _DP, _DP]),
     "oracle_spmv_f32_threads": (C.c_double, [C.c_int32, _I32P, _I32P, _FP, _FP, _FP, C.c_int32, C.c_int32]),
+    "oracle_spmv_f64_threads": (C.c_double, [C.c_int32, _I32P, _I32P, _DP, _DP, _DP, C.c_int32, C.c_int32]),
     "oracle_merbit_ref_tiles": (C.c_int, [C.c_int32, C.c_int64, _I32P, C.c_int32, _U32P, _U32P, _U32P]),
     "oracle_merge_search": (None, [_I32P, C.c_int32, C.c_int64, C.c_int64, _I64P, _I64P]),
     "oracle_merbit_spmv": (None, [C.c_int32, C.c_int64, _I32P, _I32P, _DP, _DP, C.c_int32, _DP]),
@@ -338,6 +339,18 @@ def spmv_f32_threads(rp, ci, val, x, threads, iters):
     y = np.empty(n, np.float32)
     ms = lib().oracle_spmv_f32_threads(n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(val, C.c_float),
                                        _p(x, C.c_float), _p(y, C.c_float), threads, iters)
+    return y, ms
+
+
+def spmv_f64_threads(rp, ci, val, x, threads, iters):
+    """FP64 twin of spmv_f32_threads (the CPU baseline's second precision)."""
+    rp, ci = _i32(rp), _i32(ci)
+    n = len(rp) - 1
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(n, np.float64)
+    ms = lib().oracle_spmv_f64_threads(n, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(val, C.c_double),
+                                       _p(x, C.c_double), _p(y, C.c_double), threads, iters)
     return y, ms
 
 

@@ -103,3 +103,47 @@ double oracle_spmv_f32_threads(int32_t n_rows, const int32_t* rp, const int32_t*
     free(th); free(jobs);
     return (double)(b.tv_sec - a.tv_sec) * 1e3 + (double)(b.tv_nsec - a.tv_nsec) * 1e-6;
 }
+
+/* FP64 of the same: the CPU baseline's second precision */
+typedef struct {
+    const int32_t* rp; const int32_t* ci; const double* val; const double* x; double* y;
+    int32_t r0, r1, iters;
+} spmv_job64;
+
+static void* spmv_worker64(void* arg)
+{
+    spmv_job64* j = (spmv_job64*)arg;
+    for (int32_t it = 0; it < j->iters; ++it)
+        for (int32_t i = j->r0; i < j->r1; ++i) {
+            double s = 0.0;
+            for (int32_t k = j->rp[i]; k < j->rp[i + 1]; ++k) s += j->val[k] * j->x[j->ci[k]];
+            j->y[i] = s;
+        }
+    return NULL;
+}
+
+double oracle_spmv_f64_threads(int32_t n_rows, const int32_t* rp, const int32_t* ci,
+                               const double* val, const double* x, double* y,
+                               int32_t threads, int32_t iters)
+{
+    if (threads < 1) threads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+    spmv_job64* jobs = (spmv_job64*)malloc(sizeof(spmv_job64) * (size_t)threads);
+    const int64_t nnz = rp[n_rows];
+    int32_t r = 0;
+    for (int32_t t = 0; t < threads; ++t) {
+        const int64_t target = nnz * (t + 1) / threads;
+        int32_t e = r;
+        if (t == threads - 1) e = n_rows;
+        else while (e < n_rows && rp[e] < target) ++e;
+        jobs[t] = (spmv_job64){rp, ci, val, x, y, r, e, iters};
+        r = e;
+    }
+    struct timespec a, b;
+    clock_gettime(CLOCK_MONOTONIC, &a);
+    for (int32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, spmv_worker64, &jobs[t]);
+    for (int32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &b);
+    free(th); free(jobs);
+    return (double)(b.tv_sec - a.tv_sec) * 1e3 + (double)(b.tv_nsec - a.tv_nsec) * 1e-6;
+}

@@ -205,3 +205,20 @@ def test_degree_orders_properties(oracle):
         assert np.all(np.diff(k) <= 0)
         same = np.diff(k) == 0
         assert np.all(np.diff(order)[same] > 0)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_cpu_spmv_baselines_match_scipy(oracle, threads):
+    """The CPU baseline's threaded CSR SpMV (FP32 and FP64, nnz-balanced row
+    ranges) against scipy, including empty rows and more threads than rows
+    carry nonzeros."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(threads)
+    a = sp.random(300, 250, density=0.03, format="csr", random_state=threads)
+    a.data = rng.uniform(-1, 1, a.nnz)
+    x = rng.uniform(-1, 1, 250)
+    rp, ci = a.indptr.astype(np.int32), a.indices.astype(np.int32)
+    y64, _ = oracle.spmv_f64_threads(rp, ci, a.data, x, threads, 2)
+    assert np.allclose(y64, a @ x, rtol=1e-12, atol=1e-12)
+    y32, _ = oracle.spmv_f32_threads(rp, ci, a.data.astype(np.float32), x.astype(np.float32), threads, 2)
+    assert np.allclose(y32, a @ x, rtol=1e-4, atol=1e-5)
