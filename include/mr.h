@@ -190,7 +190,8 @@ int mr_pgorder_order(mr_mat m, int width, int window, uint32_t* gpu_order, uint3
  * threads as the reference's ThreadPool(128); 0 = all the process may use):
  * final_order = gpu_order refined, as mr_pgorder_order's final_order; *ms =
  * the phase's wall time without the host copies of the lists.  For timing the
- * phase apart from the GPU loop it overlaps in mr_reorder_pgorder. */
+ * phase apart from the GPU loop it overlaps in mr_reorder_pgorder.
+ * MR_ERR_NOT_PERMUTATION unless gpu_order is a permutation of the n vertices. */
 int mr_pgorder_cpu_phase(mr_mat m, int width, int window, const uint32_t* gpu_order, uint32_t* final_order,
                          int threads, double* ms);
 /* GRAPH::gorder5 (class.cu:2727-2873), the earlier pGOrder schedule: batches

@@ -627,6 +627,17 @@ int mr_pgorder_cpu_phase(mr_mat m, int width, int window, const uint32_t* gpu_or
     if (W < 32 || W > (1 << 22) || (W & (W - 1)))
         return set_error(MR_ERR_UNSUPPORTED, "pGOrder supports power-of-two widths 32..2^22 (got %d)", W);
     const int32_t n = (int32_t)n64;
+    {
+        // the placement must be a permutation of the vertices (it indexes
+        // the host lists below)
+        std::vector<uint8_t> seen((size_t)n, 0);
+        for (int32_t i = 0; i < n; ++i) {
+            const uint32_t v = gpu_order[i];
+            if (v >= (uint32_t)n || seen[v])
+                return set_error(MR_ERR_NOT_PERMUTATION, "gpu_order is not a permutation of the %d vertices", n);
+            seen[v] = 1;
+        }
+    }
     std::vector<int32_t> Ps, Pl, Os, Ol;
     const int32_t B = pg_schedule(n, W, Ps, Pl, Os, Ol);
     for (int32_t b = 0; b < B; ++b)

@@ -210,6 +210,15 @@ def test_pgorder_cpu_phase_alone(ctx, mr, oracle, case, threads):
     assert ms >= 0.0
     with pytest.raises(ValueError):
         mat.pgorder_cpu_phase(gpu_order[:-1], W, 5, threads)
+    if len(gpu_order) > 1:
+        bad = gpu_order.copy()
+        bad[0] = bad[1]
+        with pytest.raises(mr.MRError) as e:
+            mat.pgorder_cpu_phase(bad, W, 5, threads)
+        assert e.value.code == 4
+        bad[0] = len(gpu_order)
+        with pytest.raises(mr.MRError):
+            mat.pgorder_cpu_phase(bad, W, 5, threads)
     mat.close()
 
 
