@@ -620,13 +620,17 @@ def section_cpu_gorder_s22(a, extra):
     log(f"sequential CPU Gorder (w=5) on R-MAT s22, limit {a.cpu_gorder_timeout:.0f} s")
     t = time.perf_counter()
     try:
+        # the child's progress lines (MR_PROGRESS) go straight to this stderr
         r = subprocess.run([sys.executable, str(pathlib.Path(__file__).resolve()), "--child-gorder-cpu"],
-                           capture_output=True, text=True, timeout=a.cpu_gorder_timeout)
+                           stdout=subprocess.PIPE, text=True, timeout=a.cpu_gorder_timeout,
+                           env=dict(os.environ, MR_PROGRESS="1"))
         line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
-        out = json.loads(line[-1]) if r.returncode == 0 and line else {"error": r.stderr[-400:]}
+        out = json.loads(line[-1]) if r.returncode == 0 and line else {"error": f"exit status {r.returncode}"}
     except subprocess.TimeoutExpired:
         out = {"timed_out_after_s": a.cpu_gorder_timeout}
     out["child_wall_s"] = time.perf_counter() - t
+    out["cpu_model"] = cpu_model()
+    out["threads"] = 1
     out["pgorder_total_ms"] = extra.get("pgorder", {}).get("total_ms") if isinstance(extra.get("pgorder"), dict) else None
     log(f"  {out}")
     extra["cpu_gorder_s22"] = out

@@ -3,7 +3,10 @@
 // order (GRAPH::order, src/class.cu:3158-3365) and Reverse Cuthill-McKee
 // (rcm/src/rcm.cpp:1-154, rcm/src/queue.cpp:39-55).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "bucket_heap.hpp"
@@ -38,12 +41,23 @@ void gorder_cpu(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* 
                     if (sts[(size_t)ti[c]]) heap.modify(ti[c], d);
         }
     };
+    // with MR_PROGRESS set, a line on stderr at most every 20 s (a run at
+    // R-MAT s22 takes minutes)
+    const bool progress = std::getenv("MR_PROGRESS") != nullptr;
+    auto last = std::chrono::steady_clock::now();
     for (int32_t id = 0; id < n; ++id) {
         const int32_t node = heap.pop();
         sts[(size_t)node] = 0;
         order[id] = (uint32_t)node;
         apply(node, 1);
         if (id >= window) apply((int32_t)order[id - window], -1);
+        if (progress && (id & 0xffff) == 0) {
+            const auto now = std::chrono::steady_clock::now();
+            if (now - last >= std::chrono::seconds(20)) {
+                std::fprintf(stderr, "[gorder_cpu] %d of %d vertices placed\n", id, n);
+                last = now;
+            }
+        }
     }
 }
 
