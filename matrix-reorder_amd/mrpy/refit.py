@@ -23,7 +23,7 @@ It picks the edge set with the lower cross-validated error and writes the
 coefficients as a C++ header for ``mr_predict``.
 
 Usage, from the package directory (``matrix-reorder_amd/``):
-    python -m mrpy.refit collect --out ../gpurun_out/mi355x_rd_train.csv
+    python -m mrpy.refit collect --out /tmp/mi355x_rd_train.csv
     python -m mrpy.refit fit --csv model/mi355x_rd_train.csv \\
         --header csrc/model_mi355x.hpp --report model/mi355x_rd_fit.json
 """

@@ -19,8 +19,8 @@ beside the published origin / gorder1 times of the same id
 (suite/rmat_s24_published.json).
 
 Usage, from the package directory:
-    python -m mrpy.suite --ids 1-25 --out ../gpurun_out/suite/rmat_1_25.json
-    python -m mrpy.suite --summarize run1.json run2.json --out ../gpurun_out/suite/rmat_s24_mi355x.json
+    python -m mrpy.suite --ids 1-25 --out /tmp/suite/rmat_1_25.json
+    python -m mrpy.suite --summarize run1.json run2.json --out /tmp/suite/rmat_s24_mi355x.json
 """
 from __future__ import annotations
 
